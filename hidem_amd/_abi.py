@@ -1,0 +1,422 @@
+"""ctypes bindings to libhidem_amd.so (the C ABI in include/hidem/hdm_abi.h).
+
+Plumbing for tests and bench.py: numpy arrays on the host side, the library
+owns every device buffer.  The product's compute path is the HIP library; this
+module only marshals arguments and raises on non-zero status codes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhidem_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "hidem", "hdm_abi.h")
+
+HDM_OK = 0
+HDM_PREC_FP64 = 0
+HDM_PREC_MIXED = 1
+HDM_TYPE_FIXED = 1
+HDM_INTEG_ICEBLOCK = 0
+HDM_INTEG_GLACIER = 1
+HDM_REBUILD_HEURISTIC = 0
+HDM_REBUILD_EXACT = 1
+HDM_REBUILD_INTERVAL = 2
+TIMER_NAMES = ("force", "integrate", "break", "skin", "grid", "neigh", "links")
+
+_dp = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class HdmParticlesHost(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(f, _dp) for f in (
+        "xCord", "yCord", "zCord", "xTheta", "yTheta", "zTheta",
+        "xDiff", "yDiff", "zDiff", "uDiff", "vDiff", "wDiff", "radius",
+        "xForce", "yForce", "zForce", "uForce", "vForce", "wForce")] + [("type", _u8p)]
+
+
+class HdmLinksHost(C.Structure):
+    _fields_ = [("n", C.c_int64), ("localPartIndA", _i32p), ("localPartIndB", _i32p),
+                ("state", _u8p), ("initialLen", _dp), ("maxLoad", _dp), ("CT", _dp * 6)]
+
+
+class HdmForceParams(C.Structure):
+    _fields_ = [(f, C.c_double) for f in (
+        "invDt", "attractDistance", "attractScaleForce", "damping", "dampingRotation", "scaleE",
+        "E", "W2", "W4", "G")]
+
+
+class HdmIntegParams(C.Structure):
+    _fields_ = ([("variant", C.c_int32)] +
+                [(f, C.c_double) for f in ("rhoIce", "g", "lift", "waterLevel", "airDrag", "waterDrag",
+                                           "invDt", "invDt2", "dtSqr", "invDtSqr", "dt",
+                                           "bedDrag", "bedInteract", "bedStiffness", "frictionCap",
+                                           "bedZ", "bedXEnd", "deepZ", "frictionXEnd")] +
+                [("ext", C.c_double * 6), ("maxVel", C.c_double), ("maxRot", C.c_double),
+                 ("velocityLimit", C.c_int32), ("bound", C.c_int32 * 6),
+                 ("lower", C.c_double * 3), ("upper", C.c_double * 3)])
+
+
+class HdmBreakParams(C.Structure):
+    _fields_ = [("bendBreak", C.c_double), ("maxLoad", C.c_double)]
+
+
+class HdmGridInfo(C.Structure):
+    _fields_ = ([(f, C.c_double) for f in ("spacing", "xMin", "yMin", "zMin", "xMax", "yMax", "zMax")] +
+                [("X", C.c_int32), ("Y", C.c_int32), ("Z", C.c_int32), ("totalCells", C.c_int64),
+                 ("occupiedCells", C.c_int64), ("maxPerCell", C.c_int32)])
+
+
+class HdmRunParams(C.Structure):
+    _fields_ = [("force", HdmForceParams), ("integ", HdmIntegParams), ("brk", HdmBreakParams),
+                ("searchDistance", C.c_double), ("interactDist", C.c_double),
+                ("dt", C.c_double), ("breakTime", C.c_double),
+                ("policy", C.c_int32), ("regenInterval", C.c_int32)]
+
+
+class HdmRunStats(C.Structure):
+    _fields_ = [("timestep", C.c_int64), ("steps", C.c_int64), ("rebuilds", C.c_int64),
+                ("brokenTotal", C.c_int64), ("seconds", C.c_double)]
+
+
+class HdmFccSpec(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("a", C.c_double),
+                ("origin", C.c_double * 3), ("radius", C.c_double), ("jitter", C.c_double),
+                ("seed", C.c_uint64), ("clampXBelow", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libhidem_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run __graft_entry__.build() (make -C hidem_amd/csrc)")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    P = C.c_void_p
+    sig = {
+        "hdm_abi_version": ([], C.c_int),
+        "hdm_device_count": ([_i32p], C.c_int),
+        "hdm_ctx_create": ([C.c_int32, C.c_int32, C.POINTER(P)], C.c_int),
+        "hdm_ctx_destroy": ([P], C.c_int),
+        "hdm_last_error": ([P], C.c_char_p),
+        "hdm_sync": ([P], C.c_int),
+        "hdm_set_timing": ([P, C.c_int32], C.c_int),
+        "hdm_kernel_times": ([P, _dp, _i64p], C.c_int),
+        "hdm_reset_kernel_times": ([P], C.c_int),
+        "hdm_device_bytes": ([P, _i64p], C.c_int),
+        "hdm_particles_upload": ([P, C.POINTER(HdmParticlesHost)], C.c_int),
+        "hdm_particles_download": ([P, C.POINTER(HdmParticlesHost)], C.c_int),
+        "hdm_links_upload": ([P, C.POINTER(HdmLinksHost)], C.c_int),
+        "hdm_links_download": ([P, C.POINTER(HdmLinksHost)], C.c_int),
+        "hdm_grid_build": ([P, C.c_double], C.c_int),
+        "hdm_grid_info_get": ([P, C.POINTER(HdmGridInfo)], C.c_int),
+        "hdm_grid_download": ([P, _i64p, _i32p], C.c_int),
+        "hdm_neighbours_build": ([P, C.c_double], C.c_int),
+        "hdm_neighbours_download": ([P, _i32p, _i64p, _i32p, C.c_int64], C.c_int),
+        "hdm_neighbours_total": ([P, _i64p, _i32p], C.c_int),
+        "hdm_links_build": ([P], C.c_int),
+        "hdm_linklists_download": ([P, _i32p, _i64p, _i32p, _i32p, _i32p, C.c_int64], C.c_int),
+        "hdm_force": ([P, C.POINTER(HdmForceParams)], C.c_int),
+        "hdm_integrate": ([P, C.POINTER(HdmIntegParams)], C.c_int),
+        "hdm_break": ([P, C.POINTER(HdmBreakParams), _i64p], C.c_int),
+        "hdm_skin_heuristic": ([P, C.c_int32, _dp], C.c_int),
+        "hdm_skin_snapshot": ([P], C.c_int),
+        "hdm_skin_exact": ([P, _dp], C.c_int),
+        "hdm_run": ([P, C.POINTER(HdmRunParams), C.c_int64, _u8p, C.POINTER(HdmRunStats)], C.c_int),
+        "hdm_set_timestep": ([P, C.c_int64], C.c_int),
+        "hdm_count_broken": ([P, _i64p], C.c_int),
+        "hdm_gen_fcc_count": ([C.POINTER(HdmFccSpec), _i64p], C.c_int),
+        "hdm_gen_fcc_fill": ([C.POINTER(HdmFccSpec), C.POINTER(HdmParticlesHost)], C.c_int),
+        "hdm_gen_links_count": ([C.POINTER(HdmParticlesHost), C.c_double, _i64p], C.c_int),
+        "hdm_gen_links_fill": ([C.POINTER(HdmParticlesHost), C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                C.POINTER(HdmLinksHost)], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+
+
+def declared_symbols(header: str = HEADER_PATH):
+    """Function names declared in hdm_abi.h."""
+    import re
+    text = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void)\s+\**(hdm_\w+)\s*\(", text, re.M)))
+
+
+def _ptr(a, ct):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+PARTICLE_FIELDS = ("xCord", "yCord", "zCord", "xTheta", "yTheta", "zTheta",
+                   "xDiff", "yDiff", "zDiff", "uDiff", "vDiff", "wDiff", "radius",
+                   "xForce", "yForce", "zForce", "uForce", "vForce", "wForce")
+
+
+@dataclass
+class Particles:
+    """Host SoA particle state (HiDEMParticleData fields on the hot path)."""
+    n: int
+    arrays: dict = field(default_factory=dict)
+    type: Optional[np.ndarray] = None
+
+    @classmethod
+    def zeros(cls, n):
+        p = cls(n=n, arrays={f: np.zeros(n) for f in PARTICLE_FIELDS}, type=np.zeros(n, np.uint8))
+        return p
+
+    def __getitem__(self, k):
+        return self.arrays[k]
+
+    def copy(self):
+        return Particles(self.n, {k: v.copy() for k, v in self.arrays.items()},
+                         None if self.type is None else self.type.copy())
+
+    def struct(self):
+        s = HdmParticlesHost()
+        s.n = self.n
+        for f in PARTICLE_FIELDS:
+            a = self.arrays.get(f)
+            if a is not None:
+                assert a.dtype == np.float64 and a.flags.c_contiguous and a.shape == (self.n,)
+            setattr(s, f, _ptr(a, C.c_double))
+        s.type = _ptr(self.type, C.c_uint8)
+        return s
+
+
+@dataclass
+class Links:
+    """Host SoA link state (HiDEMLinkData); CT arrays in checkpoint order (2l+end)."""
+    n: int
+    a: np.ndarray
+    b: np.ndarray
+    state: np.ndarray
+    initialLen: np.ndarray
+    maxLoad: Optional[np.ndarray]
+    ct: list
+
+    @classmethod
+    def zeros(cls, n, with_max_load=True):
+        return cls(n, np.zeros(n, np.int32), np.zeros(n, np.int32), np.ones(n, np.uint8), np.zeros(n),
+                   np.zeros(n) if with_max_load else None, [np.zeros(2 * n) for _ in range(6)])
+
+    def copy(self):
+        return Links(self.n, self.a.copy(), self.b.copy(), self.state.copy(), self.initialLen.copy(),
+                     None if self.maxLoad is None else self.maxLoad.copy(), [c.copy() for c in self.ct])
+
+    def struct(self):
+        s = HdmLinksHost()
+        s.n = self.n
+        s.localPartIndA = _ptr(self.a, C.c_int32)
+        s.localPartIndB = _ptr(self.b, C.c_int32)
+        s.state = _ptr(self.state, C.c_uint8)
+        s.initialLen = _ptr(self.initialLen, C.c_double)
+        s.maxLoad = _ptr(self.maxLoad, C.c_double)
+        for k in range(6):
+            s.CT[k] = _ptr(self.ct[k], C.c_double)
+        return s
+
+
+class HdmError(RuntimeError):
+    pass
+
+
+def check(rc, ctx=None, what=""):
+    if rc != HDM_OK:
+        msg = ""
+        if ctx is not None:
+            m = lib().hdm_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise HdmError(f"{what} failed with status {rc}: {msg}")
+
+
+# ---- generator -----------------------------------------------------------------
+def gen_fcc(nx, ny, nz, a, radius, jitter, seed, origin=(0.0, 0.0, 0.0), clamp_x_below=-np.inf) -> Particles:
+    spec = HdmFccSpec(nx, ny, nz, a, (C.c_double * 3)(*origin), radius, jitter, seed, clamp_x_below)
+    n = C.c_int64()
+    check(lib().hdm_gen_fcc_count(C.byref(spec), C.byref(n)), what="hdm_gen_fcc_count")
+    p = Particles.zeros(n.value)
+    s = p.struct()
+    check(lib().hdm_gen_fcc_fill(C.byref(spec), C.byref(s)), what="hdm_gen_fcc_fill")
+    return p
+
+
+def gen_links(p: Particles, link_gen_scale, threshold, spread, seed) -> Links:
+    ps = p.struct()
+    n = C.c_int64()
+    check(lib().hdm_gen_links_count(C.byref(ps), link_gen_scale, C.byref(n)), what="hdm_gen_links_count")
+    l = Links.zeros(n.value)
+    ls = l.struct()
+    check(lib().hdm_gen_links_fill(C.byref(ps), link_gen_scale, threshold, spread, seed, C.byref(ls)),
+          what="hdm_gen_links_fill")
+    return l
+
+
+# ---- context -------------------------------------------------------------------
+class Context:
+    """One hdm_ctx: device mirrors + accelerator structures on one HIP stream."""
+
+    def __init__(self, precision=HDM_PREC_FP64, device=0):
+        self._h = C.c_void_p()
+        rc = lib().hdm_ctx_create(precision, device, C.byref(self._h))
+        if rc != HDM_OK:
+            raise HdmError(f"hdm_ctx_create failed with status {rc} (no gfx950 device?)")
+        self.precision = precision
+        self.n = 0
+        self.L = 0
+
+    def close(self):
+        if self._h:
+            lib().hdm_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc, what):
+        check(rc, self._h, what)
+
+    def upload(self, p: Particles, l: Optional[Links] = None):
+        s = p.struct()
+        self._c(lib().hdm_particles_upload(self._h, C.byref(s)), "hdm_particles_upload")
+        self.n = p.n
+        if l is not None:
+            ls = l.struct()
+            self._c(lib().hdm_links_upload(self._h, C.byref(ls)), "hdm_links_upload")
+            self.L = l.n
+
+    def download_particles(self) -> Particles:
+        p = Particles.zeros(self.n)
+        s = p.struct()
+        self._c(lib().hdm_particles_download(self._h, C.byref(s)), "hdm_particles_download")
+        return p
+
+    def download_links(self, with_max_load=True) -> Links:
+        l = Links.zeros(self.L, with_max_load)
+        s = l.struct()
+        self._c(lib().hdm_links_download(self._h, C.byref(s)), "hdm_links_download")
+        return l
+
+    def grid_build(self, search_distance):
+        self._c(lib().hdm_grid_build(self._h, search_distance), "hdm_grid_build")
+
+    def grid_info(self) -> HdmGridInfo:
+        g = HdmGridInfo()
+        self._c(lib().hdm_grid_info_get(self._h, C.byref(g)), "hdm_grid_info_get")
+        return g
+
+    def grid_download(self):
+        col = np.zeros(self.n, np.int64)
+        order = np.zeros(self.n, np.int32)
+        self._c(lib().hdm_grid_download(self._h, _ptr(col, C.c_int64), _ptr(order, C.c_int32)), "hdm_grid_download")
+        return col, order
+
+    def neighbours_build(self, interact_dist):
+        self._c(lib().hdm_neighbours_build(self._h, interact_dist), "hdm_neighbours_build")
+
+    def neighbours_download(self):
+        tot = C.c_int64()
+        mx = C.c_int32()
+        self._c(lib().hdm_neighbours_total(self._h, C.byref(tot), C.byref(mx)), "hdm_neighbours_total")
+        counts = np.zeros(self.n, np.int32)
+        starts = np.zeros(self.n + 1, np.int64)
+        lst = np.zeros(max(tot.value, 1), np.int32)
+        self._c(lib().hdm_neighbours_download(self._h, _ptr(counts, C.c_int32), _ptr(starts, C.c_int64),
+                                              _ptr(lst, C.c_int32), lst.size), "hdm_neighbours_download")
+        return counts, starts, lst[:tot.value]
+
+    def links_build(self):
+        self._c(lib().hdm_links_build(self._h), "hdm_links_build")
+
+    def linklists_download(self):
+        counts = np.zeros(self.n, np.int32)
+        starts = np.zeros(self.n + 1, np.int64)
+        cap = 2 * self.L
+        lid = np.zeros(max(cap, 1), np.int32)
+        oth = np.zeros(max(cap, 1), np.int32)
+        end = np.zeros(max(cap, 1), np.int32)
+        self._c(lib().hdm_linklists_download(self._h, _ptr(counts, C.c_int32), _ptr(starts, C.c_int64),
+                                             _ptr(lid, C.c_int32), _ptr(oth, C.c_int32), _ptr(end, C.c_int32),
+                                             max(cap, 1)), "hdm_linklists_download")
+        return counts, starts, lid[:cap], oth[:cap], end[:cap]
+
+    def force(self, fp: HdmForceParams):
+        self._c(lib().hdm_force(self._h, C.byref(fp)), "hdm_force")
+
+    def integrate(self, ip: HdmIntegParams):
+        self._c(lib().hdm_integrate(self._h, C.byref(ip)), "hdm_integrate")
+
+    def brk(self, bp: HdmBreakParams):
+        nb = C.c_int64()
+        self._c(lib().hdm_break(self._h, C.byref(bp), C.byref(nb)), "hdm_break")
+        return nb.value
+
+    def skin_heuristic(self, step_size=10):
+        d = C.c_double()
+        self._c(lib().hdm_skin_heuristic(self._h, step_size, C.byref(d)), "hdm_skin_heuristic")
+        return d.value
+
+    def skin_snapshot(self):
+        self._c(lib().hdm_skin_snapshot(self._h), "hdm_skin_snapshot")
+
+    def skin_exact(self):
+        d = C.c_double()
+        self._c(lib().hdm_skin_exact(self._h, C.byref(d)), "hdm_skin_exact")
+        return d.value
+
+    def run(self, rp: HdmRunParams, n_steps, log_rebuilds=False):
+        st = HdmRunStats()
+        log = np.zeros(max(n_steps, 1), np.uint8) if log_rebuilds else None
+        self._c(lib().hdm_run(self._h, C.byref(rp), n_steps, _ptr(log, C.c_uint8), C.byref(st)), "hdm_run")
+        return (st, log[:n_steps]) if log_rebuilds else st
+
+    def set_timestep(self, ts):
+        self._c(lib().hdm_set_timestep(self._h, ts), "hdm_set_timestep")
+
+    def count_broken(self):
+        b = C.c_int64()
+        self._c(lib().hdm_count_broken(self._h, C.byref(b)), "hdm_count_broken")
+        return b.value
+
+    def sync(self):
+        self._c(lib().hdm_sync(self._h), "hdm_sync")
+
+    def set_timing(self, on=True):
+        self._c(lib().hdm_set_timing(self._h, 1 if on else 0), "hdm_set_timing")
+
+    def reset_kernel_times(self):
+        self._c(lib().hdm_reset_kernel_times(self._h), "hdm_reset_kernel_times")
+
+    def kernel_times(self):
+        ms = (C.c_double * 7)()
+        cnt = (C.c_int64 * 7)()
+        self._c(lib().hdm_kernel_times(self._h, ms, cnt), "hdm_kernel_times")
+        return {TIMER_NAMES[k]: (ms[k], cnt[k]) for k in range(7)}
+
+    def device_bytes(self):
+        b = C.c_int64()
+        self._c(lib().hdm_device_bytes(self._h, C.byref(b)), "hdm_device_bytes")
+        return b.value

@@ -1,0 +1,250 @@
+// Seeded FCC block and link generator (setup, host side).
+//
+// The reference generators are non-deterministic (std::random_device seeding,
+// FCCParticleGenerator.h:172-173; OpenMP-critical append order, :200-206;
+// atomic link order, LinkGenerator.h:101), so a run cannot be reproduced from
+// a seed.  This generator is deterministic and order-canonical:
+//   * FCC unit cell basis of FCCParticleGenerator::createUnitCell
+//     (FCCParticleGenerator.h:47-50): (0,0,0), (0,a/2,a/2), (a/2,0,a/2),
+//     (a/2,a/2,0);
+//   * particle index = global key = ((k*ny + j)*nx + i)*4 + basis;
+//   * jitter: each coordinate + jitter*(2u-1), u from a counter-based
+//     SplitMix64 hash of (seed, 3*index + component);
+//   * links: pairs with dist <= (r_i + r_j)*linkGenScale, dist computed as in
+//     LinkGenerator::genNewLinks (LinkGenerator.h:83-89); end A is the smaller
+//     key (:64), links sorted by (A, B); initialLen = dist (:99);
+//   * per-link fracture threshold threshold*(1 + spread*(2u-1)), u hashed
+//     from (seed, A, B) so it does not depend on link order.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../include/hidem/hdm_abi.h"
+
+namespace {
+
+inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+inline double uniform01(uint64_t seed, uint64_t counter) {
+    uint64_t z = splitmix64(seed ^ splitmix64(counter));
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+int nthreads() {
+    unsigned h = std::thread::hardware_concurrency();
+    if (h == 0) h = 1;
+    if (h > 16) h = 16;
+    return (int)h;
+}
+
+template <class F>
+void parallel_for(int64_t n, F f) {
+    int T = nthreads();
+    if (n < 4096) T = 1;
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++) {
+        int64_t b = n * t / T, e = n * (t + 1) / T;
+        th.emplace_back([=]() { f(t, b, e); });
+    }
+    for (auto &x : th) x.join();
+}
+
+struct CellList {
+    double xMin, yMin, zMin, s;
+    int64_t X, Y, Z;
+    std::vector<int64_t> start;  // X*Y*Z + 1
+    std::vector<int32_t> items;
+
+    void build(const hdm_particles_host *p, double cell) {
+        int64_t n = p->n;
+        xMin = yMin = zMin = INFINITY;
+        double xMax = -INFINITY, yMax = -INFINITY, zMax = -INFINITY;
+        for (int64_t i = 0; i < n; i++) {
+            xMin = std::min(xMin, p->xCord[i]); xMax = std::max(xMax, p->xCord[i]);
+            yMin = std::min(yMin, p->yCord[i]); yMax = std::max(yMax, p->yCord[i]);
+            zMin = std::min(zMin, p->zCord[i]); zMax = std::max(zMax, p->zCord[i]);
+        }
+        s = cell;
+        X = 1 + (int64_t)std::floor((xMax - xMin) / s);
+        Y = 1 + (int64_t)std::floor((yMax - yMin) / s);
+        Z = 1 + (int64_t)std::floor((zMax - zMin) / s);
+        start.assign((size_t)(X * Y * Z + 1), 0);
+        std::vector<int64_t> cellOf((size_t)n);
+        for (int64_t i = 0; i < n; i++) {
+            int64_t c = cell_of(p->xCord[i], p->yCord[i], p->zCord[i]);
+            cellOf[(size_t)i] = c;
+            start[(size_t)c + 1]++;
+        }
+        for (size_t c = 1; c < start.size(); c++) start[c] += start[c - 1];
+        items.assign((size_t)n, 0);
+        std::vector<int64_t> cur(start.begin(), start.end() - 1);
+        for (int64_t i = 0; i < n; i++) items[(size_t)cur[(size_t)cellOf[(size_t)i]]++] = (int32_t)i;
+    }
+    int64_t clampi(int64_t v, int64_t hi) const { return v < 0 ? 0 : (v >= hi ? hi - 1 : v); }
+    int64_t cell_of(double x, double y, double z) const {
+        int64_t ix = clampi((int64_t)std::floor((x - xMin) / s), X);
+        int64_t iy = clampi((int64_t)std::floor((y - yMin) / s), Y);
+        int64_t iz = clampi((int64_t)std::floor((z - zMin) / s), Z);
+        return ix + X * (iy + Y * iz);
+    }
+};
+
+// Visits candidate partners q > p within the link distance, ascending q.
+template <class F>
+void for_link_partners(const hdm_particles_host *p, const CellList &cl, double linkGenScale, int64_t i,
+                       std::vector<int32_t> &scratch, F f) {
+    double x = p->xCord[i], y = p->yCord[i], z = p->zCord[i];
+    int64_t ix = cl.clampi((int64_t)std::floor((x - cl.xMin) / cl.s), cl.X);
+    int64_t iy = cl.clampi((int64_t)std::floor((y - cl.yMin) / cl.s), cl.Y);
+    int64_t iz = cl.clampi((int64_t)std::floor((z - cl.zMin) / cl.s), cl.Z);
+    scratch.clear();
+    for (int64_t dz = -1; dz <= 1; dz++)
+        for (int64_t dy = -1; dy <= 1; dy++)
+            for (int64_t dx = -1; dx <= 1; dx++) {
+                int64_t jx = ix + dx, jy = iy + dy, jz = iz + dz;
+                if (jx < 0 || jy < 0 || jz < 0 || jx >= cl.X || jy >= cl.Y || jz >= cl.Z) continue;
+                int64_t c = jx + cl.X * (jy + cl.Y * jz);
+                for (int64_t k = cl.start[(size_t)c]; k < cl.start[(size_t)c + 1]; k++) {
+                    int32_t q = cl.items[(size_t)k];
+                    if (q <= i) continue;
+                    double xDist = p->xCord[q] - p->xCord[i];
+                    double yDist = p->yCord[q] - p->yCord[i];
+                    double zDist = p->zCord[q] - p->zCord[i];
+                    double dist = std::sqrt((xDist * xDist) + (yDist * yDist) + (zDist * zDist));
+                    double radiusSum = p->radius[q] + p->radius[i];
+                    if (dist > radiusSum * linkGenScale) continue;
+                    scratch.push_back(q);
+                }
+            }
+    std::sort(scratch.begin(), scratch.end());
+    for (int32_t q : scratch) f(q);
+}
+
+double max_radius(const hdm_particles_host *p) {
+    double r = 0;
+    for (int64_t i = 0; i < p->n; i++) r = std::max(r, p->radius[i]);
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hdm_gen_fcc_count(const hdm_fcc_spec *spec, int64_t *nParticles) {
+    if (!spec || !nParticles || spec->nx <= 0 || spec->ny <= 0 || spec->nz <= 0) return HDM_E_INVALID;
+    *nParticles = (int64_t)spec->nx * spec->ny * spec->nz * 4;
+    return HDM_OK;
+}
+
+int hdm_gen_fcc_fill(const hdm_fcc_spec *spec, hdm_particles_host *out) {
+    int64_t n = 0;
+    int r = hdm_gen_fcc_count(spec, &n);
+    if (r != HDM_OK) return r;
+    if (!out || out->n != n || !out->xCord || !out->yCord || !out->zCord || !out->radius) return HDM_E_INVALID;
+    const double a = spec->a, h = spec->a / 2.0;
+    const double basis[4][3] = {{0, 0, 0}, {0, h, h}, {h, 0, h}, {h, h, 0}};
+    const int64_t nx = spec->nx, ny = spec->ny;
+    parallel_for(n, [&](int, int64_t b, int64_t e) {
+        for (int64_t idx = b; idx < e; idx++) {
+            int64_t cell = idx >> 2;
+            int bi = (int)(idx & 3);
+            int64_t i = cell % nx, j = (cell / nx) % ny, k = cell / (nx * ny);
+            double pos[3] = {spec->origin[0] + (double)i * a + basis[bi][0],
+                             spec->origin[1] + (double)j * a + basis[bi][1],
+                             spec->origin[2] + (double)k * a + basis[bi][2]};
+            for (int c = 0; c < 3; c++) {
+                double u = uniform01(spec->seed, (uint64_t)idx * 3u + (uint64_t)c);
+                pos[c] += spec->jitter * (2.0 * u - 1.0);
+            }
+            out->xCord[idx] = pos[0];
+            out->yCord[idx] = pos[1];
+            out->zCord[idx] = pos[2];
+            out->radius[idx] = spec->radius;
+            double *zeros[] = {out->xTheta, out->yTheta, out->zTheta, out->xDiff, out->yDiff, out->zDiff,
+                               out->uDiff, out->vDiff, out->wDiff, out->xForce, out->yForce, out->zForce,
+                               out->uForce, out->vForce, out->wForce};
+            for (double *z : zeros)
+                if (z) z[idx] = 0.0;
+            if (out->type) out->type[idx] = (pos[0] < spec->clampXBelow) ? (uint8_t)HDM_TYPE_FIXED : (uint8_t)0;
+        }
+    });
+    return HDM_OK;
+}
+
+int hdm_gen_links_count(const hdm_particles_host *p, double linkGenScale, int64_t *nLinks) {
+    if (!p || !nLinks || p->n < 0 || !(linkGenScale > 0)) return HDM_E_INVALID;
+    if (p->n == 0) {
+        *nLinks = 0;
+        return HDM_OK;
+    }
+    CellList cl;
+    cl.build(p, 2.0 * max_radius(p) * linkGenScale);
+    int T = nthreads();
+    std::vector<int64_t> counts((size_t)T, 0);
+    parallel_for(p->n, [&](int t, int64_t b, int64_t e) {
+        std::vector<int32_t> scratch;
+        int64_t cnt = 0;
+        for (int64_t i = b; i < e; i++) for_link_partners(p, cl, linkGenScale, i, scratch, [&](int32_t) { cnt++; });
+        counts[(size_t)t] = cnt;
+    });
+    int64_t tot = 0;
+    for (auto c : counts) tot += c;
+    *nLinks = tot;
+    return HDM_OK;
+}
+
+int hdm_gen_links_fill(const hdm_particles_host *p, double linkGenScale, double threshold, double spread,
+                       uint64_t seed, hdm_links_host *out) {
+    if (!p || !out || !(linkGenScale > 0)) return HDM_E_INVALID;
+    if (out->n > 0 && (!out->localPartIndA || !out->localPartIndB || !out->state || !out->initialLen))
+        return HDM_E_INVALID;
+    if (p->n == 0) return out->n == 0 ? HDM_OK : HDM_E_INVALID;
+    CellList cl;
+    cl.build(p, 2.0 * max_radius(p) * linkGenScale);
+    int T = nthreads();
+    if (p->n < 4096) T = 1;
+    std::vector<std::vector<int32_t>> pa((size_t)T), pb((size_t)T);
+    std::vector<std::vector<double>> len((size_t)T);
+    parallel_for(p->n, [&](int t, int64_t b, int64_t e) {
+        std::vector<int32_t> scratch;
+        for (int64_t i = b; i < e; i++)
+            for_link_partners(p, cl, linkGenScale, i, scratch, [&](int32_t q) {
+                double xDist = p->xCord[q] - p->xCord[i];
+                double yDist = p->yCord[q] - p->yCord[i];
+                double zDist = p->zCord[q] - p->zCord[i];
+                pa[(size_t)t].push_back((int32_t)i);
+                pb[(size_t)t].push_back(q);
+                len[(size_t)t].push_back(std::sqrt((xDist * xDist) + (yDist * yDist) + (zDist * zDist)));
+            });
+    });
+    int64_t tot = 0;
+    for (auto &v : pa) tot += (int64_t)v.size();
+    if (tot != out->n) return HDM_E_INVALID;
+    int64_t pos = 0;
+    for (int t = 0; t < T; t++) {
+        for (size_t k = 0; k < pa[(size_t)t].size(); k++, pos++) {
+            int32_t A = pa[(size_t)t][k], B = pb[(size_t)t][k];
+            out->localPartIndA[pos] = A;
+            out->localPartIndB[pos] = B;
+            out->state[pos] = 1;
+            out->initialLen[pos] = len[(size_t)t][k];
+            if (out->maxLoad) {
+                double u = uniform01(seed ^ 0x5851F42D4C957F2Dull, ((uint64_t)(uint32_t)A << 32) | (uint32_t)B);
+                out->maxLoad[pos] = threshold * (1.0 + spread * (2.0 * u - 1.0));
+            }
+        }
+    }
+    for (int c = 0; c < 6; c++)
+        if (out->CT[c]) std::memset(out->CT[c], 0, sizeof(double) * 2 * (size_t)out->n);
+    return HDM_OK;
+}
+
+}  // extern "C"

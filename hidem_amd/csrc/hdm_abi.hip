@@ -1,0 +1,690 @@
+// C ABI implementation: host-side orchestration of the gfx950 kernels.
+// Mirrors the accelerator calls HiDEMLib makes (HiDEMLib.h:256-382, 533-628)
+// on device data owned by the context; see include/hidem/hdm_abi.h.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "hdm_context.h"
+
+using hdm::Context;
+
+struct hdm_ctx {
+    Context c;
+};
+
+namespace {
+
+int fail(hdm_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->c.err = msg;
+    return code;
+}
+
+int hip_fail(hdm_ctx *ctx, hipError_t e, const char *where) {
+    return fail(ctx, e == hipErrorOutOfMemory ? HDM_E_NOMEM : HDM_E_HIP,
+                std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_CHECK(ctx, expr)                                      \
+    do {                                                          \
+        hipError_t _e = (expr);                                   \
+        if (_e != hipSuccess) return hip_fail((ctx), _e, #expr);  \
+    } while (0)
+
+// Event-pair timing of one kernel class on the context stream.
+struct Timer {
+    Context &c;
+    int cls;
+    hipEvent_t start = nullptr, stop = nullptr;
+    Timer(Context &cc, int k) : c(cc), cls(k) {
+        if (!c.timing) return;
+        start = take();
+        stop = take();
+        if (start && stop) (void)hipEventRecord(start, c.stream);
+    }
+    ~Timer() {
+        if (!c.timing || !start || !stop) return;
+        (void)hipEventRecord(stop, c.stream);
+        c.pending.push_back(hdm::TimedLaunch{cls, start, stop});
+    }
+    hipEvent_t take() {
+        if (!c.eventPool.empty()) {
+            hipEvent_t e = c.eventPool.back();
+            c.eventPool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+};
+
+int collect_timing(hdm_ctx *ctx) {
+    Context &c = ctx->c;
+    if (c.pending.empty()) return HDM_OK;
+    HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
+    for (auto &t : c.pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+            c.ms[t.cls] += ms;
+            c.launches[t.cls] += 1;
+        }
+        c.eventPool.push_back(t.start);
+        c.eventPool.push_back(t.stop);
+    }
+    c.pending.clear();
+    return HDM_OK;
+}
+
+template <class T>
+hipError_t upload_as(hdm::DevBuf &dst, const double *src, int64_t n, hipStream_t s) {
+    hipError_t e = dst.ensure(sizeof(T) * (size_t)(n > 0 ? n : 1));
+    if (e != hipSuccess || n == 0) return e;
+    if (src == nullptr) return hipMemsetAsync(dst.ptr, 0, sizeof(T) * (size_t)n, s);
+    if (sizeof(T) == sizeof(double)) return hipMemcpyAsync(dst.ptr, src, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s);
+    std::vector<T> tmp((size_t)n);
+    for (int64_t i = 0; i < n; i++) tmp[(size_t)i] = (T)src[i];
+    e = hipMemcpyAsync(dst.ptr, tmp.data(), sizeof(T) * (size_t)n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
+template <class T>
+hipError_t download_as(double *dst, const hdm::DevBuf &src, int64_t n, hipStream_t s) {
+    if (dst == nullptr || n == 0) return hipSuccess;
+    if (sizeof(T) == sizeof(double)) {
+        hipError_t e = hipMemcpyAsync(dst, src.ptr, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s);
+        return e == hipSuccess ? hipStreamSynchronize(s) : e;
+    }
+    std::vector<T> tmp((size_t)n);
+    hipError_t e = hipMemcpyAsync(tmp.data(), src.ptr, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    for (int64_t i = 0; i < n; i++) dst[i] = (double)tmp[(size_t)i];
+    return hipSuccess;
+}
+
+hipError_t d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
+bool valid(const hdm_ctx *ctx) { return ctx != nullptr; }
+
+}  // namespace
+
+namespace hdm {
+int xcd_grid(int64_t nBlocks) { return (int)nBlocks; }
+}  // namespace hdm
+
+extern "C" {
+
+int hdm_abi_version(void) { return HDM_ABI_VERSION; }
+
+int hdm_device_count(int32_t *count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (count) *count = (e == hipSuccess) ? n : 0;
+    return e == hipSuccess ? HDM_OK : HDM_E_NODEVICE;
+}
+
+int hdm_ctx_create(int32_t precision, int32_t device, hdm_ctx **out) {
+    if (!out) return HDM_E_INVALID;
+    *out = nullptr;
+    if (precision != HDM_PREC_FP64 && precision != HDM_PREC_MIXED) return HDM_E_INVALID;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return HDM_E_NODEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return HDM_E_NODEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return HDM_E_NODEVICE;
+    if (hipSetDevice(device) != hipSuccess) return HDM_E_NODEVICE;
+    hdm_ctx *ctx = new (std::nothrow) hdm_ctx();
+    if (!ctx) return HDM_E_NOMEM;
+    Context &c = ctx->c;
+    c.prec = precision;
+    c.device = device;
+    c.numCU = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return HDM_E_HIP;
+    }
+    if (c.red.ensure(sizeof(uint64_t) * 64) != hipSuccess) {
+        (void)hipStreamDestroy(c.stream);
+        delete ctx;
+        return HDM_E_NOMEM;
+    }
+    *out = ctx;
+    return HDM_OK;
+}
+
+int hdm_ctx_destroy(hdm_ctx *ctx) {
+    if (!ctx) return HDM_OK;
+    Context &c = ctx->c;
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    for (auto &t : c.pending) {
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    for (auto e : c.eventPool) (void)hipEventDestroy(e);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    delete ctx;
+    return HDM_OK;
+}
+
+const char *hdm_last_error(const hdm_ctx *ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int hdm_sync(hdm_ctx *ctx) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    HIP_CHECK(ctx, hipStreamSynchronize(ctx->c.stream));
+    return HDM_OK;
+}
+
+int hdm_set_timing(hdm_ctx *ctx, int32_t enabled) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    int r = collect_timing(ctx);
+    ctx->c.timing = enabled != 0;
+    return r;
+}
+
+int hdm_kernel_times(hdm_ctx *ctx, double ms[HDM_T_COUNT], int64_t launches[HDM_T_COUNT]) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    int r = collect_timing(ctx);
+    if (r != HDM_OK) return r;
+    for (int k = 0; k < HDM_T_COUNT; k++) {
+        if (ms) ms[k] = ctx->c.ms[k];
+        if (launches) launches[k] = ctx->c.launches[k];
+    }
+    return HDM_OK;
+}
+
+int hdm_reset_kernel_times(hdm_ctx *ctx) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    int r = collect_timing(ctx);
+    for (int k = 0; k < HDM_T_COUNT; k++) {
+        ctx->c.ms[k] = 0;
+        ctx->c.launches[k] = 0;
+    }
+    return r;
+}
+
+int hdm_device_bytes(hdm_ctx *ctx, int64_t *bytes) {
+    if (!valid(ctx) || !bytes) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    size_t t = 0;
+    const hdm::DevBuf *bufs[] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz, &c.radius, &c.type, &c.la, &c.lb,
+                                 &c.state, &c.initialLen, &c.maxLoad, &c.linkCount, &c.linkCursor, &c.slotLink,
+                                 &c.slotOther, &c.sidePos, &c.nbrCount, &c.nbr, &c.colour, &c.order,
+                                 &c.cellStart, &c.cellCursor, &c.scanPartials, &c.red, &c.x0, &c.y0, &c.z0};
+    for (auto *b : bufs) t += b->bytes;
+    for (int k = 0; k < 6; k++) t += c.diff[k].bytes + c.force[k].bytes + c.ct[k].bytes + c.ctStage[k].bytes;
+    *bytes = (int64_t)t;
+    return HDM_OK;
+}
+
+// ---------------------------------------------------------------------------
+int hdm_particles_upload(hdm_ctx *ctx, const hdm_particles_host *p) {
+    if (!valid(ctx) || !p || p->n < 0) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (p->n > 0 && (!p->xCord || !p->yCord || !p->zCord || !p->radius || !p->xTheta || !p->yTheta ||
+                     !p->zTheta || !p->xDiff || !p->yDiff || !p->zDiff || !p->uDiff || !p->vDiff || !p->wDiff))
+        return fail(ctx, HDM_E_INVALID, "hdm_particles_upload: missing required field");
+    if (p->n >= (int64_t)INT32_MAX) return fail(ctx, HDM_E_CAPACITY, "more than int32 particles");
+    if (c.L > 0 && p->n != c.n) return fail(ctx, HDM_E_STATE, "particle count changed while links are bound");
+    (void)hipSetDevice(c.device);
+    c.n = p->n;
+    hipStream_t s = c.stream;
+    const double *dsrc[6] = {p->xDiff, p->yDiff, p->zDiff, p->uDiff, p->vDiff, p->wDiff};
+    const double *fsrc[6] = {p->xForce, p->yForce, p->zForce, p->uForce, p->vForce, p->wForce};
+    HIP_CHECK(ctx, upload_as<double>(c.x, p->xCord, c.n, s));
+    HIP_CHECK(ctx, upload_as<double>(c.y, p->yCord, c.n, s));
+    HIP_CHECK(ctx, upload_as<double>(c.z, p->zCord, c.n, s));
+    HIP_CHECK(ctx, upload_as<double>(c.tx, p->xTheta, c.n, s));
+    HIP_CHECK(ctx, upload_as<double>(c.ty, p->yTheta, c.n, s));
+    HIP_CHECK(ctx, upload_as<double>(c.tz, p->zTheta, c.n, s));
+    HIP_CHECK(ctx, upload_as<double>(c.radius, p->radius, c.n, s));
+    for (int k = 0; k < 6; k++) HIP_CHECK(ctx, upload_as<double>(c.diff[k], dsrc[k], c.n, s));
+    for (int k = 0; k < 6; k++) {
+        if (c.prec == HDM_PREC_FP64) HIP_CHECK(ctx, upload_as<double>(c.force[k], fsrc[k], c.n, s));
+        else HIP_CHECK(ctx, upload_as<float>(c.force[k], fsrc[k], c.n, s));
+    }
+    HIP_CHECK(ctx, c.type.ensure((size_t)(c.n > 0 ? c.n : 1)));
+    if (c.n > 0) {
+        if (p->type) HIP_CHECK(ctx, hipMemcpyAsync(c.type.ptr, p->type, (size_t)c.n, hipMemcpyHostToDevice, s));
+        else HIP_CHECK(ctx, hipMemsetAsync(c.type.ptr, 0, (size_t)c.n, s));
+    }
+    for (hdm::DevBuf *b : {&c.x0, &c.y0, &c.z0}) HIP_CHECK(ctx, b->ensure(sizeof(double) * (size_t)(c.n > 0 ? c.n : 1)));
+    HIP_CHECK(ctx, hipStreamSynchronize(s));
+    c.grid.built = false;
+    c.neighBuilt = false;
+    c.snapshotValid = false;
+    return HDM_OK;
+}
+
+int hdm_particles_download(hdm_ctx *ctx, hdm_particles_host *p) {
+    if (!valid(ctx) || !p) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (p->n != c.n) return fail(ctx, HDM_E_INVALID, "hdm_particles_download: n mismatch");
+    hipStream_t s = c.stream;
+    HIP_CHECK(ctx, download_as<double>(p->xCord, c.x, c.n, s));
+    HIP_CHECK(ctx, download_as<double>(p->yCord, c.y, c.n, s));
+    HIP_CHECK(ctx, download_as<double>(p->zCord, c.z, c.n, s));
+    HIP_CHECK(ctx, download_as<double>(p->xTheta, c.tx, c.n, s));
+    HIP_CHECK(ctx, download_as<double>(p->yTheta, c.ty, c.n, s));
+    HIP_CHECK(ctx, download_as<double>(p->zTheta, c.tz, c.n, s));
+    HIP_CHECK(ctx, download_as<double>(p->radius, c.radius, c.n, s));
+    double *dd[6] = {p->xDiff, p->yDiff, p->zDiff, p->uDiff, p->vDiff, p->wDiff};
+    double *fd[6] = {p->xForce, p->yForce, p->zForce, p->uForce, p->vForce, p->wForce};
+    for (int k = 0; k < 6; k++) HIP_CHECK(ctx, download_as<double>(dd[k], c.diff[k], c.n, s));
+    for (int k = 0; k < 6; k++) {
+        if (c.prec == HDM_PREC_FP64) HIP_CHECK(ctx, download_as<double>(fd[k], c.force[k], c.n, s));
+        else HIP_CHECK(ctx, download_as<float>(fd[k], c.force[k], c.n, s));
+    }
+    if (p->type && c.n > 0) HIP_CHECK(ctx, d2h(p->type, c.type.ptr, (size_t)c.n, s));
+    return HDM_OK;
+}
+
+int hdm_links_upload(hdm_ctx *ctx, const hdm_links_host *l) {
+    if (!valid(ctx) || !l || l->n < 0) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (l->n > 0 && (!l->localPartIndA || !l->localPartIndB || !l->state || !l->initialLen))
+        return fail(ctx, HDM_E_INVALID, "hdm_links_upload: missing required field");
+    if (2 * l->n >= (int64_t)INT32_MAX) return fail(ctx, HDM_E_CAPACITY, "more than int32 link sides");
+    hipStream_t s = c.stream;
+    c.L = l->n;
+    size_t L = (size_t)(c.L > 0 ? c.L : 1);
+    HIP_CHECK(ctx, c.la.ensure(sizeof(int32_t) * L));
+    HIP_CHECK(ctx, c.lb.ensure(sizeof(int32_t) * L));
+    HIP_CHECK(ctx, c.state.ensure(L));
+    if (c.L > 0) {
+        HIP_CHECK(ctx, hipMemcpyAsync(c.la.ptr, l->localPartIndA, sizeof(int32_t) * c.L, hipMemcpyHostToDevice, s));
+        HIP_CHECK(ctx, hipMemcpyAsync(c.lb.ptr, l->localPartIndB, sizeof(int32_t) * c.L, hipMemcpyHostToDevice, s));
+        HIP_CHECK(ctx, hipMemcpyAsync(c.state.ptr, l->state, (size_t)c.L, hipMemcpyHostToDevice, s));
+    }
+    HIP_CHECK(ctx, upload_as<double>(c.initialLen, l->initialLen, c.L, s));
+    c.hasMaxLoad = l->maxLoad != nullptr;
+    if (c.hasMaxLoad) HIP_CHECK(ctx, upload_as<double>(c.maxLoad, l->maxLoad, c.L, s));
+    for (int k = 0; k < 6; k++) HIP_CHECK(ctx, upload_as<double>(c.ctStage[k], l->CT[k], 2 * c.L, s));
+    HIP_CHECK(ctx, hipStreamSynchronize(s));
+    c.linksBuilt = false;
+    return HDM_OK;
+}
+
+int hdm_links_download(hdm_ctx *ctx, hdm_links_host *l) {
+    if (!valid(ctx) || !l) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (l->n != c.L) return fail(ctx, HDM_E_INVALID, "hdm_links_download: n mismatch");
+    hipStream_t s = c.stream;
+    if (c.L == 0) return HDM_OK;
+    if (l->localPartIndA) HIP_CHECK(ctx, d2h(l->localPartIndA, c.la.ptr, sizeof(int32_t) * c.L, s));
+    if (l->localPartIndB) HIP_CHECK(ctx, d2h(l->localPartIndB, c.lb.ptr, sizeof(int32_t) * c.L, s));
+    if (l->state) HIP_CHECK(ctx, d2h(l->state, c.state.ptr, (size_t)c.L, s));
+    HIP_CHECK(ctx, download_as<double>(l->initialLen, c.initialLen, c.L, s));
+    if (l->maxLoad && c.hasMaxLoad) HIP_CHECK(ctx, download_as<double>(l->maxLoad, c.maxLoad, c.L, s));
+    bool want = false;
+    for (int k = 0; k < 6; k++) want |= l->CT[k] != nullptr;
+    if (want) {
+        if (c.linksBuilt) HIP_CHECK(ctx, hdm::launch_ct_export(c));
+        for (int k = 0; k < 6; k++) HIP_CHECK(ctx, download_as<double>(l->CT[k], c.ctStage[k], 2 * c.L, s));
+    }
+    return HDM_OK;
+}
+
+// ---------------------------------------------------------------------------
+int hdm_grid_build(hdm_ctx *ctx, double searchDistance) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!(searchDistance > 0)) return fail(ctx, HDM_E_INVALID, "searchDistance must be > 0");
+    if (c.n <= 1) return fail(ctx, HDM_E_STATE, "Generating grid for empty data (ParticleData::getLimits)");
+    Timer t(c, HDM_T_GRID);
+    HIP_CHECK(ctx, hdm::launch_bbox(c));
+    uint64_t keys[6];
+    HIP_CHECK(ctx, d2h(keys, c.red.ptr, sizeof(keys), c.stream));
+    hdm::GridState &g = c.grid;
+    g.xMin = hdm::ordered_value(keys[0]);
+    g.yMin = hdm::ordered_value(keys[1]);
+    g.zMin = hdm::ordered_value(keys[2]);
+    g.xMax = hdm::ordered_value(keys[3]);
+    g.yMax = hdm::ordered_value(keys[4]);
+    g.zMax = hdm::ordered_value(keys[5]);
+    g.spacing = searchDistance;
+    double ex = std::floor((g.xMax - g.xMin) / searchDistance);
+    double ey = std::floor((g.yMax - g.yMin) / searchDistance);
+    double ez = std::floor((g.zMax - g.zMin) / searchDistance);
+    if (!(ex < 1e9 && ey < 1e9 && ez < 1e9)) return fail(ctx, HDM_E_CAPACITY, "grid extent not finite or too large");
+    g.X = 1 + (int)ex;
+    g.Y = 1 + (int)ey;
+    g.Z = 1 + (int)ez;
+    double cells = (double)g.X * (double)g.Y * (double)g.Z;
+    const double maxCells = (double)(1ll << 30);
+    if (cells > maxCells)
+        return fail(ctx, HDM_E_CAPACITY, "grid has " + std::to_string(cells) +
+                                             " cells (limit 2^30): particles spread too far, did something explode?");
+    g.totalCells = (int64_t)g.X * g.Y * g.Z;
+    HIP_CHECK(ctx, c.colour.ensure(sizeof(int32_t) * (size_t)c.n));
+    HIP_CHECK(ctx, c.order.ensure(sizeof(int32_t) * (size_t)c.n));
+    HIP_CHECK(ctx, c.cellStart.ensure(sizeof(int32_t) * (size_t)(g.totalCells + 1)));
+    HIP_CHECK(ctx, c.cellCursor.ensure(sizeof(int32_t) * (size_t)(g.totalCells + 1)));
+    HIP_CHECK(ctx, hdm::launch_colour_count(c));
+    HIP_CHECK(ctx, hdm::launch_exclusive_scan_i32(c, c.cellCursor.as<int32_t>(), c.cellStart.as<int32_t>(), g.totalCells));
+    HIP_CHECK(ctx, hdm::launch_scatter_sort(c));
+    uint64_t st[3];
+    HIP_CHECK(ctx, d2h(st, c.red.as<uint64_t>() + 8, sizeof(st), c.stream));
+    if (st[0]) return fail(ctx, HDM_E_HIP, "grid colour out of range");
+    g.occupied = (int64_t)st[1];
+    g.maxPerCell = (int32_t)st[2];
+    g.built = true;
+    c.neighBuilt = false;
+    return HDM_OK;
+}
+
+int hdm_grid_info_get(hdm_ctx *ctx, hdm_grid_info *info) {
+    if (!valid(ctx) || !info) return HDM_E_INVALID;
+    const hdm::GridState &g = ctx->c.grid;
+    if (!g.built) return fail(ctx, HDM_E_STATE, "grid not built");
+    info->spacing = g.spacing;
+    info->xMin = g.xMin; info->yMin = g.yMin; info->zMin = g.zMin;
+    info->xMax = g.xMax; info->yMax = g.yMax; info->zMax = g.zMax;
+    info->X = g.X; info->Y = g.Y; info->Z = g.Z;
+    info->totalCells = g.totalCells;
+    info->occupiedCells = g.occupied;
+    info->maxPerCell = g.maxPerCell;
+    return HDM_OK;
+}
+
+int hdm_grid_download(hdm_ctx *ctx, int64_t *colours, int32_t *iterationOrder) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!c.grid.built) return fail(ctx, HDM_E_STATE, "grid not built");
+    if (colours) {
+        std::vector<int32_t> tmp((size_t)c.n);
+        HIP_CHECK(ctx, d2h(tmp.data(), c.colour.ptr, sizeof(int32_t) * c.n, c.stream));
+        for (int64_t i = 0; i < c.n; i++) colours[i] = tmp[(size_t)i];
+    }
+    if (iterationOrder) HIP_CHECK(ctx, d2h(iterationOrder, c.order.ptr, sizeof(int32_t) * c.n, c.stream));
+    return HDM_OK;
+}
+
+int hdm_neighbours_build(hdm_ctx *ctx, double interactDist) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!c.grid.built) return fail(ctx, HDM_E_STATE, "grid not built");
+    Timer t(c, HDM_T_NEIGH);
+    HIP_CHECK(ctx, c.nbrCount.ensure(sizeof(int32_t) * (size_t)c.n));
+    HIP_CHECK(ctx, hdm::launch_neigh_count(c, interactDist));
+    uint64_t *mx = c.red.as<uint64_t>() + 12;
+    HIP_CHECK(ctx, hdm::launch_max_i32(c, c.nbrCount.as<int32_t>(), c.n, mx));
+    uint64_t m = 0;
+    HIP_CHECK(ctx, d2h(&m, mx, sizeof(m), c.stream));
+    c.maxNeigh = (int32_t)(m > 0 ? m : 1);
+    HIP_CHECK(ctx, c.nbr.ensure(sizeof(int32_t) * (size_t)c.tiles() * 64 * (size_t)c.maxNeigh));
+    HIP_CHECK(ctx, hdm::launch_neigh_fill(c, interactDist));
+    c.neighBuilt = true;
+    return HDM_OK;
+}
+
+int hdm_neighbours_total(hdm_ctx *ctx, int64_t *total, int32_t *maxPerParticle) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!c.neighBuilt) return fail(ctx, HDM_E_STATE, "neighbour list not built");
+    std::vector<int32_t> cnt((size_t)c.n);
+    HIP_CHECK(ctx, d2h(cnt.data(), c.nbrCount.ptr, sizeof(int32_t) * c.n, c.stream));
+    int64_t t = 0;
+    for (auto v : cnt) t += v;
+    if (total) *total = t;
+    if (maxPerParticle) *maxPerParticle = c.maxNeigh;
+    return HDM_OK;
+}
+
+int hdm_neighbours_download(hdm_ctx *ctx, int32_t *counts, int64_t *starts, int32_t *list, int64_t listCap) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!c.neighBuilt) return fail(ctx, HDM_E_STATE, "neighbour list not built");
+    std::vector<int32_t> cnt((size_t)c.n);
+    HIP_CHECK(ctx, d2h(cnt.data(), c.nbrCount.ptr, sizeof(int32_t) * c.n, c.stream));
+    int64_t total = 0;
+    for (auto v : cnt) total += v;
+    if (counts) std::memcpy(counts, cnt.data(), sizeof(int32_t) * c.n);
+    if (starts) {
+        starts[0] = 0;
+        for (int64_t i = 0; i < c.n; i++) starts[i + 1] = starts[i] + cnt[(size_t)i];
+    }
+    if (list) {
+        if (listCap < total) return fail(ctx, HDM_E_CAPACITY, "neighbour list buffer too small");
+        size_t w = (size_t)c.maxNeigh;
+        std::vector<int32_t> ell((size_t)c.tiles() * 64 * w);
+        HIP_CHECK(ctx, d2h(ell.data(), c.nbr.ptr, sizeof(int32_t) * ell.size(), c.stream));
+        int64_t pos = 0;
+        for (int64_t p = 0; p < c.n; p++)
+            for (int32_t k = 0; k < cnt[(size_t)p]; k++)
+                list[pos++] = ell[(size_t)(p >> 6) * 64 * w + (size_t)k * 64 + (size_t)(p & 63)];
+    }
+    return HDM_OK;
+}
+
+int hdm_links_build(hdm_ctx *ctx) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (c.n == 0) return fail(ctx, HDM_E_STATE, "no particles");
+    Timer t(c, HDM_T_LINKS);
+    // Side data already in wave tiles goes back to checkpoint order first, so
+    // a rebuild is a pure re-permutation (LinkDataHandler::reorderSideDataPadded).
+    if (c.linksBuilt) HIP_CHECK(ctx, hdm::launch_ct_export(c));
+    HIP_CHECK(ctx, c.linkCount.ensure(sizeof(uint32_t) * (size_t)c.n));
+    HIP_CHECK(ctx, c.linkCursor.ensure(sizeof(uint32_t) * (size_t)c.n));
+    HIP_CHECK(ctx, hdm::launch_link_count(c));
+    uint64_t *mx = c.red.as<uint64_t>() + 12;
+    HIP_CHECK(ctx, hdm::launch_max_i32(c, c.linkCount.as<int32_t>(), c.n, mx));
+    uint64_t m[1], errf = 0;
+    HIP_CHECK(ctx, d2h(m, mx, sizeof(m), c.stream));
+    HIP_CHECK(ctx, d2h(&errf, c.red.as<uint64_t>() + 8, sizeof(errf), c.stream));
+    if (errf) return fail(ctx, HDM_E_INVALID, "Dangling links found, connecting links failed (LinkData.h:105)");
+    c.maxLinks = (int32_t)(m[0] > 0 ? m[0] : 1);
+    size_t slots = (size_t)c.tiles() * 64 * (size_t)c.maxLinks;
+    HIP_CHECK(ctx, c.slotLink.ensure(sizeof(int32_t) * slots));
+    HIP_CHECK(ctx, c.slotOther.ensure(sizeof(int32_t) * slots));
+    HIP_CHECK(ctx, c.sidePos.ensure(sizeof(int32_t) * (size_t)(2 * c.L > 0 ? 2 * c.L : 1)));
+    for (int k = 0; k < 6; k++) HIP_CHECK(ctx, c.ct[k].ensure(c.ctBytes() * slots));
+    HIP_CHECK(ctx, hdm::launch_link_fill(c));
+    HIP_CHECK(ctx, hdm::launch_ct_import(c));
+    HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
+    c.linksBuilt = true;
+    return HDM_OK;
+}
+
+int hdm_linklists_download(hdm_ctx *ctx, int32_t *counts, int64_t *starts, int32_t *linkId, int32_t *other,
+                           int32_t *end, int64_t cap) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!c.linksBuilt) return fail(ctx, HDM_E_STATE, "link lists not built");
+    std::vector<int32_t> cnt((size_t)c.n);
+    HIP_CHECK(ctx, d2h(cnt.data(), c.linkCount.ptr, sizeof(int32_t) * c.n, c.stream));
+    int64_t total = 0;
+    for (auto v : cnt) total += v;
+    if (counts) std::memcpy(counts, cnt.data(), sizeof(int32_t) * c.n);
+    if (starts) {
+        starts[0] = 0;
+        for (int64_t i = 0; i < c.n; i++) starts[i + 1] = starts[i] + cnt[(size_t)i];
+    }
+    if (linkId || other || end) {
+        if (cap < total) return fail(ctx, HDM_E_CAPACITY, "link list buffer too small");
+        size_t w = (size_t)c.maxLinks;
+        size_t slots = (size_t)c.tiles() * 64 * w;
+        std::vector<int32_t> sl(slots), so(slots), A((size_t)c.L), B((size_t)c.L);
+        HIP_CHECK(ctx, d2h(sl.data(), c.slotLink.ptr, sizeof(int32_t) * slots, c.stream));
+        HIP_CHECK(ctx, d2h(so.data(), c.slotOther.ptr, sizeof(int32_t) * slots, c.stream));
+        if (c.L) HIP_CHECK(ctx, d2h(B.data(), c.lb.ptr, sizeof(int32_t) * c.L, c.stream));
+        std::vector<int32_t> sp((size_t)(2 * c.L));
+        if (c.L) HIP_CHECK(ctx, d2h(sp.data(), c.sidePos.ptr, sizeof(int32_t) * 2 * c.L, c.stream));
+        int64_t pos = 0;
+        for (int64_t p = 0; p < c.n; p++)
+            for (int32_t k = 0; k < cnt[(size_t)p]; k++) {
+                size_t s = (size_t)(p >> 6) * 64 * w + (size_t)k * 64 + (size_t)(p & 63);
+                int32_t l = sl[s];
+                if (linkId) linkId[pos] = l;
+                if (other) other[pos] = so[s];
+                // the end is whichever of the link's two sides maps to this slot
+                if (end) end[pos] = (sp[(size_t)(2 * l + 1)] == (int32_t)s) ? 1 : 0;
+                pos++;
+            }
+    }
+    return HDM_OK;
+}
+
+// ---------------------------------------------------------------------------
+int hdm_force(hdm_ctx *ctx, const hdm_force_params *fp) {
+    if (!valid(ctx) || !fp) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!c.neighBuilt || !c.linksBuilt) return fail(ctx, HDM_E_STATE, "neighbour list / link lists not built");
+    Timer t(c, HDM_T_FORCE);
+    HIP_CHECK(ctx, hdm::launch_force(c, *fp));
+    return HDM_OK;
+}
+
+int hdm_integrate(hdm_ctx *ctx, const hdm_integ_params *ip) {
+    if (!valid(ctx) || !ip) return HDM_E_INVALID;
+    if (ip->variant != HDM_INTEG_ICEBLOCK && ip->variant != HDM_INTEG_GLACIER)
+        return fail(ctx, HDM_E_INVALID, "unknown integrator variant");
+    Timer t(ctx->c, HDM_T_INTEGRATE);
+    HIP_CHECK(ctx, hdm::launch_integrate(ctx->c, *ip));
+    return HDM_OK;
+}
+
+int hdm_break(hdm_ctx *ctx, const hdm_break_params *bp, int64_t *nBroken) {
+    if (!valid(ctx) || !bp) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    uint64_t *counter = c.red.as<uint64_t>() + 16;
+    if (nBroken) HIP_CHECK(ctx, hdm::launch_fill_u64(c.stream, counter, 0ull, 1));
+    {
+        Timer t(c, HDM_T_BREAK);
+        HIP_CHECK(ctx, hdm::launch_break(c, *bp, nBroken ? counter : nullptr));
+    }
+    if (nBroken) {
+        uint64_t v = 0;
+        HIP_CHECK(ctx, d2h(&v, counter, sizeof(v), c.stream));
+        *nBroken = (int64_t)v;
+    }
+    return HDM_OK;
+}
+
+int hdm_skin_heuristic(hdm_ctx *ctx, int32_t stepSize, double *dist) {
+    if (!valid(ctx) || !dist) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    uint64_t *out = c.red.as<uint64_t>() + 20;
+    {
+        Timer t(c, HDM_T_SKIN);
+        HIP_CHECK(ctx, hdm::launch_skin_heuristic(c, out));
+    }
+    uint64_t k[3];
+    HIP_CHECK(ctx, d2h(k, out, sizeof(k), c.stream));
+    double xm = hdm::ordered_value(k[0]), ym = hdm::ordered_value(k[1]), zm = hdm::ordered_value(k[2]);
+    *dist = stepSize * std::sqrt((xm * xm) + (ym * ym) + (zm * zm));
+    return HDM_OK;
+}
+
+int hdm_skin_snapshot(hdm_ctx *ctx) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    HIP_CHECK(ctx, hdm::launch_snapshot(ctx->c));
+    ctx->c.snapshotValid = true;
+    return HDM_OK;
+}
+
+int hdm_skin_exact(hdm_ctx *ctx, double *maxDisp) {
+    if (!valid(ctx) || !maxDisp) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!c.snapshotValid) return fail(ctx, HDM_E_STATE, "no position snapshot");
+    uint64_t *out = c.red.as<uint64_t>() + 24;
+    {
+        Timer t(c, HDM_T_SKIN);
+        HIP_CHECK(ctx, hdm::launch_skin_exact(c, out));
+    }
+    uint64_t k = 0;
+    HIP_CHECK(ctx, d2h(&k, out, sizeof(k), c.stream));
+    *maxDisp = std::sqrt(hdm::ordered_value(k));
+    return HDM_OK;
+}
+
+int hdm_count_broken(hdm_ctx *ctx, int64_t *broken) {
+    if (!valid(ctx) || !broken) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    uint64_t *counter = c.red.as<uint64_t>() + 28;
+    HIP_CHECK(ctx, hdm::launch_count_broken(c, counter));
+    uint64_t v = 0;
+    HIP_CHECK(ctx, d2h(&v, counter, sizeof(v), c.stream));
+    *broken = (int64_t)v;
+    return HDM_OK;
+}
+
+int hdm_set_timestep(hdm_ctx *ctx, int64_t ts) {
+    if (!valid(ctx) || ts < 0) return HDM_E_INVALID;
+    ctx->c.ts = ts;
+    ctx->c.accumulatedDist = 0;
+    return HDM_OK;
+}
+
+// gridService (HiDEMLib.h:533-628) -> computePartForce -> applyForce ->
+// linkUpdate (ts*dt > breakTime) -> finalizeTs, nSteps times.
+int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats *stats) {
+    if (!valid(ctx) || !rp || nSteps < 0) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!c.linksBuilt) {
+        int r = hdm_links_build(ctx);
+        if (r != HDM_OK) return r;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    int64_t rebuilds = 0;
+    for (int64_t step = 0; step < nSteps; step++) {
+        bool regen = !c.neighBuilt || !c.snapshotValid;
+        if (rp->policy == HDM_REBUILD_INTERVAL) {
+            if (rp->regenInterval > 0 && (c.ts % rp->regenInterval) == 0) regen = true;
+        } else if (rp->policy == HDM_REBUILD_HEURISTIC) {
+            if (c.ts % 10 == 0) {
+                double d = 0;
+                int r = hdm_skin_heuristic(ctx, 10, &d);
+                if (r != HDM_OK) return r;
+                c.accumulatedDist += d;
+                if (c.accumulatedDist >= rp->interactDist / 2.0) regen = true;
+            }
+        } else if (!regen) {
+            double d = 0;
+            int r = hdm_skin_exact(ctx, &d);
+            if (r != HDM_OK) return r;
+            if (2.0 * d >= rp->interactDist - rp->force.attractDistance) regen = true;
+        }
+        if (rebuildLog) rebuildLog[step] = regen ? 1 : 0;
+        if (regen) {
+            c.accumulatedDist = 0;
+            int r = hdm_grid_build(ctx, rp->searchDistance);
+            if (r == HDM_OK) r = hdm_neighbours_build(ctx, rp->interactDist);
+            if (r == HDM_OK) r = hdm_skin_snapshot(ctx);
+            if (r != HDM_OK) return r;
+            rebuilds++;
+        }
+        int r = hdm_force(ctx, &rp->force);
+        if (r == HDM_OK) r = hdm_integrate(ctx, &rp->integ);
+        if (r == HDM_OK && (double)c.ts * rp->dt > rp->breakTime) r = hdm_break(ctx, &rp->brk, nullptr);
+        if (r != HDM_OK) return r;
+        c.ts++;
+    }
+    HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
+    auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        stats->timestep = c.ts;
+        stats->steps = nSteps;
+        stats->rebuilds = rebuilds;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+        int64_t b = 0;
+        int r = hdm_count_broken(ctx, &b);
+        if (r != HDM_OK) return r;
+        stats->brokenTotal = b;
+    }
+    return HDM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// Internal state of an hdm_ctx: device mirrors of the hot-path fields, the
+// neighbour grid, the neighbour list, the per-particle link lists and the side
+// (CT) store, all owned by the context and ordered on its HIP stream.
+//
+// Device layout (every array is one hipMalloc, structure of arrays):
+//   particles  x/y/zCord, x/y/zTheta, x..wDiff, radius: f64;  x..wForce: f64
+//              (FP64 form) or f32 (mixed form); type: u8.  Index = pInd.
+//   links      localPartIndA/B: i32, state: u8, initialLen, maxLoad: f64.
+//   slots      "wave tiles": particle p owns slot k of tile p/64 at
+//              tile*64*width + k*64 + p%64, so the 64 lanes of a wavefront
+//              that process 64 consecutive particles read slot k of all of
+//              them from one contiguous 256 B (i32) / 512 B (f64) run.
+//              Neighbour list (width maxNeigh) and link slots (width
+//              maxLinks: link id, other endpoint, CT0..CT5 side state) use it.
+//   grid       colour (i32 per particle), iterationOrder (i32), dense
+//              cellStart (i32, totalCells+1).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hidem/hdm_abi.h"
+
+namespace hdm {
+
+// Raw device allocation that grows on demand (contents are not preserved).
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t want) {
+        if (want <= bytes && ptr) return hipSuccess;
+        release();
+        if (want == 0) want = 256;
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e != hipSuccess) { ptr = nullptr; return e; }
+        bytes = want;
+        return hipSuccess;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+struct TimedLaunch {
+    int cls;
+    hipEvent_t start, stop;
+};
+
+struct GridState {
+    double spacing = 0, xMin = 0, yMin = 0, zMin = 0, xMax = 0, yMax = 0, zMax = 0;
+    int32_t X = 0, Y = 0, Z = 0;
+    int64_t totalCells = 0;
+    int64_t occupied = 0;
+    int32_t maxPerCell = 0;
+    bool built = false;
+};
+
+struct Context {
+    int32_t prec = HDM_PREC_FP64;
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int32_t numCU = 0;
+
+    // particles
+    int64_t n = 0;
+    DevBuf x, y, z, tx, ty, tz, diff[6], radius, force[6], type;
+    // links
+    int64_t L = 0;
+    bool hasMaxLoad = false;
+    DevBuf la, lb, state, initialLen, maxLoad;
+    DevBuf ctStage[6];     // CT in checkpoint order (2l + end) during upload/download
+    // link slots (wave tiles) and side store
+    int32_t maxLinks = 0;
+    bool linksBuilt = false;
+    DevBuf linkCount, linkCursor, slotLink, slotOther, ct[6];
+    DevBuf sidePos;        // 2L: tiled slot of side (l, end)
+    // neighbour list (wave tiles)
+    int32_t maxNeigh = 0;
+    bool neighBuilt = false;
+    DevBuf nbrCount, nbr;
+    // grid
+    GridState grid;
+    DevBuf colour, order, cellStart, cellCursor, scanPartials;
+    // reductions / scalars
+    DevBuf red;            // u64 scratch for bbox / max reductions and counters
+    DevBuf x0, y0, z0;     // positions at the last neighbour-list build
+    bool snapshotValid = false;
+    // run state
+    int64_t ts = 0;
+    double accumulatedDist = 0;
+    // timing
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> eventPool;
+    double ms[HDM_T_COUNT] = {0};
+    int64_t launches[HDM_T_COUNT] = {0};
+
+    int64_t tiles() const { return (n + 63) / 64; }
+    size_t ctBytes() const { return prec == HDM_PREC_FP64 ? 8 : 4; }
+    size_t forceBytes() const { return prec == HDM_PREC_FP64 ? 8 : 4; }
+};
+
+// Launch helpers shared by the kernel files.
+int xcd_grid(int64_t nBlocks);
+
+// Kernel launchers (defined in kernels_*.hip).  All enqueue on ctx.stream.
+hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, int64_t n);
+hipError_t launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t v, int64_t n);
+hipError_t launch_bbox(Context &c);
+hipError_t launch_colour_count(Context &c);
+hipError_t launch_exclusive_scan_i32(Context &c, const int32_t *in, int32_t *out, int64_t count);
+hipError_t launch_scatter_sort(Context &c);
+hipError_t launch_neigh_count(Context &c, double interactDist);
+hipError_t launch_neigh_fill(Context &c, double interactDist);
+hipError_t launch_max_i32(Context &c, const int32_t *v, int64_t n, uint64_t *out);
+hipError_t launch_link_count(Context &c);
+hipError_t launch_link_fill(Context &c);
+hipError_t launch_ct_import(Context &c);   // ctStage (checkpoint order) -> ct (tiled)
+hipError_t launch_ct_export(Context &c);   // ct (tiled) -> ctStage (checkpoint order)
+hipError_t launch_force(Context &c, const hdm_force_params &fp);
+hipError_t launch_integrate(Context &c, const hdm_integ_params &ip);
+hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter);
+hipError_t launch_skin_heuristic(Context &c, uint64_t *out3);
+hipError_t launch_skin_exact(Context &c, uint64_t *out1);
+hipError_t launch_snapshot(Context &c);
+hipError_t launch_count_broken(Context &c, uint64_t *counter);
+
+// Monotone map between doubles and unsigned 64-bit keys (used for
+// order-independent, hence deterministic, atomic min/max of doubles).
+__host__ __device__ inline uint64_t ordered_key(double d) {
+    uint64_t u = __builtin_bit_cast(uint64_t, d);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__host__ __device__ inline double ordered_value(uint64_t k) {
+    uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __builtin_bit_cast(double, u);
+}
+
+}  // namespace hdm

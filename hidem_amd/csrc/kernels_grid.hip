@@ -1,0 +1,371 @@
+// Neighbour grid and neighbour list kernels for gfx950.
+//
+//   bbox           ParticleData::getLimits (ParticleData.h:47-59): min/max of
+//                  the coordinates, reduced per wavefront with DPP shuffles and
+//                  merged with order-independent 64-bit atomics (exact, so the
+//                  result does not depend on scheduling).
+//   colour_count   NeighbourGrid::colourParticles (NeighbourGrid.cpp:116-161):
+//                  cell id per particle + per-cell population.
+//   scan           exclusive scan of the populations -> dense cellStart (the
+//                  reference keeps a hash of occupied colours -> bucket; a
+//                  dense array over the X*Y*Z box answers the same lookups).
+//   scatter_sort   NeighbourGrid::generateIterationList (:163-184): particles
+//                  grouped by cell; each cell's members then sorted by pInd so
+//                  iterationOrder is exactly "sorted by (colour, pInd)".
+//   neigh_count /  NeighAccel::generateNumNeigh / iterateBlocks
+//   neigh_fill     (NeighAccel.h:85-137) over NeighbourGrid::
+//                  iteratePossibleNeighbours (NeighbourGrid.h:85-105): 27
+//                  linear colour offsets visited in z,y,x order, members in
+//                  iterationOrder, self skipped, kept if
+//                  sqrt(d.d) - (r_i + r_j) < interactDist (FP64).  Output is
+//                  a wave-tiled ELL list (see hdm_context.h).
+#include "hdm_context.h"
+#include "kernel_common.h"
+
+namespace hdm {
+
+namespace {
+
+constexpr int TPB = 256;
+
+__global__ void __launch_bounds__(TPB) k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void __launch_bounds__(TPB) k_fill_u64(uint64_t *p, uint64_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// out[0..2] = min keys of x,y,z; out[3..5] = max keys.
+__global__ void __launch_bounds__(TPB) k_bbox(const double *__restrict__ x, const double *__restrict__ y,
+                                               const double *__restrict__ z, int64_t n, uint64_t *out) {
+    double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+        double v[3] = {x[i], y[i], z[i]};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            mn[c] = v[c] < mn[c] ? v[c] : mn[c];
+            mx[c] = v[c] > mx[c] ? v[c] : mx[c];
+        }
+    }
+    __shared__ double smn[3][TPB / 64], smx[3][TPB / 64];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        mn[c] = wave_min(mn[c]);
+        mx[c] = wave_max(mx[c]);
+    }
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) { smn[c][w] = mn[c]; smx[c][w] = mx[c]; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int c = threadIdx.x;
+        double a = smn[c][0], b = smx[c][0];
+        for (int k = 1; k < TPB / 64; k++) {
+            a = smn[c][k] < a ? smn[c][k] : a;
+            b = smx[c][k] > b ? smx[c][k] : b;
+        }
+        atomicMin((unsigned long long *)&out[c], (unsigned long long)ordered_key(a));
+        atomicMax((unsigned long long *)&out[3 + c], (unsigned long long)ordered_key(b));
+    }
+}
+
+struct GridArgs {
+    double xMin, yMin, zMin, s;
+    int32_t X, Y;
+    int64_t totalCells;
+};
+
+__global__ void __launch_bounds__(TPB) k_colour_count(const double *__restrict__ x, const double *__restrict__ y,
+                                                       const double *__restrict__ z, int64_t n, GridArgs g,
+                                                       int32_t *__restrict__ colour, uint32_t *__restrict__ cellCount,
+                                                       uint64_t *errFlag) {
+    int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    if (p >= n) return;
+    int ix = (int)floor((x[p] - g.xMin) / g.s);
+    int iy = (int)floor((y[p] - g.yMin) / g.s);
+    int iz = (int)floor((z[p] - g.zMin) / g.s);
+    int64_t c64 = (int64_t)ix + (int64_t)iy * (int64_t)g.X + (int64_t)iz * (int64_t)g.X * (int64_t)g.Y;
+    int c = (int)c64;
+    colour[p] = c;
+    if (c < 0 || (int64_t)c >= g.totalCells) {
+        atomicOr((unsigned long long *)errFlag, 1ull);
+        return;
+    }
+    atomicAdd(&cellCount[c], 1u);
+}
+
+// ---- exclusive scan (int32), 1024 elements per block --------------------------
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = TPB * SCAN_ITEMS;
+
+__device__ inline int32_t block_exclusive_scan(int32_t v, int32_t *total) {
+    __shared__ int32_t wsum[TPB / 64];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int32_t inc = wave_inclusive_sum_i32(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int32_t off = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < TPB / 64; k++) {
+        off += (k < w) ? wsum[k] : 0;
+        tot += wsum[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+__global__ void __launch_bounds__(TPB) k_scan_partials(const int32_t *__restrict__ in, int64_t count,
+                                                        int32_t *__restrict__ partials) {
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) s += (base + k < count) ? in[base + k] : 0;
+    int32_t tot;
+    (void)block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// Single workgroup: exclusive scan of the per-tile partial sums, in place.
+__global__ void __launch_bounds__(TPB) k_scan_partials_scan(int32_t *partials, int64_t nParts) {
+    int32_t carry = 0;
+    for (int64_t base = 0; base < nParts; base += TPB) {
+        int64_t i = base + threadIdx.x;
+        int32_t v = (i < nParts) ? partials[i] : 0;
+        int32_t tot;
+        int32_t ex = block_exclusive_scan(v, &tot);
+        if (i < nParts) partials[i] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_scan_apply(const int32_t *__restrict__ in, int64_t count,
+                                                     const int32_t *__restrict__ partials, int32_t *__restrict__ out) {
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int32_t v[SCAN_ITEMS];
+    int32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = (base + k < count) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int32_t tot;
+    int32_t ex = block_exclusive_scan(s, &tot) + partials[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < count) out[base + k] = ex;
+        ex += v[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == TPB - 1) out[count] = ex;
+}
+
+__global__ void __launch_bounds__(TPB) k_scatter(const int32_t *__restrict__ colour, int64_t n,
+                                                  int32_t *__restrict__ cursor, int32_t *__restrict__ order) {
+    int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    if (p >= n) return;
+    int32_t pos = atomicAdd(&cursor[colour[p]], 1);
+    order[pos] = (int32_t)p;
+}
+
+// Sorts each cell's members by particle index; reports occupied cells and
+// the largest population (stats[0], stats[1]).
+__global__ void __launch_bounds__(TPB) k_cell_sort(const int32_t *__restrict__ cellStart, int64_t totalCells,
+                                                    int32_t *__restrict__ order, uint64_t *stats) {
+    int64_t c = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    uint32_t occ = 0, mx = 0;
+    if (c < totalCells) {
+        int32_t s = cellStart[c], e = cellStart[c + 1];
+        if (e > s) {
+            occ = 1;
+            mx = (uint32_t)(e - s);
+            for (int32_t i = s + 1; i < e; i++) {
+                int32_t v = order[i];
+                int32_t j = i - 1;
+                while (j >= s && order[j] > v) {
+                    order[j + 1] = order[j];
+                    j--;
+                }
+                order[j + 1] = v;
+            }
+        }
+    }
+    occ = wave_sum_u32(occ);
+    mx = wave_max_u32(mx);
+    if ((threadIdx.x & 63) == 0) {
+        if (occ) atomicAdd((unsigned long long *)&stats[0], (unsigned long long)occ);
+        if (mx) atomicMax((unsigned long long *)&stats[1], (unsigned long long)mx);
+    }
+}
+
+struct NeighArgs {
+    int64_t dirOffsets[27];
+    int64_t totalCells;
+    double interactDist;
+    int32_t width;  // ELL width (fill pass)
+};
+
+template <bool FILL>
+__global__ void __launch_bounds__(TPB) k_neigh(const double *__restrict__ x, const double *__restrict__ y,
+                                                const double *__restrict__ z, const double *__restrict__ radius,
+                                                int64_t n, const int32_t *__restrict__ colour,
+                                                const int32_t *__restrict__ cellStart,
+                                                const int32_t *__restrict__ order, NeighArgs a,
+                                                int32_t *__restrict__ count, int32_t *__restrict__ list) {
+    int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    if (p >= n) return;
+    const double xCord = x[p], yCord = y[p], zCord = z[p], rad = radius[p];
+    const int col = colour[p];
+    int32_t neighbours = 0;
+    int32_t *out = nullptr;
+    if (FILL) out = list + (p >> 6) * (int64_t)64 * a.width + (p & 63);
+    for (int nn = 0; nn < 27; ++nn) {
+        int target = (int)((int64_t)col + a.dirOffsets[nn]);
+        if (target < 0 || (int64_t)target >= a.totalCells) continue;
+        int32_t s = cellStart[target], e = cellStart[target + 1];
+        for (int32_t i = s; i < e; ++i) {
+            int32_t nInd = order[i];
+            if (nInd == p) continue;
+            double xDist = xCord - x[nInd];
+            double xDist2 = xDist * xDist;
+            double yDist = yCord - y[nInd];
+            double yDist2 = yDist * yDist;
+            double zDist = zCord - z[nInd];
+            double zDist2 = zDist * zDist;
+            double particleDistance2 = xDist2 + yDist2 + zDist2;
+            double particleDistance = sqrt(particleDistance2);
+            double radSum = rad + radius[nInd];
+            double dist = particleDistance - radSum;
+            if (dist < a.interactDist) {
+                if (FILL) {
+                    if (neighbours < a.width) out[(int64_t)neighbours * 64] = nInd;
+                }
+                neighbours++;
+            }
+        }
+    }
+    if (!FILL) count[p] = neighbours;
+}
+
+__global__ void __launch_bounds__(TPB) k_max_i32(const int32_t *__restrict__ v, int64_t n, uint64_t *out) {
+    uint32_t m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+        uint32_t a = (uint32_t)(v[i] < 0 ? 0 : v[i]);
+        m = a > m ? a : m;
+    }
+    m = wave_max_u32(m);
+    if ((threadIdx.x & 63) == 0 && m) atomicMax((unsigned long long *)out, (unsigned long long)m);
+}
+
+inline int blocks_for(int64_t n) { return (int)((n + TPB - 1) / TPB); }
+
+}  // namespace
+
+hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, int64_t n) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n)), dim3(TPB), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t v, int64_t n) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(n)), dim3(TPB), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_bbox(Context &c) {
+    uint64_t *r = c.red.as<uint64_t>();
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(TPB), 0, c.stream, r, ordered_key(__builtin_inf()), (int64_t)3);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(TPB), 0, c.stream, r + 3, ordered_key(-__builtin_inf()), (int64_t)3);
+    int64_t nb = blocks_for(c.n);
+    int cap = c.numCU > 0 ? c.numCU * 4 : 1024;
+    int grid = (int)(nb < cap ? nb : cap);
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_bbox, dim3(grid), dim3(TPB), 0, c.stream, c.x.as<double>(), c.y.as<double>(),
+                       c.z.as<double>(), c.n, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_colour_count(Context &c) {
+    GridArgs g{c.grid.xMin, c.grid.yMin, c.grid.zMin, c.grid.spacing, c.grid.X, c.grid.Y, c.grid.totalCells};
+    hipError_t e = launch_fill_u32(c.stream, c.cellCursor.as<uint32_t>(), 0u, c.grid.totalCells);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(TPB), 0, c.stream, c.red.as<uint64_t>() + 8, 0ull, (int64_t)1);
+    hipLaunchKernelGGL(k_colour_count, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
+                       c.y.as<double>(), c.z.as<double>(), c.n, g, c.colour.as<int32_t>(),
+                       c.cellCursor.as<uint32_t>(), c.red.as<uint64_t>() + 8);
+    return hipGetLastError();
+}
+
+hipError_t launch_exclusive_scan_i32(Context &c, const int32_t *in, int32_t *out, int64_t count) {
+    int64_t nParts = (count + SCAN_TILE - 1) / SCAN_TILE;
+    if (nParts < 1) nParts = 1;
+    hipError_t e = c.scanPartials.ensure(sizeof(int32_t) * (size_t)nParts);
+    if (e != hipSuccess) return e;
+    int32_t *parts = c.scanPartials.as<int32_t>();
+    hipLaunchKernelGGL(k_scan_partials, dim3((unsigned)nParts), dim3(TPB), 0, c.stream, in, count, parts);
+    hipLaunchKernelGGL(k_scan_partials_scan, dim3(1), dim3(TPB), 0, c.stream, parts, nParts);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nParts), dim3(TPB), 0, c.stream, in, count, parts, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_sort(Context &c) {
+    hipError_t e = hipMemcpyAsync(c.cellCursor.ptr, c.cellStart.ptr, sizeof(int32_t) * (size_t)c.grid.totalCells,
+                                  hipMemcpyDeviceToDevice, c.stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.colour.as<int32_t>(), c.n,
+                       c.cellCursor.as<int32_t>(), c.order.as<int32_t>());
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(TPB), 0, c.stream, c.red.as<uint64_t>() + 9, 0ull, (int64_t)2);
+    hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(c.grid.totalCells)), dim3(TPB), 0, c.stream,
+                       c.cellStart.as<int32_t>(), c.grid.totalCells, c.order.as<int32_t>(), c.red.as<uint64_t>() + 9);
+    return hipGetLastError();
+}
+
+static NeighArgs neigh_args(Context &c, double interactDist) {
+    NeighArgs a;
+    int i = 0;
+    for (int zD = -1; zD < 2; zD++)
+        for (int yD = -1; yD < 2; yD++)
+            for (int xD = -1; xD < 2; xD++)
+                a.dirOffsets[i++] = (int64_t)xD + (int64_t)yD * c.grid.X + (int64_t)zD * c.grid.X * (int64_t)c.grid.Y;
+    a.totalCells = c.grid.totalCells;
+    a.interactDist = interactDist;
+    a.width = c.maxNeigh;
+    return a;
+}
+
+hipError_t launch_neigh_count(Context &c, double interactDist) {
+    NeighArgs a = neigh_args(c, interactDist);
+    hipLaunchKernelGGL(k_neigh<false>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
+                       c.y.as<double>(), c.z.as<double>(), c.radius.as<double>(), c.n, c.colour.as<int32_t>(),
+                       c.cellStart.as<int32_t>(), c.order.as<int32_t>(), a, c.nbrCount.as<int32_t>(),
+                       (int32_t *)nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_neigh_fill(Context &c, double interactDist) {
+    NeighArgs a = neigh_args(c, interactDist);
+    hipLaunchKernelGGL(k_neigh<true>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
+                       c.y.as<double>(), c.z.as<double>(), c.radius.as<double>(), c.n, c.colour.as<int32_t>(),
+                       c.cellStart.as<int32_t>(), c.order.as<int32_t>(), a, (int32_t *)nullptr,
+                       c.nbr.as<int32_t>());
+    return hipGetLastError();
+}
+
+hipError_t launch_max_i32(Context &c, const int32_t *v, int64_t n, uint64_t *out) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(TPB), 0, c.stream, out, 0ull, (int64_t)1);
+    int64_t nb = blocks_for(n);
+    int cap = c.numCU > 0 ? c.numCU * 4 : 1024;
+    int grid = (int)(nb < cap ? nb : cap);
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_max_i32, dim3(grid), dim3(TPB), 0, c.stream, v, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace hdm

@@ -1,0 +1,213 @@
+// Per-particle link lists and the side (CT) layout for gfx950.
+//
+//   link_count   LinkAccelSide::getMaxLinksPerParticle (LinkAccelSide.h:69-90)
+//   link_fill    LinkAccelSide::generateLinks (:92-143): each link id placed
+//                in a slot of both endpoints, then every particle's slots are
+//                sorted ascending (the reference bubble-sorts, :123-137), and
+//                generatePerParticleIndexes (:43-67) resolves the other
+//                endpoint and which end this particle is.
+//   ct_import /  LinkAccelSide::sortSideData + LinkDataHandler::
+//   ct_export    reorderSideDataPadded (LinkDataHandler.h:176-191): the CT
+//                side arrays are permuted so that particle p's k-th link side
+//                lives in wave-tile slot k of p (the reference's AOS_CT order
+//                makes each particle's sides contiguous; the wave tile makes
+//                slot k of 64 consecutive particles contiguous instead, which
+//                is what a 64-lane wavefront reads in one access).  sidePos
+//                plays the role of sideIndEndA/B (LinkData::reindexSideData,
+//                LinkData.h:183-208).
+#include "hdm_context.h"
+#include "kernel_common.h"
+
+namespace hdm {
+
+namespace {
+
+constexpr int TPB = 256;
+
+__device__ inline int64_t slot_index(int64_t p, int32_t k, int32_t width) {
+    return (p >> 6) * (int64_t)64 * width + (int64_t)k * 64 + (p & 63);
+}
+
+__global__ void __launch_bounds__(TPB) k_link_count(const int32_t *__restrict__ A, const int32_t *__restrict__ B,
+                                                     int64_t L, int64_t n, uint32_t *__restrict__ count,
+                                                     uint64_t *errFlag) {
+    int64_t l = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (l >= L) return;
+    int32_t a = A[l], b = B[l];
+    if (a < 0 || a >= n || b < 0 || b >= n || a == b) {
+        atomicOr((unsigned long long *)errFlag, 1ull);
+        return;
+    }
+    atomicAdd(&count[a], 1u);
+    atomicAdd(&count[b], 1u);
+}
+
+__global__ void __launch_bounds__(TPB) k_link_fill(const int32_t *__restrict__ A, const int32_t *__restrict__ B,
+                                                    int64_t L, int32_t width, uint32_t *__restrict__ cursor,
+                                                    int32_t *__restrict__ slotLink) {
+    int64_t l = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (l >= L) return;
+    int32_t a = A[l], b = B[l];
+    uint32_t ka = atomicAdd(&cursor[a], 1u);
+    uint32_t kb = atomicAdd(&cursor[b], 1u);
+    slotLink[slot_index(a, (int32_t)ka, width)] = (int32_t)l;
+    slotLink[slot_index(b, (int32_t)kb, width)] = (int32_t)l;
+}
+
+__global__ void __launch_bounds__(TPB) k_link_sort(const int32_t *__restrict__ A, const int32_t *__restrict__ B,
+                                                    int64_t n, int32_t width, const uint32_t *__restrict__ count,
+                                                    int32_t *__restrict__ slotLink, int32_t *__restrict__ slotOther,
+                                                    int32_t *__restrict__ sidePos) {
+    int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    if (p >= n) return;
+    int32_t cnt = (int32_t)count[p];
+    int64_t base = slot_index(p, 0, width);
+    for (int32_t i = 1; i < cnt; i++) {
+        int32_t v = slotLink[base + (int64_t)i * 64];
+        int32_t j = i - 1;
+        while (j >= 0 && slotLink[base + (int64_t)j * 64] > v) {
+            slotLink[base + (int64_t)(j + 1) * 64] = slotLink[base + (int64_t)j * 64];
+            j--;
+        }
+        slotLink[base + (int64_t)(j + 1) * 64] = v;
+    }
+    for (int32_t k = 0; k < cnt; k++) {
+        int64_t s = base + (int64_t)k * 64;
+        int32_t l = slotLink[s];
+        int32_t other = B[l];
+        int32_t end = 0;
+        if (other == (int32_t)p) {
+            other = A[l];
+            end = 1;
+        }
+        slotOther[s] = other;
+        sidePos[2 * (int64_t)l + end] = (int32_t)s;
+    }
+}
+
+template <class CT>
+__global__ void __launch_bounds__(TPB) k_ct_import(const int32_t *__restrict__ sidePos, int64_t nSides,
+                                                    const double *__restrict__ s0, const double *__restrict__ s1,
+                                                    const double *__restrict__ s2, const double *__restrict__ s3,
+                                                    const double *__restrict__ s4, const double *__restrict__ s5,
+                                                    CT *__restrict__ c0, CT *__restrict__ c1, CT *__restrict__ c2,
+                                                    CT *__restrict__ c3, CT *__restrict__ c4, CT *__restrict__ c5) {
+    int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= nSides) return;
+    int64_t d = sidePos[i];
+    c0[d] = (CT)s0[i];
+    c1[d] = (CT)s1[i];
+    c2[d] = (CT)s2[i];
+    c3[d] = (CT)s3[i];
+    c4[d] = (CT)s4[i];
+    c5[d] = (CT)s5[i];
+}
+
+template <class CT>
+__global__ void __launch_bounds__(TPB) k_ct_export(const int32_t *__restrict__ sidePos, int64_t nSides,
+                                                    const CT *__restrict__ c0, const CT *__restrict__ c1,
+                                                    const CT *__restrict__ c2, const CT *__restrict__ c3,
+                                                    const CT *__restrict__ c4, const CT *__restrict__ c5,
+                                                    double *__restrict__ s0, double *__restrict__ s1,
+                                                    double *__restrict__ s2, double *__restrict__ s3,
+                                                    double *__restrict__ s4, double *__restrict__ s5) {
+    int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= nSides) return;
+    int64_t d = sidePos[i];
+    s0[i] = (double)c0[d];
+    s1[i] = (double)c1[d];
+    s2[i] = (double)c2[d];
+    s3[i] = (double)c3[d];
+    s4[i] = (double)c4[d];
+    s5[i] = (double)c5[d];
+}
+
+__global__ void __launch_bounds__(TPB) k_count_broken(const uint8_t *__restrict__ state, int64_t L, uint64_t *out) {
+    uint32_t c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < L; i += (int64_t)gridDim.x * TPB)
+        c += state[i] == 0 ? 1u : 0u;
+    c = wave_sum_u32(c);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd((unsigned long long *)out, (unsigned long long)c);
+}
+
+inline int blocks_for(int64_t n) { return (int)((n + TPB - 1) / TPB); }
+
+}  // namespace
+
+hipError_t launch_link_count(Context &c) {
+    hipError_t e = launch_fill_u32(c.stream, c.linkCount.as<uint32_t>(), 0u, c.n);
+    if (e != hipSuccess) return e;
+    e = launch_fill_u64(c.stream, c.red.as<uint64_t>() + 8, 0ull, 1);
+    if (e != hipSuccess) return e;
+    if (c.L > 0)
+        hipLaunchKernelGGL(k_link_count, dim3(blocks_for(c.L)), dim3(TPB), 0, c.stream, c.la.as<int32_t>(),
+                           c.lb.as<int32_t>(), c.L, c.n, c.linkCount.as<uint32_t>(), c.red.as<uint64_t>() + 8);
+    return hipGetLastError();
+}
+
+hipError_t launch_link_fill(Context &c) {
+    hipError_t e = launch_fill_u32(c.stream, c.linkCursor.as<uint32_t>(), 0u, c.n);
+    if (e != hipSuccess) return e;
+    if (c.L > 0)
+        hipLaunchKernelGGL(k_link_fill, dim3(blocks_for(c.L)), dim3(TPB), 0, c.stream, c.la.as<int32_t>(),
+                           c.lb.as<int32_t>(), c.L, c.maxLinks, c.linkCursor.as<uint32_t>(), c.slotLink.as<int32_t>());
+    hipLaunchKernelGGL(k_link_sort, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.la.as<int32_t>(),
+                       c.lb.as<int32_t>(), c.n, c.maxLinks, c.linkCount.as<uint32_t>(), c.slotLink.as<int32_t>(),
+                       c.slotOther.as<int32_t>(), c.sidePos.as<int32_t>());
+    return hipGetLastError();
+}
+
+hipError_t launch_ct_import(Context &c) {
+    int64_t nSides = 2 * c.L;
+    if (nSides == 0) return hipSuccess;
+    const double *s[6];
+    for (int k = 0; k < 6; k++) s[k] = c.ctStage[k].as<double>();
+    if (c.prec == HDM_PREC_FP64) {
+        double *d[6];
+        for (int k = 0; k < 6; k++) d[k] = c.ct[k].as<double>();
+        hipLaunchKernelGGL(k_ct_import<double>, dim3(blocks_for(nSides)), dim3(TPB), 0, c.stream,
+                           c.sidePos.as<int32_t>(), nSides, s[0], s[1], s[2], s[3], s[4], s[5], d[0], d[1], d[2],
+                           d[3], d[4], d[5]);
+    } else {
+        float *d[6];
+        for (int k = 0; k < 6; k++) d[k] = c.ct[k].as<float>();
+        hipLaunchKernelGGL(k_ct_import<float>, dim3(blocks_for(nSides)), dim3(TPB), 0, c.stream,
+                           c.sidePos.as<int32_t>(), nSides, s[0], s[1], s[2], s[3], s[4], s[5], d[0], d[1], d[2],
+                           d[3], d[4], d[5]);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ct_export(Context &c) {
+    int64_t nSides = 2 * c.L;
+    if (nSides == 0) return hipSuccess;
+    double *s[6];
+    for (int k = 0; k < 6; k++) s[k] = c.ctStage[k].as<double>();
+    if (c.prec == HDM_PREC_FP64) {
+        const double *d[6];
+        for (int k = 0; k < 6; k++) d[k] = c.ct[k].as<double>();
+        hipLaunchKernelGGL(k_ct_export<double>, dim3(blocks_for(nSides)), dim3(TPB), 0, c.stream,
+                           c.sidePos.as<int32_t>(), nSides, d[0], d[1], d[2], d[3], d[4], d[5], s[0], s[1], s[2],
+                           s[3], s[4], s[5]);
+    } else {
+        const float *d[6];
+        for (int k = 0; k < 6; k++) d[k] = c.ct[k].as<float>();
+        hipLaunchKernelGGL(k_ct_export<float>, dim3(blocks_for(nSides)), dim3(TPB), 0, c.stream,
+                           c.sidePos.as<int32_t>(), nSides, d[0], d[1], d[2], d[3], d[4], d[5], s[0], s[1], s[2],
+                           s[3], s[4], s[5]);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_count_broken(Context &c, uint64_t *counter) {
+    hipError_t e = launch_fill_u64(c.stream, counter, 0ull, 1);
+    if (e != hipSuccess) return e;
+    if (c.L == 0) return hipSuccess;
+    int64_t nb = blocks_for(c.L);
+    int cap = c.numCU > 0 ? c.numCU * 4 : 1024;
+    int grid = (int)(nb < cap ? nb : cap);
+    hipLaunchKernelGGL(k_count_broken, dim3(grid), dim3(TPB), 0, c.stream, c.state.as<uint8_t>(), c.L, counter);
+    return hipGetLastError();
+}
+
+}  // namespace hdm

@@ -1,0 +1,506 @@
+// Per-timestep kernels for gfx950: force gather, time integration, link
+// break and the rebuild (skin) reductions.
+//
+// Force gather (CombinedAccel::iterateCombined, CombinedAccel.h:121-197):
+// one lane per particle, 64 consecutive particles per wavefront.  A lane
+// zeroes its force, applies the contact law to every neighbour-list entry in
+// list order, then the beam law to each of its link sides in ascending link
+// id, updating that side's CT0..CT5 in place, and finally overwrites its
+// force.  Each particle owns its sides, so there are no atomics and the
+// summation order is fixed: results are deterministic and, in the FP64 form,
+// bit-identical to the sequential CPU order (this translation unit is built
+// with -ffp-contract=off; FP64 sqrt and division are correctly rounded).
+//
+// The neighbour list and the link sides live in wave tiles: slot k of the 64
+// particles of a tile is one contiguous run, so every list/CT access of a
+// wavefront is a single coalesced 256 B / 512 B transaction.  Partner state
+// (coordinates, Diff, radius) is gathered through L2; particles are in
+// lattice order and blocks are renumbered per XCD so partners of a tile were
+// usually fetched by a neighbouring tile on the same XCD.
+#include "hdm_context.h"
+#include "kernel_common.h"
+
+namespace hdm {
+
+namespace {
+
+constexpr int TPB = 256;
+
+template <class R>
+struct ForceParams {
+    R invDt, attractDistance, attractScaleForce, damping, dampingRotation, scaleE;
+    R E, W2, W4, G;
+};
+
+template <class R>
+ForceParams<R> cast_params(const hdm_force_params &f) {
+    return ForceParams<R>{(R)f.invDt, (R)f.attractDistance, (R)f.attractScaleForce, (R)f.damping,
+                          (R)f.dampingRotation, (R)f.scaleE, (R)f.E, (R)f.W2, (R)f.W4, (R)f.G};
+}
+
+// HiDEMDefNeighForceFunc::operator() (HiDEMDefaultFunctions.h:38-86).
+template <class R>
+__device__ __forceinline__ void contact_law(R mask, R xDist, R yDist, R zDist, const R self[7], const R other[7],
+                                            const ForceParams<R> &fp, R F[6]) {
+    R xDist2 = xDist * xDist;
+    R yDist2 = yDist * yDist;
+    R zDist2 = zDist * zDist;
+    R particleDistance2 = xDist2 + yDist2 + zDist2;
+    R radSum = self[6] + other[6];
+    R activeDist2 = (radSum + fp.attractDistance) * (radSum + fp.attractDistance);
+    mask = (particleDistance2 >= activeDist2) ? R(0) : mask;
+    R particleDistance = sqrt(particleDistance2);
+    R invDist = R(1) / particleDistance;
+    R xComp = xDist * invDist;
+    R yComp = yDist * invDist;
+    R zComp = zDist * invDist;
+    R Cnst = 0;
+    R dimDist = radSum - particleDistance;
+    Cnst = (particleDistance < radSum) ? R(sqrt(dimDist * dimDist * dimDist)) : Cnst;
+    Cnst = (particleDistance >= radSum && particleDistance < radSum + fp.attractDistance)
+               ? fp.attractScaleForce * dimDist : Cnst;
+    Cnst *= fp.scaleE;
+    F[0] += mask * ((-fp.damping) * ((self[0] - other[0]) * fp.invDt) + (Cnst * xComp));
+    F[1] += mask * ((-fp.damping) * ((self[1] - other[1]) * fp.invDt) + (Cnst * yComp));
+    F[2] += mask * ((-fp.damping) * ((self[2] - other[2]) * fp.invDt) + (Cnst * zComp));
+    F[3] += mask * ((-fp.dampingRotation) * ((self[3] - other[3]) * fp.invDt));
+    F[4] += mask * ((-fp.dampingRotation) * ((self[4] - other[4]) * fp.invDt));
+    F[5] += mask * ((-fp.dampingRotation) * ((self[5] - other[5]) * fp.invDt));
+}
+
+// HiDEMDefLinkForceFunc::operator() (HiDEMDefaultFunctions.h:105-203).
+template <class R>
+__device__ __forceinline__ void beam_law(R mask, R xDist, R yDist, R zDist, const R s[7], const R o[7],
+                                         const ForceParams<R> &fp, R ct[6], R F[6]) {
+    F[0] += (-fp.damping * ((s[0] - o[0]) * fp.invDt)) * mask;
+    F[1] += (-fp.damping * ((s[1] - o[1]) * fp.invDt)) * mask;
+    F[2] += (-fp.damping * ((s[2] - o[2]) * fp.invDt)) * mask;
+    F[3] += (-fp.dampingRotation * ((s[3] - o[3]) * fp.invDt)) * mask;
+    F[4] += (-fp.dampingRotation * ((s[4] - o[4]) * fp.invDt)) * mask;
+    F[5] += (-fp.dampingRotation * ((s[5] - o[5]) * fp.invDt)) * mask;
+
+    R xDist2 = xDist * xDist;
+    R yDist2 = yDist * yDist;
+    if (xDist == R(0) && yDist == R(0)) {
+        xDist2 = R(1e-10);
+        yDist2 = R(1e-10);
+    }
+    R zDist2 = zDist * zDist;
+    R distSum = xDist2 + yDist2 + zDist2;
+    R invLI = R(1) / sqrt(distSum);
+    R invLP = R(1) / sqrt(xDist2 + yDist2);
+    R invLILP = invLI * invLP;
+    R LI = invLI * distSum;
+
+    R T00 = xDist * invLI, T01 = yDist * invLI, T02 = zDist * invLI;
+    R T10 = -yDist * invLP, T11 = xDist * invLP;
+    R T20 = -(zDist * xDist) * invLILP;
+    R T21 = -(zDist * yDist) * invLILP;
+    R T22 = (xDist2 + yDist2) * invLILP;
+
+    R B0 = T00 * s[0] + T01 * s[1] + T02 * s[2];
+    R B1 = T10 * s[0] + T11 * s[1];
+    R B2 = T20 * s[0] + T21 * s[1] + T22 * s[2];
+    R B4 = T10 * s[3] + T11 * s[4];
+    R B5 = T20 * s[3] + T21 * s[4] + T22 * s[5];
+    R B6 = T00 * o[0] + T01 * o[1] + T02 * o[2];
+    R B7 = T10 * o[0] + T11 * o[1];
+    R B8 = T20 * o[0] + T21 * o[1] + T22 * o[2];
+    R B10 = T10 * o[3] + T11 * o[4];
+    R B11 = T20 * o[3] + T21 * o[4] + T22 * o[5];
+
+    R L2 = LI * LI;
+    R L3 = LI * LI * LI;
+    R LIDiv6 = (R(1) / R(6)) * invLI;
+
+    R K0 = fp.E * fp.W2 * invLI;
+    R K1 = fp.E * fp.W4 / L3;
+    R K5 = fp.E * fp.W4 / (R(2) * L2);
+
+    ct[0] += K0 * B0 + (-K0) * B6;
+    ct[1] += K1 * B1 + K5 * B5 + (-K1) * B7 + K5 * B11;
+    ct[2] += K1 * B2 + (-K5) * B4 + (-K1) * B8 + (-K5) * B10;
+
+    F[0] += (-(T00 * ct[0] + T10 * ct[1] + T20 * ct[2])) * mask;
+    F[1] += (-(T01 * ct[0] + T11 * ct[1] + T21 * ct[2])) * mask;
+    F[2] += (-(T02 * ct[0] + T22 * ct[2])) * mask;
+
+    R K2 = fp.G * fp.W4 * LIDiv6;
+    R K3 = fp.E * fp.W4 * LIDiv6 * R(2);
+    R K4 = fp.E * fp.W4 * LIDiv6;
+
+    R B3 = T00 * s[3] + T01 * s[4] + T02 * s[5];
+    R B9 = T00 * o[3] + T01 * o[4] + T02 * o[5];
+
+    ct[3] += K2 * B3 + (-K2) * B9;
+    ct[4] += (-K5) * B2 + K3 * B4 + K5 * B8 + K4 * B10;
+    ct[5] += K5 * B1 + K3 * B5 + (-K5) * B7 + K4 * B11;
+
+    F[3] += (-(T00 * ct[3] + T10 * ct[4] + T20 * ct[5])) * mask;
+    F[4] += (-(T01 * ct[3] + T11 * ct[4] + T21 * ct[5])) * mask;
+    F[5] += (-(T02 * ct[3] + T22 * ct[5])) * mask;
+}
+
+template <class R, class CT, class FO>
+struct ForceArgs {
+    const double *x, *y, *z, *d0, *d1, *d2, *d3, *d4, *d5, *radius;
+    int64_t n;
+    const int32_t *nbrCount, *nbr;
+    int32_t nbrWidth;
+    const uint32_t *linkCount;
+    const int32_t *slotLink, *slotOther;
+    int32_t linkWidth;
+    const uint8_t *state;
+    CT *c0, *c1, *c2, *c3, *c4, *c5;
+    FO *f0, *f1, *f2, *f3, *f4, *f5;
+    ForceParams<R> fp;
+};
+
+template <class R, class CT, class FO>
+__global__ void __launch_bounds__(TPB) k_force(ForceArgs<R, CT, FO> a) {
+    const int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    if (p >= a.n) return;
+    const double xCord = a.x[p], yCord = a.y[p], zCord = a.z[p];
+    R self[7] = {(R)a.d0[p], (R)a.d1[p], (R)a.d2[p], (R)a.d3[p], (R)a.d4[p], (R)a.d5[p], (R)a.radius[p]};
+    R F[6] = {0, 0, 0, 0, 0, 0};
+    const int64_t lane = p & 63;
+    {
+        const int32_t cnt = a.nbrCount[p];
+        const int32_t *nl = a.nbr + (p >> 6) * (int64_t)64 * a.nbrWidth + lane;
+        for (int32_t k = 0; k < cnt; k++) {
+            const int32_t q = nl[(int64_t)k * 64];
+            R other[7] = {(R)a.d0[q], (R)a.d1[q], (R)a.d2[q], (R)a.d3[q], (R)a.d4[q], (R)a.d5[q], (R)a.radius[q]};
+            R xDist = (R)(xCord - a.x[q]);
+            R yDist = (R)(yCord - a.y[q]);
+            R zDist = (R)(zCord - a.z[q]);
+            contact_law<R>(R(1), xDist, yDist, zDist, self, other, a.fp, F);
+        }
+    }
+    {
+        const int32_t cnt = (int32_t)a.linkCount[p];
+        const int64_t base = (p >> 6) * (int64_t)64 * a.linkWidth + lane;
+        for (int32_t k = 0; k < cnt; k++) {
+            const int64_t sidx = base + (int64_t)k * 64;
+            const int32_t l = a.slotLink[sidx];
+            const int32_t q = a.slotOther[sidx];
+            R other[7] = {(R)a.d0[q], (R)a.d1[q], (R)a.d2[q], (R)a.d3[q], (R)a.d4[q], (R)a.d5[q], (R)a.radius[q]};
+            R xDist = (R)(xCord - a.x[q]);
+            R yDist = (R)(yCord - a.y[q]);
+            R zDist = (R)(zCord - a.z[q]);
+            R mask = 1;
+            if (a.state[l] == 0) mask = 0;
+            R ct[6] = {(R)a.c0[sidx], (R)a.c1[sidx], (R)a.c2[sidx], (R)a.c3[sidx], (R)a.c4[sidx], (R)a.c5[sidx]};
+            beam_law<R>(mask, xDist, yDist, zDist, self, other, a.fp, ct, F);
+            a.c0[sidx] = (CT)ct[0];
+            a.c1[sidx] = (CT)ct[1];
+            a.c2[sidx] = (CT)ct[2];
+            a.c3[sidx] = (CT)ct[3];
+            a.c4[sidx] = (CT)ct[4];
+            a.c5[sidx] = (CT)ct[5];
+        }
+    }
+    a.f0[p] = (FO)F[0];
+    a.f1[p] = (FO)F[1];
+    a.f2[p] = (FO)F[2];
+    a.f3[p] = (FO)F[3];
+    a.f4[p] = (FO)F[4];
+    a.f5[p] = (FO)F[5];
+}
+
+// ---- time integration --------------------------------------------------------
+template <class FO>
+struct IntegArgs {
+    double *x, *y, *z, *tx, *ty, *tz, *d0, *d1, *d2, *d3, *d4, *d5;
+    const double *radius;
+    const FO *f0, *f1, *f2, *f3, *f4, *f5;
+    const uint8_t *type;
+    int64_t n;
+    hdm_integ_params ip;
+};
+
+// Damped central difference (HiDEMDefTimeIntegration, HiDEMDefaultFunctions.h
+// :317-370; same expressions as the inline IceBlock form).
+__device__ __forceinline__ void central_difference(double dragCoeff, double mass, double inertia,
+                                                   const hdm_integ_params &ip, const double F[6],
+                                                   const double diff[6], double cord[3], double cordPrev[3],
+                                                   double theta[3], double thetaPrev[3], double vel[6]) {
+    double dragInvDt2 = dragCoeff * ip.invDt2;
+    double massDtSqr = mass * ip.invDtSqr;
+    double pMassDtSqr = mass / ip.dtSqr;
+    double pMemDtSqr = inertia / ip.dtSqr;
+    double invC1 = 1.0 / (pMassDtSqr + dragCoeff * ip.invDt2);
+    double invC2 = 1.0 / (pMemDtSqr + dragCoeff * ip.invDt2);
+    double interDtSqr = inertia * ip.invDtSqr;
+    double next[6];
+#pragma unroll
+    for (int c = 0; c < 3; c++) next[c] = (F[c] + massDtSqr * diff[c] + dragInvDt2 * -diff[c]) * invC1;
+#pragma unroll
+    for (int c = 3; c < 6; c++) next[c] = (F[c] + interDtSqr * diff[c] + dragInvDt2 * -diff[c]) * invC2;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        cordPrev[c] = cord[c];
+        thetaPrev[c] = theta[c];
+        cord[c] = cord[c] + next[c];
+        theta[c] = theta[c] + next[3 + c];
+        vel[c] = next[c] * ip.invDt;
+        vel[3 + c] = (theta[c] - thetaPrev[c]) * ip.invDt;
+    }
+}
+
+// applyForce: IceBlock (MainIceBlock.cpp:299-400, clamp = HDM_TYPE_FIXED) or
+// GlacierDem (MainDem.cpp:145-259 with an analytic bed, constant external
+// force, HiDEMDefVelocityLimit and HiDEMDefBoundaryCondition).
+template <class FO>
+__global__ void __launch_bounds__(TPB) k_integrate(IntegArgs<FO> a) {
+    const int64_t i = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    if (i >= a.n) return;
+    const hdm_integ_params &ip = a.ip;
+    if (ip.variant == HDM_INTEG_ICEBLOCK && (a.type[i] & HDM_TYPE_FIXED)) {
+        a.d0[i] = 0; a.d1[i] = 0; a.d2[i] = 0;
+        a.d3[i] = 0; a.d4[i] = 0; a.d5[i] = 0;
+        return;
+    }
+    double cord[3] = {a.x[i], a.y[i], a.z[i]};
+    double theta[3] = {a.tx[i], a.ty[i], a.tz[i]};
+    double diff[6] = {a.d0[i], a.d1[i], a.d2[i], a.d3[i], a.d4[i], a.d5[i]};
+    double F[6] = {(double)a.f0[i], (double)a.f1[i], (double)a.f2[i], (double)a.f3[i], (double)a.f4[i], (double)a.f5[i]};
+    double cordPrev[3], thetaPrev[3], vel[6];
+    const double radius = a.radius[i];
+    const double diameter = radius * 2.0;
+    const double zCord = cord[2];
+    const double particleMass = ip.rhoIce * diameter * diameter * diameter;
+    const double particleMomentInertia = particleMass * diameter * diameter / 6.0;
+    double dragCoeff;
+    if (ip.variant == HDM_INTEG_ICEBLOCK) {
+        dragCoeff = (zCord > ip.waterLevel) ? ip.airDrag : ip.waterDrag;
+        if (zCord > ip.waterLevel) F[2] += -ip.g * particleMass;
+        if (zCord <= ip.waterLevel) F[2] += ip.lift * particleMass;
+        central_difference(dragCoeff, particleMass, particleMomentInertia, ip, F, diff, cord, cordPrev, theta,
+                           thetaPrev, vel);
+    } else {
+        const double bedzCord = (cord[0] < ip.bedXEnd) ? ip.bedZ : ip.deepZ;
+        const double fric = (cord[0] < ip.frictionXEnd) ? 1.0 : 0.0;
+        if ((zCord + radius) - bedzCord < ip.bedInteract) {
+            double d = ip.bedDrag * fric;
+            dragCoeff = (ip.frictionCap < d) ? ip.frictionCap : d;
+        } else if (zCord > ip.waterLevel) {
+            dragCoeff = ip.airDrag;
+        } else {
+            dragCoeff = ip.waterDrag;
+        }
+        if (zCord > ip.waterLevel) F[2] += -ip.g * particleMass;
+        if (zCord <= ip.waterLevel) F[2] += ip.lift * particleMass;
+        if (zCord + radius < bedzCord) F[2] += ip.bedStiffness * (bedzCord - (zCord + radius));
+#pragma unroll
+        for (int c = 0; c < 6; c++) F[c] += ip.ext[c];
+        central_difference(dragCoeff, particleMass, particleMomentInertia, ip, F, diff, cord, cordPrev, theta,
+                           thetaPrev, vel);
+        if (ip.velocityLimit) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                if (vel[c] > ip.maxVel) cord[c] = ip.dt * ip.maxVel + cordPrev[c];
+                if (vel[3 + c] > ip.maxRot) theta[c] = ip.dt * ip.maxRot + thetaPrev[c];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                if (vel[c] < -ip.maxVel) cord[c] = ip.dt * -ip.maxVel + cordPrev[c];
+                if (vel[3 + c] < -ip.maxRot) theta[c] = ip.dt * -ip.maxRot + thetaPrev[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            if (cord[c] + radius < ip.lower[c] && ip.bound[2 * c]) cord[c] = cordPrev[c];
+            if (cord[c] - radius > ip.upper[c] && ip.bound[2 * c + 1]) cord[c] = cordPrev[c];
+        }
+    }
+    a.x[i] = cord[0];
+    a.y[i] = cord[1];
+    a.z[i] = cord[2];
+    a.tx[i] = theta[0];
+    a.ty[i] = theta[1];
+    a.tz[i] = theta[2];
+    a.d0[i] = cord[0] - cordPrev[0];
+    a.d1[i] = cord[1] - cordPrev[1];
+    a.d2[i] = cord[2] - cordPrev[2];
+    a.d3[i] = theta[0] - thetaPrev[0];
+    a.d4[i] = theta[1] - thetaPrev[1];
+    a.d5[i] = theta[2] - thetaPrev[2];
+}
+
+// ---- link break --------------------------------------------------------------
+struct BreakArgs {
+    const double *x, *y, *z, *tx, *ty, *tz;
+    const int32_t *A, *B;
+    uint8_t *state;
+    const double *initialLen, *maxLoad;
+    int64_t L;
+    double bendBreak, scalarMaxLoad;
+    uint64_t *counter;
+};
+
+// CombinedAccel::iterateLinks (CombinedAccel.h:200-235) applying
+// HiDEMDefLinkBreakFunc (HiDEMDefaultFunctions.h:217-241); the strain is FP64
+// in every precision form.
+__global__ void __launch_bounds__(TPB) k_break(BreakArgs a) {
+    const int64_t l = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    uint32_t broke = 0;
+    if (l < a.L && a.state[l] != 0) {
+        const int32_t i = a.A[l], j = a.B[l];
+        double xDist = a.x[i] - a.x[j];
+        double yDist = a.y[i] - a.y[j];
+        double zDist = a.z[i] - a.z[j];
+        double stretchLength = sqrt((xDist * xDist) + (yDist * yDist) + (zDist * zDist));
+        double bx = fabs(a.tx[i] - a.tx[j]);
+        double by = fabs(a.ty[i] - a.ty[j]);
+        double bz = fabs(a.tz[i] - a.tz[j]);
+        double inner = (by < bz) ? bz : by;
+        double maxBend = (bx < inner) ? inner : bx;
+        double initialLen = a.initialLen[l];
+        double load = (stretchLength - initialLen) / initialLen + a.bendBreak * maxBend;
+        double maxLoad = a.maxLoad ? a.maxLoad[l] : a.scalarMaxLoad;
+        if (load >= maxLoad) {
+            a.state[l] = 0;
+            broke = 1;
+        }
+    }
+    broke = wave_sum_u32(broke);
+    if ((threadIdx.x & 63) == 0 && broke && a.counter)
+        atomicAdd((unsigned long long *)a.counter, (unsigned long long)broke);
+}
+
+// ---- skin reductions -----------------------------------------------------------
+// HiDEMLib::getMaxDistOwned (HiDEMLib.h:424-461): max over particles of the
+// signed Diff, starting from 0.0.
+__global__ void __launch_bounds__(TPB) k_skin_heuristic(const double *__restrict__ dx, const double *__restrict__ dy,
+                                                         const double *__restrict__ dz, int64_t n, uint64_t *out) {
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+        double v[3] = {dx[i], dy[i], dz[i]};
+#pragma unroll
+        for (int c = 0; c < 3; c++) m[c] = v[c] > m[c] ? v[c] : m[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double w = wave_max(m[c]);
+        if ((threadIdx.x & 63) == 0) atomicMax((unsigned long long *)&out[c], (unsigned long long)ordered_key(w));
+    }
+}
+
+// Exact test: max over particles of |x - x0|^2 (FP64).
+__global__ void __launch_bounds__(TPB) k_skin_exact(const double *__restrict__ x, const double *__restrict__ y,
+                                                     const double *__restrict__ z, const double *__restrict__ x0,
+                                                     const double *__restrict__ y0, const double *__restrict__ z0,
+                                                     int64_t n, uint64_t *out) {
+    double m2 = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+        double ddx = x[i] - x0[i], ddy = y[i] - y0[i], ddz = z[i] - z0[i];
+        double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        m2 = d2 > m2 ? d2 : m2;
+    }
+    m2 = wave_max(m2);
+    if ((threadIdx.x & 63) == 0) atomicMax((unsigned long long *)out, (unsigned long long)ordered_key(m2));
+}
+
+inline int blocks_for(int64_t n) { return (int)((n + TPB - 1) / TPB); }
+inline int reduce_grid(const Context &c, int64_t n) {
+    int64_t nb = blocks_for(n);
+    int cap = c.numCU > 0 ? c.numCU * 4 : 1024;
+    int g = (int)(nb < cap ? nb : cap);
+    return g < 1 ? 1 : g;
+}
+
+template <class R, class CT, class FO>
+hipError_t force_impl(Context &c, const hdm_force_params &fp) {
+    ForceArgs<R, CT, FO> a;
+    a.x = c.x.as<double>(); a.y = c.y.as<double>(); a.z = c.z.as<double>();
+    a.d0 = c.diff[0].as<double>(); a.d1 = c.diff[1].as<double>(); a.d2 = c.diff[2].as<double>();
+    a.d3 = c.diff[3].as<double>(); a.d4 = c.diff[4].as<double>(); a.d5 = c.diff[5].as<double>();
+    a.radius = c.radius.as<double>();
+    a.n = c.n;
+    a.nbrCount = c.nbrCount.as<int32_t>(); a.nbr = c.nbr.as<int32_t>(); a.nbrWidth = c.maxNeigh;
+    a.linkCount = c.linkCount.as<uint32_t>(); a.slotLink = c.slotLink.as<int32_t>();
+    a.slotOther = c.slotOther.as<int32_t>(); a.linkWidth = c.maxLinks;
+    a.state = c.state.as<uint8_t>();
+    a.c0 = c.ct[0].as<CT>(); a.c1 = c.ct[1].as<CT>(); a.c2 = c.ct[2].as<CT>();
+    a.c3 = c.ct[3].as<CT>(); a.c4 = c.ct[4].as<CT>(); a.c5 = c.ct[5].as<CT>();
+    a.f0 = c.force[0].as<FO>(); a.f1 = c.force[1].as<FO>(); a.f2 = c.force[2].as<FO>();
+    a.f3 = c.force[3].as<FO>(); a.f4 = c.force[4].as<FO>(); a.f5 = c.force[5].as<FO>();
+    a.fp = cast_params<R>(fp);
+    hipLaunchKernelGGL((k_force<R, CT, FO>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, a);
+    return hipGetLastError();
+}
+
+template <class FO>
+hipError_t integrate_impl(Context &c, const hdm_integ_params &ip) {
+    IntegArgs<FO> a;
+    a.x = c.x.as<double>(); a.y = c.y.as<double>(); a.z = c.z.as<double>();
+    a.tx = c.tx.as<double>(); a.ty = c.ty.as<double>(); a.tz = c.tz.as<double>();
+    a.d0 = c.diff[0].as<double>(); a.d1 = c.diff[1].as<double>(); a.d2 = c.diff[2].as<double>();
+    a.d3 = c.diff[3].as<double>(); a.d4 = c.diff[4].as<double>(); a.d5 = c.diff[5].as<double>();
+    a.radius = c.radius.as<double>();
+    a.f0 = c.force[0].as<FO>(); a.f1 = c.force[1].as<FO>(); a.f2 = c.force[2].as<FO>();
+    a.f3 = c.force[3].as<FO>(); a.f4 = c.force[4].as<FO>(); a.f5 = c.force[5].as<FO>();
+    a.type = c.type.as<uint8_t>();
+    a.n = c.n;
+    a.ip = ip;
+    hipLaunchKernelGGL((k_integrate<FO>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_force(Context &c, const hdm_force_params &fp) {
+    if (c.n == 0) return hipSuccess;
+    if (c.prec == HDM_PREC_FP64) return force_impl<double, double, double>(c, fp);
+    return force_impl<float, float, float>(c, fp);
+}
+
+hipError_t launch_integrate(Context &c, const hdm_integ_params &ip) {
+    if (c.n == 0) return hipSuccess;
+    if (c.prec == HDM_PREC_FP64) return integrate_impl<double>(c, ip);
+    return integrate_impl<float>(c, ip);
+}
+
+hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter) {
+    if (c.L == 0) return hipSuccess;
+    BreakArgs a;
+    a.x = c.x.as<double>(); a.y = c.y.as<double>(); a.z = c.z.as<double>();
+    a.tx = c.tx.as<double>(); a.ty = c.ty.as<double>(); a.tz = c.tz.as<double>();
+    a.A = c.la.as<int32_t>(); a.B = c.lb.as<int32_t>();
+    a.state = c.state.as<uint8_t>();
+    a.initialLen = c.initialLen.as<double>();
+    a.maxLoad = c.hasMaxLoad ? c.maxLoad.as<double>() : nullptr;
+    a.L = c.L;
+    a.bendBreak = bp.bendBreak;
+    a.scalarMaxLoad = bp.maxLoad;
+    a.counter = counter;
+    hipLaunchKernelGGL(k_break, dim3(blocks_for(c.L)), dim3(TPB), 0, c.stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_skin_heuristic(Context &c, uint64_t *out3) {
+    hipError_t e = launch_fill_u64(c.stream, out3, ordered_key(0.0), 3);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_skin_heuristic, dim3(reduce_grid(c, c.n)), dim3(TPB), 0, c.stream, c.diff[0].as<double>(),
+                       c.diff[1].as<double>(), c.diff[2].as<double>(), c.n, out3);
+    return hipGetLastError();
+}
+
+hipError_t launch_skin_exact(Context &c, uint64_t *out1) {
+    hipError_t e = launch_fill_u64(c.stream, out1, ordered_key(0.0), 1);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_skin_exact, dim3(reduce_grid(c, c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
+                       c.y.as<double>(), c.z.as<double>(), c.x0.as<double>(), c.y0.as<double>(), c.z0.as<double>(),
+                       c.n, out1);
+    return hipGetLastError();
+}
+
+hipError_t launch_snapshot(Context &c) {
+    size_t b = sizeof(double) * (size_t)c.n;
+    hipError_t e = hipMemcpyAsync(c.x0.ptr, c.x.ptr, b, hipMemcpyDeviceToDevice, c.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c.y0.ptr, c.y.ptr, b, hipMemcpyDeviceToDevice, c.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c.z0.ptr, c.z.ptr, b, hipMemcpyDeviceToDevice, c.stream);
+    return e;
+}
+
+}  // namespace hdm

@@ -1,0 +1,242 @@
+/*
+ * hdm_abi.h -- C ABI of the MI355X (gfx950) HiDEM timestep hot path.
+ *
+ * Plain C: POD structs, raw pointers and sizes, int status codes.  A context
+ * (hdm_ctx) is the accelerator object that HiDEMLib selects at
+ * HiDEMLib.h:54-60 (CombinedAccel / CombinedAccelSmem); it owns device
+ * mirrors of the HiDEMParticleData / HiDEMLinkData fields the step touches,
+ * the neighbour grid, the neighbour list and the per-particle link lists.
+ * All work is ordered on one HIP stream owned by the context; no call
+ * performs an implicit device-wide synchronisation unless it returns data to
+ * the host.  A context is not re-entrant: one host thread at a time.
+ *
+ * Every function returns HDM_OK (0) or a negative HDM_E_* code; the message
+ * of the last failure is available from hdm_last_error().  The C++ wrapper
+ * (CombinedAccelHip.h) maps failures to the reference's convention (message,
+ * then exit(1); DevCall.h:69-91).
+ *
+ * Precision forms (hdm_ctx_create):
+ *   HDM_PREC_FP64   every quantity in FP64 (reference build with
+ *                   SET_FP32=OFF, SET_CT_FP32=OFF).
+ *   HDM_PREC_MIXED  contact/beam arithmetic, CT side state and forces in
+ *                   FP32; coordinates, rotations, per-step displacements
+ *                   (Diff), the integrator and the break strain in FP64.
+ * Host-side structs always carry FP64 values; the context converts.
+ */
+#ifndef HIDEM_HDM_ABI_H
+#define HIDEM_HDM_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HDM_ABI_VERSION 1
+
+enum {
+    HDM_OK = 0,
+    HDM_E_INVALID = -1,     /* bad argument or call order */
+    HDM_E_HIP = -2,         /* HIP runtime error */
+    HDM_E_NOMEM = -3,       /* device or host allocation failed */
+    HDM_E_NODEVICE = -4,    /* no usable gfx950 device */
+    HDM_E_CAPACITY = -5,    /* a size exceeds what the structures allow */
+    HDM_E_STATE = -6        /* data not uploaded / structures not built */
+};
+
+enum { HDM_PREC_FP64 = 0, HDM_PREC_MIXED = 1 };
+
+/* Particle type bits (ParticleData::type is uint8_t, ParticleData.h:31). */
+#define HDM_TYPE_FIXED 1u   /* clamped particle: never moves, Diff = 0 */
+
+typedef struct hdm_ctx hdm_ctx;
+
+/* Host view of the HiDEMParticleData fields on the hot path
+ * (HiDEMParticleData.h:20-39, ParticleData.h:29-37).  Any pointer may be NULL
+ * on download (field skipped); on upload only the force fields may be NULL
+ * (zero-filled).  Arrays hold n entries. */
+typedef struct hdm_particles_host {
+    int64_t n;
+    double *xCord, *yCord, *zCord;
+    double *xTheta, *yTheta, *zTheta;
+    double *xDiff, *yDiff, *zDiff, *uDiff, *vDiff, *wDiff;
+    double *radius;
+    double *xForce, *yForce, *zForce, *uForce, *vForce, *wForce;
+    uint8_t *type;
+} hdm_particles_host;
+
+/* Host view of HiDEMLinkData (HiDEMLinkData.h:24-31, LinkData.h:28-43).
+ * CT[c] holds 2n side entries in checkpoint order: side 2l is the
+ * localPartIndA end of link l, side 2l+1 the localPartIndB end (the order
+ * LinkDataHandler::appendInitLinks creates, LinkDataHandler.h:229-236).
+ * maxLoad is the per-link fracture threshold; NULL means "use the scalar
+ * maxLoad of hdm_break_params". */
+typedef struct hdm_links_host {
+    int64_t n;
+    int32_t *localPartIndA, *localPartIndB;
+    uint8_t *state;
+    double *initialLen;
+    double *maxLoad;
+    double *CT[6];
+} hdm_links_host;
+
+/* HiDEMDefNeighForceFunc (HiDEMDefaultFunctions.h:21-35) and
+ * HiDEMDefLinkForceFunc (:90-103) parameters. */
+typedef struct hdm_force_params {
+    double invDt, attractDistance, attractScaleForce, damping, dampingRotation, scaleE;
+    double E, W2, W4, G;
+} hdm_force_params;
+
+enum { HDM_INTEG_ICEBLOCK = 0, HDM_INTEG_GLACIER = 1 };
+
+/* applyForce integrator parameters.  HDM_INTEG_ICEBLOCK: MainIceBlock.cpp
+ * applyForce (:299-400) plus the clamp (HDM_TYPE_FIXED).  HDM_INTEG_GLACIER:
+ * MainDem.cpp applyForce (:145-259) with an analytic bed: height bedZ for
+ * x < bedXEnd else deepZ, friction 1 for x < frictionXEnd else 0, a constant
+ * external force ext[6], HiDEMDefVelocityLimit and HiDEMDefBoundaryCondition. */
+typedef struct hdm_integ_params {
+    int32_t variant;
+    double rhoIce, g, lift, waterLevel, airDrag, waterDrag;
+    double invDt, invDt2, dtSqr, invDtSqr, dt;
+    double bedDrag, bedInteract, bedStiffness, frictionCap;
+    double bedZ, bedXEnd, deepZ, frictionXEnd;
+    double ext[6];
+    double maxVel, maxRot;
+    int32_t velocityLimit;
+    int32_t bound[6];              /* xMin, xMax, yMin, yMax, zMin, zMax */
+    double lower[3], upper[3];
+} hdm_integ_params;
+
+/* HiDEMDefLinkBreakFunc (HiDEMDefaultFunctions.h:207-215). */
+typedef struct hdm_break_params {
+    double bendBreak;
+    double maxLoad;
+} hdm_break_params;
+
+typedef struct hdm_grid_info {
+    double spacing, xMin, yMin, zMin, xMax, yMax, zMax;
+    int32_t X, Y, Z;
+    int64_t totalCells;
+    int64_t occupiedCells;
+    int32_t maxPerCell;
+} hdm_grid_info;
+
+/* Rebuild policies for hdm_run (gridService, HiDEMLib.h:533-628). */
+enum {
+    HDM_REBUILD_HEURISTIC = 0,  /* reference accumulated-Diff estimate (HiDEMLib.h:548-559) */
+    HDM_REBUILD_EXACT = 1,      /* max |x - x_at_last_build| vs (interactDist - attractDistance)/2 */
+    HDM_REBUILD_INTERVAL = 2    /* every regenInterval steps ([grid] gridRegenInterval) */
+};
+
+typedef struct hdm_run_params {
+    hdm_force_params force;
+    hdm_integ_params integ;
+    hdm_break_params brk;
+    double searchDistance;      /* grid cell size */
+    double interactDist;        /* neighbour-list skin */
+    double dt, breakTime;
+    int32_t policy;
+    int32_t regenInterval;
+} hdm_run_params;
+
+typedef struct hdm_run_stats {
+    int64_t timestep;           /* current timestep after the call */
+    int64_t steps;              /* steps run by this call */
+    int64_t rebuilds;           /* grid + neighbour-list rebuilds in this call */
+    int64_t brokenTotal;        /* links with state == 0 after the call */
+    double seconds;             /* wall time of the call, device-synchronised */
+} hdm_run_stats;
+
+/* Per-kernel-class device time accumulated with HIP events on the context's
+ * stream since the last reset (milliseconds, and launch counts). */
+enum { HDM_T_FORCE = 0, HDM_T_INTEGRATE, HDM_T_BREAK, HDM_T_SKIN, HDM_T_GRID, HDM_T_NEIGH, HDM_T_LINKS, HDM_T_COUNT };
+
+/* ---- context ------------------------------------------------------------ */
+int  hdm_ctx_create(int32_t precision, int32_t device, hdm_ctx **out);
+int  hdm_ctx_destroy(hdm_ctx *ctx);
+const char *hdm_last_error(const hdm_ctx *ctx);
+int  hdm_abi_version(void);
+int  hdm_device_count(int32_t *count);
+int  hdm_sync(hdm_ctx *ctx);
+int  hdm_set_timing(hdm_ctx *ctx, int32_t enabled);
+int  hdm_kernel_times(hdm_ctx *ctx, double ms[HDM_T_COUNT], int64_t launches[HDM_T_COUNT]);
+int  hdm_reset_kernel_times(hdm_ctx *ctx);
+int  hdm_device_bytes(hdm_ctx *ctx, int64_t *bytes);
+
+/* ---- data (ParticleDataHandler / LinkDataHandler ownership) --------------- */
+int  hdm_particles_upload(hdm_ctx *ctx, const hdm_particles_host *p);
+int  hdm_particles_download(hdm_ctx *ctx, hdm_particles_host *p);
+int  hdm_links_upload(hdm_ctx *ctx, const hdm_links_host *l);
+int  hdm_links_download(hdm_ctx *ctx, hdm_links_host *l);
+
+/* ---- neighbour grid: NeighbourGrid::generate (NeighbourGrid.cpp:48-70) ---- */
+int  hdm_grid_build(hdm_ctx *ctx, double searchDistance);
+int  hdm_grid_info_get(hdm_ctx *ctx, hdm_grid_info *info);
+/* colours[n] (cell id per particle), iterationOrder[n] (pInd sorted by
+ * (colour, pInd)), either may be NULL. */
+int  hdm_grid_download(hdm_ctx *ctx, int64_t *colours, int32_t *iterationOrder);
+
+/* ---- neighbour list: NeighAccel::generateNeighbours (NeighAccel.h:25-57) -- */
+int  hdm_neighbours_build(hdm_ctx *ctx, double interactDist);
+/* CSR export: counts[n], starts[n+1], list[starts[n]] (capacity listCap). */
+int  hdm_neighbours_download(hdm_ctx *ctx, int32_t *counts, int64_t *starts, int32_t *list, int64_t listCap);
+int  hdm_neighbours_total(hdm_ctx *ctx, int64_t *total, int32_t *maxPerParticle);
+
+/* ---- per-particle link lists + side layout: LinkAccelSide::generateLinks
+ *      (LinkAccelSide.h:92-143) and sortSideData (:159-417) ---------------- */
+int  hdm_links_build(hdm_ctx *ctx);
+/* CSR export per particle in ascending link id: linkId, other endpoint, end
+ * (0 = this particle is localPartIndA, 1 = localPartIndB). */
+int  hdm_linklists_download(hdm_ctx *ctx, int32_t *counts, int64_t *starts, int32_t *linkId,
+                            int32_t *other, int32_t *end, int64_t cap);
+
+/* ---- per-step kernels ------------------------------------------------------ */
+/* computePartForce with HiDEMDefNeighForceFunc + HiDEMDefLinkForceFunc:
+ * CombinedAccel::iterateCombined (CombinedAccel.h:121-197). */
+int  hdm_force(hdm_ctx *ctx, const hdm_force_params *fp);
+/* applyForce (HiDEMLib.h:303-314) with the integrator of ip->variant. */
+int  hdm_integrate(hdm_ctx *ctx, const hdm_integ_params *ip);
+/* linkUpdate with HiDEMDefLinkBreakFunc: CombinedAccel::iterateLinks
+ * (CombinedAccel.h:200-235).  nBroken (may be NULL) receives the number of
+ * links broken by this call (reading it synchronises). */
+int  hdm_break(hdm_ctx *ctx, const hdm_break_params *bp, int64_t *nBroken);
+/* Reference rebuild estimate (HiDEMLib.h:424-474): stepSize*|max+ Diff|. */
+int  hdm_skin_heuristic(hdm_ctx *ctx, int32_t stepSize, double *dist);
+/* Exact test: max_i |x_i - x_i(at last snapshot)|. */
+int  hdm_skin_snapshot(hdm_ctx *ctx);
+int  hdm_skin_exact(hdm_ctx *ctx, double *maxDisp);
+
+/* ---- the timestep loop: nSteps of gridService + computePartForce +
+ *      applyForce + linkUpdate (when ts*dt > breakTime) + finalizeTs
+ *      (MainIceBlock.cpp:409-447).  rebuildLog (nSteps bytes, may be NULL)
+ *      receives 1 for steps that rebuilt the grid and neighbour list. ------- */
+int  hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats *stats);
+int  hdm_set_timestep(hdm_ctx *ctx, int64_t ts);
+int  hdm_count_broken(hdm_ctx *ctx, int64_t *broken);
+
+/* ---- seeded FCC block and link generator (setup; build-defined, replaces the
+ *      non-deterministic FCCParticleGenerator/LinkGenerator) ---------------- */
+typedef struct hdm_fcc_spec {
+    int32_t nx, ny, nz;         /* unit cells */
+    double a;                   /* lattice constant */
+    double origin[3];
+    double radius;
+    double jitter;              /* uniform +-jitter per coordinate */
+    uint64_t seed;
+    double clampXBelow;         /* particles with x < clampXBelow get HDM_TYPE_FIXED (use -inf to disable) */
+} hdm_fcc_spec;
+
+int  hdm_gen_fcc_count(const hdm_fcc_spec *spec, int64_t *nParticles);
+int  hdm_gen_fcc_fill(const hdm_fcc_spec *spec, hdm_particles_host *out);
+/* Links join every pair with dist <= (r_i + r_j) * linkGenScale
+ * (LinkGenerator.h:83-89); end A = the smaller global key (index), links in
+ * ascending (A, B); initialLen = dist; maxLoad = threshold*(1 + spread*U(-1,1))
+ * from a counter-based hash of (seed, A, B). */
+int  hdm_gen_links_count(const hdm_particles_host *p, double linkGenScale, int64_t *nLinks);
+int  hdm_gen_links_fill(const hdm_particles_host *p, double linkGenScale, double threshold, double spread,
+                        uint64_t seed, hdm_links_host *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

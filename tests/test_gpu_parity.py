@@ -1,0 +1,367 @@
+"""GPU parity: every kernel of the hot path against the CPU oracle on the same
+seeded inputs, through the C ABI.
+
+Bar: grid cells, iteration order, neighbour lists and link lists bit-exact;
+FP64 forces, CT, positions bit-identical (the kernels keep the oracle's
+operation order, no FMA contraction, correctly rounded sqrt/div) and in any
+case within 1e-12 relative; the broken-link set over 500 steps of the 16k
+block identical step by step.
+"""
+import numpy as np
+import pytest
+
+from hidem_amd import _abi as A
+from hidem_amd import params
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def cfg_inputs(name, cells=None, **hdm_overrides):
+    cfg = params.load_config(name)
+    cfg["hdm"].update(hdm_overrides)
+    return params.build_inputs(cfg, cells)
+
+
+def perturb(p, seed, diff_scale=1e-5, theta_scale=1e-4):
+    q = p.copy()
+    rng = np.random.default_rng(seed)
+    for f in ("xDiff", "yDiff", "zDiff", "uDiff", "vDiff", "wDiff"):
+        q.arrays[f][:] = rng.normal(scale=diff_scale, size=p.n)
+    for f in ("xTheta", "yTheta", "zTheta"):
+        q.arrays[f][:] = rng.normal(scale=theta_scale, size=p.n)
+    return q
+
+
+@pytest.fixture(scope="module")
+def block():
+    return cfg_inputs("iceblock_16k")
+
+
+@pytest.fixture(scope="module")
+def squeezed():
+    """FCC block squeezed to NN distance 0.97 m (< diameter): contacts active."""
+    cfg = params.load_config("iceblock_16k")
+    cfg["lattice"]["fccSpacing"] = 0.97 * 2 ** 0.5
+    cfg["lattice"]["linkGenScale"] = 1.05 * 0.97
+    cfg["hdm"]["cells"] = [12, 10, 8]
+    return params.build_inputs(cfg)
+
+
+def new_ctx(p, l, prec=A.HDM_PREC_FP64):
+    ctx = A.Context(prec)
+    ctx.upload(p, l)
+    return ctx
+
+
+def rel_err(a, b):
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+def assert_same(a, b, what, rtol=1e-12):
+    err = rel_err(a, b)
+    assert err <= rtol, f"{what}: relative error {err:.3e} > {rtol}"
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+# ---- grid / lists -------------------------------------------------------------
+@pytest.mark.parametrize("case", ["block", "squeezed"])
+def test_grid_bit_exact(case, request, orc):
+    s, p, l = request.getfixturevalue(case)
+    say("grid", case, p.n)
+    with new_ctx(p, l) as ctx:
+        ctx.grid_build(s.search_distance)
+        info = ctx.grid_info()
+        col, order = ctx.grid_download()
+    _, g = orc.grid_build(p, s.search_distance)
+    assert (info.X, info.Y, info.Z) == g["dims"]
+    assert (info.xMin, info.yMin, info.zMin) == g["mins"]
+    assert info.totalCells == g["totalCells"]
+    assert info.occupiedCells == g["nOcc"]
+    assert np.array_equal(col, g["colours"])
+    assert np.array_equal(order, g["order"])
+
+
+def test_grid_reference_cases(orc):
+    # NeighbourGridTester.cpp:56-76 on the GPU: shuffled 6^3 box at spacing 2
+    pts = [(x, y, z) for x in range(6) for y in range(6) for z in range(6)]
+    rng = np.random.default_rng(1)
+    pts = np.array([pts[i] for i in rng.permutation(len(pts))], dtype=np.float64)
+    p = A.Particles.zeros(len(pts))
+    for k, f in enumerate(("xCord", "yCord", "zCord")):
+        p.arrays[f][:] = pts[:, k]
+    p.arrays["radius"][:] = 0.5
+    say("grid reference case")
+    with new_ctx(p, None) as ctx:
+        ctx.grid_build(2.0)
+        info = ctx.grid_info()
+        col, order = ctx.grid_download()
+    assert info.totalCells == 27 and info.occupiedCells == 27 and info.maxPerCell == 8
+    _, g = orc.grid_build(p, 2.0)
+    assert np.array_equal(order, g["order"])
+
+
+@pytest.mark.parametrize("case", ["block", "squeezed"])
+def test_neighbours_bit_exact(case, request, orc):
+    s, p, l = request.getfixturevalue(case)
+    say("neighbours", case)
+    with new_ctx(p, l) as ctx:
+        ctx.grid_build(s.search_distance)
+        ctx.neighbours_build(s.interact_dist)
+        counts, starts, lst = ctx.neighbours_download()
+    oc, os_, ol = orc.neigh_build(p, s.search_distance, s.interact_dist)
+    assert np.array_equal(counts, oc)
+    assert np.array_equal(starts, os_)
+    assert np.array_equal(lst, ol)
+
+
+@pytest.mark.parametrize("case", ["block", "squeezed"])
+def test_linklists_bit_exact(case, request, orc):
+    s, p, l = request.getfixturevalue(case)
+    say("linklists", case)
+    with new_ctx(p, l) as ctx:
+        ctx.links_build()
+        counts, starts, lid, oth, end = ctx.linklists_download()
+    oc, os_, olid, ooth, oside = orc.link_lists(p.n, l)
+    assert np.array_equal(counts, oc)
+    assert np.array_equal(starts, os_)
+    assert np.array_equal(lid, olid)
+    assert np.array_equal(oth, ooth)
+    assert np.array_equal(end, oside & 1)
+
+
+def test_ct_roundtrip_through_side_layout():
+    s, p, l = cfg_inputs("iceblock_16k", cells=[6, 5, 4])
+    rng = np.random.default_rng(2)
+    l2 = l.copy()
+    for c in range(6):
+        l2.ct[c][:] = rng.normal(size=2 * l.n)
+    say("ct roundtrip")
+    with new_ctx(p, l2) as ctx:
+        ctx.links_build()
+        ctx.links_build()  # rebuilding the side layout is a pure permutation
+        back = ctx.download_links()
+    for c in range(6):
+        assert bits_equal(back.ct[c], l2.ct[c])
+
+
+# ---- force --------------------------------------------------------------------
+def gpu_force(s, p, l):
+    with new_ctx(p, l) as ctx:
+        ctx.grid_build(s.search_distance)
+        ctx.neighbours_build(s.interact_dist)
+        ctx.links_build()
+        ctx.force(s.force)
+        return ctx.download_particles(), ctx.download_links()
+
+
+def oracle_force(orc, s, p, l):
+    sim = orc.OracleSim(p, l, s)
+    sim.rebuild()
+    sim.force()
+    return sim.p, sim.l
+
+
+FORCE_FIELDS = ("xForce", "yForce", "zForce", "uForce", "vForce", "wForce")
+
+
+@pytest.mark.parametrize("case", ["block", "squeezed"])
+def test_force_fp64_parity(case, request, orc):
+    s, p0, l0 = request.getfixturevalue(case)
+    p = perturb(p0, 11)
+    l = l0.copy()
+    rng = np.random.default_rng(12)
+    for c in range(6):
+        l.ct[c][:] = rng.normal(scale=1e3, size=2 * l.n)
+    l.state[rng.random(l.n) < 0.1] = 0  # some broken links: mask path
+    say("force fp64", case)
+    gp, gl = gpu_force(s, p, l)
+    op, ol = oracle_force(orc, s, p, l)
+    for f in FORCE_FIELDS:
+        assert_same(gp[f], op[f], f)
+    for c in range(6):
+        assert_same(gl.ct[c], ol.ct[c], f"CT{c}")
+    nz = np.abs(op["xForce"]) > 0
+    assert nz.mean() > 0.9
+
+
+@pytest.mark.parametrize("case", ["block", "squeezed"])
+def test_force_fp64_bit_identical(case, request, orc):
+    s, p0, l0 = request.getfixturevalue(case)
+    p = perturb(p0, 21)
+    say("force fp64 bits", case)
+    gp, gl = gpu_force(s, p, l0)
+    op, ol = oracle_force(orc, s, p, l0)
+    for f in FORCE_FIELDS:
+        assert bits_equal(gp[f], op[f]), f
+    for c in range(6):
+        assert bits_equal(gl.ct[c], ol.ct[c]), f"CT{c}"
+
+
+def test_contacts_are_active_in_squeezed_case(squeezed, orc):
+    s, p, l = squeezed
+    op, _ = oracle_force(orc, s, p, l)
+    # undeformed links carry no force, so the force comes from the Hertz contacts
+    assert np.max(np.abs(op["xForce"])) > 1e5
+
+
+# ---- integrator ----------------------------------------------------------------
+@pytest.mark.parametrize("cfg_name", ["iceblock_16k", "glacier_8m"])
+def test_integrate_bit_identical(cfg_name, orc):
+    s, p0, l0 = cfg_inputs(cfg_name, cells=[10, 6, 5])
+    p = perturb(p0, 31, diff_scale=1e-4)
+    rng = np.random.default_rng(32)
+    for f in FORCE_FIELDS:
+        p.arrays[f][:] = rng.normal(scale=1e4, size=p.n)
+    if cfg_name == "glacier_8m":
+        p.arrays["zDiff"][:5] = -0.01  # beyond maxVel: velocity limit path
+    say("integrate", cfg_name)
+    with new_ctx(p, l0) as ctx:
+        ctx.integrate(s.integ)
+        gp = ctx.download_particles()
+    sim = orc.OracleSim(p, l0, s)
+    sim.integrate()
+    for f in A.PARTICLE_FIELDS:
+        assert bits_equal(gp[f], sim.p[f]), f
+
+
+# ---- break ---------------------------------------------------------------------
+def test_break_bit_identical(block, orc):
+    s, p0, l0 = block
+    p = p0.copy()
+    rng = np.random.default_rng(41)
+    p.arrays["xCord"][:] += rng.normal(scale=3e-4, size=p.n)
+    p.arrays["yTheta"][:] = rng.normal(scale=5e-3, size=p.n)
+    l = l0.copy()
+    l.state[rng.random(l.n) < 0.05] = 0
+    say("break")
+    with new_ctx(p, l) as ctx:
+        nb = ctx.brk(s.brk)
+        gl = ctx.download_links()
+    sim = orc.OracleSim(p, l, s)
+    onb = sim.brk()
+    assert nb == onb and 0 < nb < l.n
+    assert np.array_equal(gl.state, sim.l.state)
+
+
+def test_skin_reductions(block, orc):
+    s, p0, l0 = block
+    p = perturb(p0, 51)
+    say("skin")
+    with new_ctx(p, l0) as ctx:
+        h = ctx.skin_heuristic(10)
+        ctx.skin_snapshot()
+    import ctypes as C
+    ps = orc.particles_struct(p)
+    assert h == orc.lib().hdmo_skin_heuristic(C.byref(ps), 10)
+
+
+# ---- whole loop ----------------------------------------------------------------
+def step_by_step(ctx, rp, n):
+    """Runs n steps one call each; returns per-step rebuild flags and newly broken links."""
+    prev = ctx.download_links(with_max_load=False).state.copy()
+    rebuilt, broken = [], []
+    for step in range(n):
+        st, log = ctx.run(rp, 1, log_rebuilds=True)
+        rebuilt.append(int(log[0]))
+        state = ctx.download_links(with_max_load=False).state
+        newly = np.nonzero((prev != 0) & (state == 0))[0]
+        broken.extend((ctx_ts, k) for ctx_ts, k in zip([st.timestep - 1] * len(newly), newly.tolist()))
+        prev = state.copy()
+    return np.array(rebuilt, np.uint8), broken
+
+
+def run_pair(orc, s, p, l, n):
+    sim = orc.OracleSim(p, l, s)
+    rlog, blog = sim.run(n, log=True)
+    with new_ctx(p, l) as ctx:
+        grl, gbroken = step_by_step(ctx, s.run, n)
+        gp = ctx.download_particles()
+        gl = ctx.download_links()
+    return sim, rlog, blog, gp, gl, grl, gbroken
+
+
+def test_block_500_steps_broken_set_identical(orc):
+    # the 16k block exactly as configured; it starts to fracture at the clamp
+    # after ~220 steps, so the comparison covers several hundred break events
+    s, p, l = cfg_inputs("iceblock_16k")
+    say("block 500 steps")
+    sim, rlog, blog, gp, gl, grl, gbroken = run_pair(orc, s, p, l, 500)
+    assert np.array_equal(grl, rlog)
+    assert [tuple(x) for x in blog.tolist()] == [tuple(x) for x in gbroken]
+    assert len(gbroken) > 0
+    assert np.array_equal(gl.state, sim.l.state)
+    for f in ("xCord", "yCord", "zCord", "xTheta", "yTheta", "zTheta", "xDiff", "zDiff"):
+        assert bits_equal(gp[f], sim.p[f]), f
+    for c in range(6):
+        assert bits_equal(gl.ct[c], sim.l.ct[c]), f"CT{c}"
+
+
+def test_weak_block_500_steps_batched(orc):
+    # thresholds / 4: fracture from step ~70 on; 500 steps in one hdm_run call
+    s, p, l = cfg_inputs("iceblock_16k")
+    l.maxLoad[:] *= 0.25
+    say("weak block 500 steps")
+    sim = orc.OracleSim(p, l, s)
+    sim.run(500)
+    with new_ctx(p, l) as ctx:
+        st = ctx.run(s.run, 500)
+        gp = ctx.download_particles()
+        gl = ctx.download_links()
+    assert st.rebuilds == sim.rebuilds
+    assert np.array_equal(gl.state, sim.l.state)
+    for f in ("xCord", "yCord", "zCord"):
+        assert bits_equal(gp[f], sim.p[f]), f
+
+
+@pytest.mark.parametrize("policy", [A.HDM_REBUILD_HEURISTIC, A.HDM_REBUILD_INTERVAL])
+def test_rebuild_policies_match(orc, policy):
+    s, p, l = cfg_inputs("iceblock_16k", cells=[8, 8, 6])
+    s.run.policy = policy
+    s.run.regenInterval = 7 if policy == A.HDM_REBUILD_INTERVAL else -1
+    l.maxLoad[:] *= 0.25
+    say("policy", policy)
+    sim, rlog, blog, gp, gl, grl, gbroken = run_pair(orc, s, p, l, 120)
+    assert np.array_equal(grl, rlog)
+    assert [tuple(x) for x in blog.tolist()] == [tuple(x) for x in gbroken]
+    assert bits_equal(gp["zCord"], sim.p["zCord"])
+
+
+def test_glacier_slab_300_steps(orc):
+    s, p, l = cfg_inputs("glacier_8m", cells=[40, 6, 8], bedXEnd=25.0, frictionXEnd=25.0)
+    say("glacier small 300 steps")
+    sim = orc.OracleSim(p, l, s)
+    rlog, blog = sim.run(300, log=True)
+    with new_ctx(p, l) as ctx:
+        st, grl = ctx.run(s.run, 300, log_rebuilds=True)
+        gp = ctx.download_particles()
+        gl = ctx.download_links()
+    assert np.array_equal(grl, rlog)
+    assert np.array_equal(gl.state, sim.l.state)
+    for f in ("xCord", "yCord", "zCord", "xTheta", "yTheta", "zTheta"):
+        assert bits_equal(gp[f], sim.p[f]), f
+
+
+# ---- mixed precision -----------------------------------------------------------
+def test_mixed_precision_error_vs_fp64(orc):
+    s, p, l = cfg_inputs("iceblock_16k")
+    say("mixed vs fp64")
+    sim = orc.OracleSim(p, l, s)
+    sim.run(200)
+    with new_ctx(p, l, A.HDM_PREC_MIXED) as ctx:
+        ctx.run(s.run, 200)
+        gp = ctx.download_particles()
+    ref = np.stack([sim.p["xCord"], sim.p["yCord"], sim.p["zCord"]])
+    got = np.stack([gp["xCord"], gp["yCord"], gp["zCord"]])
+    disp = ref - np.stack([p["xCord"], p["yCord"], p["zCord"]])
+    err = np.max(np.abs(got - ref))
+    say("mixed max |dx| =", err, "max displacement =", np.max(np.abs(disp)))
+    assert err / np.max(np.abs(ref)) <= 1e-5
+    assert err <= 1e-3 * np.max(np.abs(disp))
