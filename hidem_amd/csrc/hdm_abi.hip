@@ -559,6 +559,7 @@ int hdm_integrate(hdm_ctx *ctx, const hdm_integ_params *ip) {
 int hdm_break(hdm_ctx *ctx, const hdm_break_params *bp, int64_t *nBroken) {
     if (!valid(ctx) || !bp) return HDM_E_INVALID;
     Context &c = ctx->c;
+    if (!c.linksBuilt) return fail(ctx, HDM_E_STATE, "link lists not built (link end indices unchecked)");
     uint64_t *counter = c.red.as<uint64_t>() + 16;
     if (nBroken) HIP_CHECK(ctx, hdm::launch_fill_u64(c.stream, counter, 0ull, 1));
     {

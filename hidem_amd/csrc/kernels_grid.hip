@@ -166,11 +166,13 @@ __global__ void __launch_bounds__(TPB) k_scan_apply(const int32_t *__restrict__ 
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == TPB - 1) out[count] = ex;
 }
 
-__global__ void __launch_bounds__(TPB) k_scatter(const int32_t *__restrict__ colour, int64_t n,
+__global__ void __launch_bounds__(TPB) k_scatter(const int32_t *__restrict__ colour, int64_t n, int64_t totalCells,
                                                   int32_t *__restrict__ cursor, int32_t *__restrict__ order) {
     int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
     if (p >= n) return;
-    int32_t pos = atomicAdd(&cursor[colour[p]], 1);
+    int32_t c = colour[p];
+    if (c < 0 || (int64_t)c >= totalCells) return;  // flagged by k_colour_count
+    int32_t pos = atomicAdd(&cursor[c], 1);
     order[pos] = (int32_t)p;
 }
 
@@ -320,7 +322,7 @@ hipError_t launch_scatter_sort(Context &c) {
                                   hipMemcpyDeviceToDevice, c.stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.colour.as<int32_t>(), c.n,
-                       c.cellCursor.as<int32_t>(), c.order.as<int32_t>());
+                       c.grid.totalCells, c.cellCursor.as<int32_t>(), c.order.as<int32_t>());
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(TPB), 0, c.stream, c.red.as<uint64_t>() + 9, 0ull, (int64_t)2);
     hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(c.grid.totalCells)), dim3(TPB), 0, c.stream,
                        c.cellStart.as<int32_t>(), c.grid.totalCells, c.order.as<int32_t>(), c.red.as<uint64_t>() + 9);
