@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Timestep throughput of the MI355X HiDEM hot path on the 8M-particle glacier slab.
+
+A "step" is one full timestep of the hot path -- rebuild check (and rebuild
+when due), contact + beam force gather, time integration, link break -- over
+all 8,000,000 particles / ~48M links resident in HBM.  Prints one JSON line.
+
+value      whole-job particle-steps/s of the headline form (--precision):
+           N_particles * K / wall time of the K timed steps, bracketed by
+           device synchronisation.
+roofline   for the dominant kernel (force gather): algorithmic bytes per
+           launch (compulsory traffic of the arrays it must touch, see
+           force_bytes) / its mean launch time from HIP events recorded on the
+           context's stream during the timed steps, against 8 TB/s.
+cpu_baseline  the CPU oracle (a C/OpenMP restatement of the reference's CPU
+           path, built with the reference's release flags) timed on this host
+           on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from hidem_amd import _abi as A  # noqa: E402
+from hidem_amd import params  # noqa: E402
+
+METRIC = "particle-steps/s at 8M particles (mixed+learned vs plain FP64); % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def force_bytes(n, n_neigh_total, n_sides, prec):
+    """Compulsory HBM bytes of one force-gather launch.
+
+    Per particle: own coordinates (3 x 8), Diff (6 x 8), radius (8),
+    neighbour and link counts (2 x 4) read; force (6 x 8 FP64 / 6 x 4 mixed)
+    written.  Per neighbour-list entry: its 4 B index.  Per link side: CT0..5
+    read and written (2 x 6 x 8 FP64 / 2 x 6 x 4 mixed), link id and other
+    endpoint (2 x 4), link state (1).  Partner coordinates/Diff are gathered
+    from particles already counted once (cache reuse), as in the roofline model
+    of the hot path."""
+    fo = 8 if prec == A.HDM_PREC_FP64 else 4
+    per_particle = 24 + 48 + 8 + 8 + 6 * fo
+    per_side = 12 * fo + 8 + 1
+    return n * per_particle + 4 * n_neigh_total + n_sides * per_side
+
+
+def integrate_bytes(n, prec):
+    fo = 8 if prec == A.HDM_PREC_FP64 else 4
+    # read cords, theta, Diff, radius, type, force; write cords, theta, Diff
+    return n * (24 + 24 + 48 + 8 + 1 + 6 * fo + 24 + 24 + 48)
+
+
+def run_gpu(setup, p, l, prec, steps, warmup):
+    with A.Context(prec) as ctx:
+        ctx.upload(p, l)
+        ctx.links_build()
+        st = ctx.run(setup.run, warmup) if warmup > 0 else None
+        n_neigh, _ = 0, None
+        ctx.sync()
+        ctx.set_timing(True)
+        ctx.reset_kernel_times()
+        t0 = time.perf_counter()
+        st = ctx.run(setup.run, steps)
+        ctx.sync()
+        t1 = time.perf_counter()
+        kt = ctx.kernel_times()
+        ctx.set_timing(False)
+        counts, _, _ = ctx.neighbours_download()
+        n_neigh = int(counts.sum())
+        broken = ctx.count_broken()
+        dev_bytes = ctx.device_bytes()
+    wall = t1 - t0
+    return dict(wall=wall, stats=st, kernels=kt, n_neigh=n_neigh, broken=broken, dev_bytes=dev_bytes)
+
+
+def cpu_baseline(setup, p, l, steps, fast=True):
+    from oracle import oracle as orc  # checker / baseline only
+    t0 = time.perf_counter()
+    sim = orc.OracleSim(p, l, setup, fast=fast)
+    sim.run(1)  # builds grid, neighbour list and link lists, one step
+    t1 = time.perf_counter()
+    sim.run(steps)
+    t2 = time.perf_counter()
+    threads = orc.num_threads(fast)
+    sim.close()
+    return dict(value=p.n * steps / (t2 - t1), seconds=t2 - t1, setup_seconds=t1 - t0, threads=threads)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="glacier_8m")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cells", type=int, nargs=3, default=None, help="override unit cells (smaller runs)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    cfg = params.load_config(args.config)
+    t0 = time.perf_counter()
+    setup, p, l = params.build_inputs(cfg, args.cells)
+    log(f"generated {p.n} particles, {l.n} links in {time.perf_counter() - t0:.1f}s")
+
+    forms = {"fp64": [A.HDM_PREC_FP64], "mixed": [A.HDM_PREC_MIXED],
+             "both": [A.HDM_PREC_FP64, A.HDM_PREC_MIXED]}[args.precision]
+    results = {}
+    for prec in forms:
+        name = "fp64" if prec == A.HDM_PREC_FP64 else "mixed"
+        log(f"running {name}: warmup {args.warmup}, timed {args.steps}")
+        r = run_gpu(setup, p, l, prec, args.steps, args.warmup)
+        r["value"] = p.n * args.steps / r["wall"]
+        fms, fcnt = r["kernels"]["force"]
+        r["force_avg_ms"] = fms / max(fcnt, 1)
+        r["force_bytes"] = force_bytes(p.n, r["n_neigh"], 2 * l.n, prec)
+        ims, icnt = r["kernels"]["integrate"]
+        r["integrate_avg_ms"] = ims / max(icnt, 1)
+        r["integrate_bytes"] = integrate_bytes(p.n, prec)
+        results[name] = r
+        log(f"  {name}: {r['value']:.4e} p-steps/s, {1e3 * r['wall'] / args.steps:.3f} ms/step, "
+            f"force {r['force_avg_ms']:.3f} ms ({r['force_bytes'] / r['force_avg_ms'] / 1e6:.0f} GB/s), "
+            f"rebuilds {r['stats'].rebuilds}, broken {r['broken']}")
+
+    head = "mixed" if "mixed" in results else "fp64"
+    h = results[head]
+    achieved = h["force_bytes"] / (h["force_avg_ms"] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"force_{head}_{args.config}_traffic.json")
+    if os.path.exists(tf) and args.cells is None:
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    if args.cpu_steps > 0:
+        log(f"cpu baseline: oracle, {args.cpu_steps} steps")
+        c = cpu_baseline(setup, p, l, args.cpu_steps)
+        cpu = {"value": c["value"], "unit": "particle-steps/s", "cores": c["threads"], "kind": "port",
+               "sample": f"{args.cpu_steps} timesteps of the full {p.n}-particle workload (after 1 setup step), "
+                         f"C/OpenMP oracle built -O3 -ffast-math, {c['seconds']:.1f} s"}
+
+    kern = {}
+    for name, r in results.items():
+        kern[name] = {k: {"ms_total": v[0], "launches": v[1]} for k, v in r["kernels"].items() if v[1]}
+    out = {
+        "metric": METRIC,
+        "value": h["value"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * h["wall"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.config}: FCC glacier slab, {p.n} particles, {l.n} links, "
+                        f"{'plain FP64' if head == 'fp64' else 'mixed FP32 force / FP64 integration'}, "
+                        f"exact skin rebuild test",
+            "particles": p.n, "links": l.n, "precision": head,
+            "rebuilds_in_timed_steps": int(h["stats"].rebuilds),
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "force gather",
+                     "bytes_per_launch": h["force_bytes"], "avg_launch_ms": h["force_avg_ms"]},
+        "cpu_baseline": cpu,
+        "forms": {name: {"value": r["value"], "ms_per_step": 1e3 * r["wall"] / args.steps,
+                         "force_avg_ms": r["force_avg_ms"],
+                         "force_GBps": r["force_bytes"] / (r["force_avg_ms"] * 1e-3) / 1e9,
+                         "integrate_avg_ms": r["integrate_avg_ms"],
+                         "integrate_GBps": r["integrate_bytes"] / (r["integrate_avg_ms"] * 1e-3) / 1e9,
+                         "rebuilds": int(r["stats"].rebuilds), "broken_links": r["broken"],
+                         "device_bytes": r["dev_bytes"], "kernels": kern[name]}
+                  for name, r in results.items()},
+    }
+    if "fp64" in results and "mixed" in results:
+        out["mixed_over_fp64"] = results["mixed"]["value"] / results["fp64"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

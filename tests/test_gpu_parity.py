@@ -243,6 +243,7 @@ def test_break_bit_identical(block, orc):
     l.state[rng.random(l.n) < 0.05] = 0
     say("break")
     with new_ctx(p, l) as ctx:
+        ctx.links_build()
         nb = ctx.brk(s.brk)
         gl = ctx.download_links()
     sim = orc.OracleSim(p, l, s)
