@@ -91,6 +91,12 @@ class HdmFccSpec(C.Structure):
                 ("seed", C.c_uint64), ("clampXBelow", C.c_double)]
 
 
+class HdmSetupInfo(C.Structure):
+    _fields_ = [("run", HdmRunParams), ("fcc", HdmFccSpec), ("linkGenScale", C.c_double),
+                ("threshold", C.c_double), ("spread", C.c_double), ("linkSeed", C.c_uint64),
+                ("endTimestep", C.c_int64), ("precision", C.c_int32)]
+
+
 _lib = None
 
 
@@ -139,6 +145,15 @@ def _declare(L):
         "hdm_run": ([P, C.POINTER(HdmRunParams), C.c_int64, _u8p, C.POINTER(HdmRunStats)], C.c_int),
         "hdm_set_timestep": ([P, C.c_int64], C.c_int),
         "hdm_count_broken": ([P, _i64p], C.c_int),
+        "hdm_config_derive": ([C.c_int32, C.POINTER(C.c_char_p), C.POINTER(HdmSetupInfo)], C.c_int),
+        "hdm_sim_create": ([C.c_int32, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(P)], C.c_int),
+        "hdm_sim_destroy": ([P], C.c_int),
+        "hdm_sim_last_error": ([], C.c_char_p),
+        "hdm_sim_ctx": ([P, C.POINTER(P)], C.c_int),
+        "hdm_sim_run_params": ([P, C.POINTER(HdmRunParams)], C.c_int),
+        "hdm_sim_info": ([P, _i64p, _i64p, _i64p, _i64p], C.c_int),
+        "hdm_sim_steps": ([P, C.c_int64, _u8p, C.POINTER(HdmRunStats)], C.c_int),
+        "hdm_sim_config_dump": ([P], C.c_char_p),
         "hdm_gen_fcc_count": ([C.POINTER(HdmFccSpec), _i64p], C.c_int),
         "hdm_gen_fcc_fill": ([C.POINTER(HdmFccSpec), C.POINTER(HdmParticlesHost)], C.c_int),
         "hdm_gen_links_count": ([C.POINTER(HdmParticlesHost), C.c_double, _i64p], C.c_int),
@@ -420,3 +435,71 @@ class Context:
         b = C.c_int64()
         self._c(lib().hdm_device_bytes(self._h, C.byref(b)), "hdm_device_bytes")
         return b.value
+
+
+# ---- TOML-driven setup (C++ configure / HiDEMLibHip) ----------------------------
+def _argv(args):
+    arr = (C.c_char_p * len(args))(*[a.encode() for a in args])
+    return len(args), arr
+
+
+def config_derive(config_path, overrides=()):
+    """Setup parameters derived by the C++ TOML path (no GPU needed)."""
+    argc, argv = _argv(list(overrides) + [config_path])
+    info = HdmSetupInfo()
+    rc = lib().hdm_config_derive(argc, argv, C.byref(info))
+    if rc != HDM_OK:
+        raise HdmError(f"hdm_config_derive: {lib().hdm_sim_last_error().decode()}")
+    return info
+
+
+class Sim:
+    """HiDEMLibHip set up from a TOML config (hdm_sim_*)."""
+
+    def __init__(self, config_path, overrides=(), precision=HDM_PREC_FP64):
+        argc, argv = _argv(list(overrides) + [config_path])
+        self._h = C.c_void_p()
+        rc = lib().hdm_sim_create(precision, argc, argv, C.byref(self._h))
+        if rc != HDM_OK:
+            raise HdmError(f"hdm_sim_create failed ({rc}): {lib().hdm_sim_last_error().decode()}")
+        n, L, ts, end = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        lib().hdm_sim_info(self._h, C.byref(n), C.byref(L), C.byref(ts), C.byref(end))
+        self.n, self.L, self.end = n.value, L.value, end.value
+        ctx = C.c_void_p()
+        lib().hdm_sim_ctx(self._h, C.byref(ctx))
+        self.ctx = Context.__new__(Context)  # borrowed view of the sim's context
+        self.ctx._h = ctx
+        self.ctx.precision = precision
+        self.ctx.n, self.ctx.L = self.n, self.L
+        self.ctx.close = lambda: None
+
+    def run_params(self):
+        rp = HdmRunParams()
+        check(lib().hdm_sim_run_params(self._h, C.byref(rp)), what="hdm_sim_run_params")
+        return rp
+
+    def steps(self, n, log_rebuilds=False):
+        st = HdmRunStats()
+        log = np.zeros(max(n, 1), np.uint8) if log_rebuilds else None
+        check(lib().hdm_sim_steps(self._h, n, _ptr(log, C.c_uint8), C.byref(st)), what="hdm_sim_steps")
+        return (st, log[:n]) if log_rebuilds else st
+
+    def config_dump(self):
+        return lib().hdm_sim_config_dump(self._h).decode()
+
+    def close(self):
+        if self._h:
+            lib().hdm_sim_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

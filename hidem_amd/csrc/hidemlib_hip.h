@@ -1,0 +1,102 @@
+// HiDEMLibHip: the timestep API of HiDEMLib (HiDEMLib/HiDEMLib.h:45-960) on
+// the gfx950 accelerator, for one GPU (one rank, World of one Domain).
+//
+// Same call sequence as the reference mains (MainIceBlock.cpp:409-447):
+//   while (hiDem.run()) {
+//       hiDem.gridService();
+//       hiDem.computePartForce(HiDEMDefNeighForceFunc(...), HiDEMDefLinkForceFunc(...));
+//       hiDem.applyForce(integrator);
+//       if (hiDem.getCurrentTimestep() * dt > breakTime) hiDem.linkUpdate(HiDEMDefLinkBreakFunc(...));
+//       hiDem.finalizeTs();
+//   }
+// setup() reads the reference's [iteration], [grid] and [lattice] keys
+// (HiDEMLib.h:707-822) plus the build-defined [hdm] table (block size,
+// jitter, clamp, threshold spread, rebuild policy) and generates the seeded
+// FCC block and its links on the host.
+#pragma once
+
+#include <chrono>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hidem/CombinedAccelHip.h"
+#include "toml_lite.h"
+
+namespace hdm {
+
+// [general] of MainIceBlock.cpp:25-51 / MainDem.cpp with the reference
+// defaults, scaled as MainIceBlock.cpp:99-121.
+struct GeneralParams {
+    double breakTime, dt, scale, beamWidth, E, waterLevel, attractDistance, attractScaleForce, maxVel, maxRot;
+    double rhoWater, rhoIce, bedStiffness, bedInteract, waterDrag, airDrag, bedDrag, g, damping, dampingRotation;
+    double maxLoad, bendBreak;
+    bool bound[6];
+    double ext[6];
+    // derived
+    double lift, invDt, invDt2, dtSqr, invDtSqr, G, W2, W4, scaleE;
+
+    static GeneralParams read(ArgParser &parser);
+    hidem::HiDEMDefNeighForceFunc neighForce() const;
+    hidem::HiDEMDefLinkForceFunc linkForce() const;
+    hidem::HiDEMDefLinkBreakFunc linkBreak() const;
+    // IceBlock (variant HDM_INTEG_ICEBLOCK) or GlacierDem (HDM_INTEG_GLACIER)
+    // applyForce parameters; bed geometry from the [hdm] table.
+    hdm_integ_params integrator(ArgParser &parser) const;
+};
+
+// The pure part of HiDEMLib::setup: everything derived from the config.
+struct SetupInfo {
+    hdm_setup_info info;
+    GeneralParams gp;
+};
+SetupInfo configure(ArgParser &parser);
+
+class HiDEMLibHip {
+public:
+    explicit HiDEMLibHip(int32_t precision = HDM_PREC_FP64, int32_t device = 0);
+
+    hidem::CombinedAccelHip accel;
+    hidem::World world;
+
+    // HiDEMLib::setup (HiDEMLib.h:707-822)
+    void setup(ArgParser &parser, double scale);
+    void setup(const hdm_setup_info &info);
+
+    bool run() const;                 // HiDEMLib.h:236-240
+    void gridService();               // HiDEMLib.h:533-628 (rebuild part)
+    void computePartForce(const hidem::HiDEMDefNeighForceFunc &n, const hidem::HiDEMDefLinkForceFunc &l);
+    void applyForce(const hdm_integ_params &ip);
+    void linkUpdate(const hidem::HiDEMDefLinkBreakFunc &b);
+    void finalizeTs();                // HiDEMLib.h:495-530
+    int getCurrentTimestep() const { return currentTimestep; }
+
+    int64_t numParticles() const { return nParticles; }
+    int64_t numLinks() const { return nLinks; }
+    int64_t rebuilds() const { return nRebuilds; }
+    bool lastStepRebuilt() const { return lastRebuilt; }
+
+    double searchDistance = 0, interactDist = 0, attractDistance = 0;
+    int32_t policy = HDM_REBUILD_HEURISTIC;
+    int gridRegenInterval = -1;
+    int iterationBlockSize = 10;
+    int currentTimestep = 0, startTimestep = 0, endTimestep = 10000;
+    unsigned int maxMinutes = (unsigned int)-1;
+    double accumulatedDist = 0;
+
+    // host copies of the generated state (checkpoint order), kept for callers
+    // that want to inspect the initial condition
+    std::vector<double> h_cords[3], h_radius;
+    std::vector<uint8_t> h_type;
+    std::vector<int32_t> h_a, h_b;
+    std::vector<double> h_len, h_maxLoad;
+
+private:
+    int64_t nParticles = 0, nLinks = 0, nRebuilds = 0;
+    bool lastRebuilt = false;
+    bool built = false;
+    std::chrono::steady_clock::time_point runStart;
+    void rebuild();
+};
+
+}  // namespace hdm

@@ -1,0 +1,446 @@
+// HiDEMLibHip and the hdm_sim_* C entry points (TOML-driven setup + the
+// reference-shaped timestep loop).
+#include "hidemlib_hip.h"
+
+#include <cmath>
+#include <cstdio>
+#include <limits>
+#include <new>
+
+namespace hdm {
+
+namespace {
+template <class T> T arg_or(ArgParser &p, const std::string &name, T dflt) { return p.getArg<T>(name).value_or(dflt); }
+}  // namespace
+
+// MainIceBlock.cpp:25-51 (defaults) and :99-121 (scaling / derived values);
+// MainDem.cpp:116-138 is identical.  Evaluated in double, in the reference's
+// expression order.
+GeneralParams GeneralParams::read(ArgParser &a) {
+    GeneralParams g;
+    g.breakTime = arg_or<double>(a, "general.breakTime", 10.0);
+    g.dt = arg_or<double>(a, "general.dt", 0.0001);
+    g.scale = arg_or<double>(a, "general.scale", 1.0);
+    g.beamWidth = arg_or<double>(a, "general.beamWidth", 0.6);
+    g.E = arg_or<double>(a, "general.E", 1e+09);
+    g.waterLevel = arg_or<double>(a, "general.waterLevel", 0.0);
+    g.attractDistance = arg_or<double>(a, "general.attractDistance", 0.0);
+    g.attractScaleForce = arg_or<double>(a, "general.attractScaleForce", 0.0);
+    g.maxVel = arg_or<double>(a, "general.maxVel", 20);
+    g.maxRot = arg_or<double>(a, "general.maxRot", 5);
+    g.rhoWater = arg_or<double>(a, "general.rhoWater", 1030.0);
+    g.rhoIce = arg_or<double>(a, "general.rhoIce", 910.0);
+    g.bedStiffness = arg_or<double>(a, "general.bedStiffness", 2e+08);
+    g.bedInteract = arg_or<double>(a, "general.bedInteract", 2.0);
+    g.waterDrag = arg_or<double>(a, "general.waterDrag", 10);
+    g.airDrag = arg_or<double>(a, "general.airDrag", 1);
+    g.bedDrag = arg_or<double>(a, "general.bedDrag", 1000.0);
+    g.g = arg_or<double>(a, "general.g", 9.81);
+    g.damping = arg_or<double>(a, "general.damping", 1000.0);
+    g.dampingRotation = arg_or<double>(a, "general.dampingRotation", 1000.0);
+    g.maxLoad = arg_or<double>(a, "general.maxLoad", 0.35);
+    g.bendBreak = arg_or<double>(a, "general.bendBreak", 0.35);
+    const char *bn[6] = {"general.xBoundaryMin", "general.xBoundaryMax", "general.yBoundaryMin",
+                         "general.yBoundaryMax", "general.zBoundaryMin", "general.zBoundaryMax"};
+    for (int k = 0; k < 6; k++) g.bound[k] = arg_or<bool>(a, bn[k], false);
+    const char *en[6] = {"externalForce.xForce", "externalForce.yForce", "externalForce.zForce",
+                         "externalForce.uForce", "externalForce.vForce", "externalForce.wForce"};
+    for (int k = 0; k < 6; k++) g.ext[k] = arg_or<double>(a, en[k], 0.0);
+
+    const double scale = g.scale;
+    g.beamWidth *= scale;
+    g.bedInteract *= scale;
+    g.bedStiffness *= scale * scale;
+    g.waterDrag *= scale * scale;
+    g.airDrag *= scale * scale;
+    g.bedDrag *= scale * scale;
+    g.damping *= scale * scale;
+    g.dampingRotation *= scale * scale;
+    g.attractDistance *= scale;
+    g.lift = (g.rhoWater - g.rhoIce) / g.rhoIce * g.g;
+    g.invDt = 1.0 / g.dt;
+    g.invDt2 = 1.0 / (2.0 * g.dt);
+    g.dtSqr = std::pow(g.dt, 2.0);
+    g.invDtSqr = 1.0 / std::pow(g.dt, 2.0);
+    g.G = g.E / 2.0;
+    g.W2 = g.beamWidth * g.beamWidth;
+    g.W4 = g.beamWidth * g.beamWidth * g.beamWidth * g.beamWidth;
+    g.scaleE = g.E * scale;
+    return g;
+}
+
+hidem::HiDEMDefNeighForceFunc GeneralParams::neighForce() const {
+    return hidem::HiDEMDefNeighForceFunc(invDt, attractDistance, attractScaleForce, damping, dampingRotation, scaleE);
+}
+hidem::HiDEMDefLinkForceFunc GeneralParams::linkForce() const {
+    return hidem::HiDEMDefLinkForceFunc(invDt, damping, dampingRotation, E, W2, W4, G);
+}
+hidem::HiDEMDefLinkBreakFunc GeneralParams::linkBreak() const { return hidem::HiDEMDefLinkBreakFunc(bendBreak, maxLoad); }
+
+hdm_integ_params GeneralParams::integrator(ArgParser &a) const {
+    hdm_integ_params ip{};
+    std::string kind = arg_or<std::string>(a, "hdm.integrator", "iceblock");
+    if (kind != "iceblock" && kind != "glacier") throw TomlError("hdm.integrator must be \"iceblock\" or \"glacier\"");
+    ip.variant = kind == "glacier" ? HDM_INTEG_GLACIER : HDM_INTEG_ICEBLOCK;
+    ip.rhoIce = rhoIce;
+    ip.g = this->g;
+    ip.lift = lift;
+    ip.waterLevel = waterLevel;
+    ip.airDrag = airDrag;
+    ip.waterDrag = waterDrag;
+    ip.invDt = invDt;
+    ip.invDt2 = invDt2;
+    ip.dtSqr = dtSqr;
+    ip.invDtSqr = invDtSqr;
+    ip.dt = dt;
+    ip.bedDrag = bedDrag;
+    ip.bedInteract = bedInteract;
+    ip.bedStiffness = bedStiffness;
+    ip.frictionCap = 1e+5 * scale * scale;  // MainDem.cpp:181
+    ip.bedZ = arg_or<double>(a, "hdm.bedZ", 0.0);
+    ip.bedXEnd = arg_or<double>(a, "hdm.bedXEnd", 1e300);
+    ip.deepZ = arg_or<double>(a, "hdm.deepZ", -1e300);
+    ip.frictionXEnd = arg_or<double>(a, "hdm.frictionXEnd", 1e300);
+    for (int k = 0; k < 6; k++) ip.ext[k] = ext[k];
+    ip.maxVel = maxVel;
+    ip.maxRot = maxRot;
+    ip.velocityLimit = ip.variant == HDM_INTEG_GLACIER ? 1 : 0;
+    for (int k = 0; k < 6; k++) ip.bound[k] = bound[k] ? 1 : 0;
+    std::vector<double> lo = arg_or<std::vector<double>>(a, "hdm.lowerLimit", {0.0, 0.0, 0.0});
+    std::vector<double> hi = arg_or<std::vector<double>>(a, "hdm.upperLimit", {0.0, 0.0, 0.0});
+    if (lo.size() != 3 || hi.size() != 3) throw TomlError("hdm.lowerLimit/upperLimit need 3 values");
+    for (int k = 0; k < 3; k++) {
+        ip.lower[k] = lo[(size_t)k];
+        ip.upper[k] = hi[(size_t)k];
+    }
+    return ip;
+}
+
+HiDEMLibHip::HiDEMLibHip(int32_t precision, int32_t device) : accel(precision, device) {}
+
+// The configuration half of HiDEMLib::setup (HiDEMLib.h:707-761 and
+// HiDEMGeometry::setupLattice, HiDEMGeometry.cpp:86-95): no device work.
+SetupInfo configure(ArgParser &parser) {
+    SetupInfo s;
+    s.gp = GeneralParams::read(parser);
+    const double scale = s.gp.scale;
+    hdm_setup_info &o = s.info;
+    o = hdm_setup_info{};
+    o.endTimestep = arg_or<int>(parser, "iteration.timeSteps", 10000);
+    double searchDistanceScale = arg_or<double>(parser, "grid.searchDistanceScale", 1.4);
+    int regen = arg_or<int>(parser, "grid.gridRegenInterval", -1);
+
+    std::string type = arg_or<std::string>(parser, "lattice.type", "");
+    if (type != "FCC") throw TomlError("lattice.type: only \"FCC\" is generated on this path");
+    auto need = [&](const char *k) {
+        auto v = parser.getArg<double>(k);
+        if (!v) throw TomlError(std::string("missing ") + k);
+        return *v;
+    };
+    double spacing = need("lattice.fccSpacing") * scale;
+    double radius = need("lattice.particleRadius") * scale;
+    o.linkGenScale = need("lattice.linkGenScale");
+    auto cells = parser.getArg<std::vector<int>>("hdm.cells");
+    if (!cells || cells->size() != 3) throw TomlError("hdm.cells needs 3 values");
+    std::vector<double> origin = arg_or<std::vector<double>>(parser, "hdm.origin", {0.0, 0.0, 0.0});
+    if (origin.size() != 3) throw TomlError("hdm.origin needs 3 values");
+    o.fcc.nx = (*cells)[0];
+    o.fcc.ny = (*cells)[1];
+    o.fcc.nz = (*cells)[2];
+    o.fcc.a = spacing;
+    for (int k = 0; k < 3; k++) o.fcc.origin[k] = origin[(size_t)k];
+    o.fcc.radius = radius;
+    o.fcc.jitter = arg_or<double>(parser, "hdm.jitter", 0.0);
+    o.fcc.seed = (uint64_t)arg_or<int64_t>(parser, "lattice.seed", 1);
+    o.fcc.clampXBelow = arg_or<double>(parser, "hdm.clampXBelow", -std::numeric_limits<double>::infinity());
+    o.spread = arg_or<double>(parser, "hdm.maxLoadSpread", 0.0);
+    o.linkSeed = (uint64_t)arg_or<int64_t>(parser, "hdm.linkSeed", 7);
+    o.threshold = s.gp.maxLoad;
+    std::string prec = arg_or<std::string>(parser, "hdm.precision", "fp64");
+    if (prec != "fp64" && prec != "mixed") throw TomlError("hdm.precision must be \"fp64\" or \"mixed\"");
+    o.precision = prec == "mixed" ? HDM_PREC_MIXED : HDM_PREC_FP64;
+
+    std::string pol = arg_or<std::string>(parser, "hdm.rebuildPolicy", "heuristic");
+    int32_t policy;
+    if (pol == "heuristic") policy = HDM_REBUILD_HEURISTIC;
+    else if (pol == "exact") policy = HDM_REBUILD_EXACT;
+    else if (pol == "interval") policy = HDM_REBUILD_INTERVAL;
+    else throw TomlError("hdm.rebuildPolicy must be heuristic, exact or interval");
+    if (regen != -1) policy = HDM_REBUILD_INTERVAL;
+
+    // HiDEMLib.h:759-761
+    double maxParticleDist = radius * 2.0;
+    o.run.searchDistance = searchDistanceScale * maxParticleDist;
+    o.run.interactDist = o.run.searchDistance - maxParticleDist;
+    o.run.force = hidem::make_force_params(s.gp.neighForce(), s.gp.linkForce());
+    o.run.integ = s.gp.integrator(parser);
+    o.run.brk = hdm_break_params{s.gp.bendBreak, s.gp.maxLoad};
+    o.run.dt = s.gp.dt;
+    o.run.breakTime = s.gp.breakTime;
+    o.run.policy = policy;
+    o.run.regenInterval = regen;
+    return s;
+}
+
+void HiDEMLibHip::setup(ArgParser &parser, double scale) {
+    SetupInfo s = configure(parser);
+    if (scale != s.gp.scale) throw TomlError("setup: scale differs from general.scale");
+    maxMinutes = arg_or<unsigned int>(parser, "iteration.maxMinutes", (unsigned int)-1);
+    setup(s.info);
+}
+
+void HiDEMLibHip::setup(const hdm_setup_info &info) {
+    runStart = std::chrono::steady_clock::now();
+    endTimestep = (int)info.endTimestep;
+    searchDistance = info.run.searchDistance;
+    interactDist = info.run.interactDist;
+    attractDistance = info.run.force.attractDistance;
+    policy = info.run.policy;
+    gridRegenInterval = info.run.regenInterval;
+    hdm_fcc_spec spec = info.fcc;
+    double radius = spec.radius;
+
+    hidem::hdm_check(nullptr, hdm_gen_fcc_count(&spec, &nParticles), "hdm_gen_fcc_count");
+    size_t n = (size_t)nParticles;
+    for (auto &v : h_cords) v.assign(n, 0.0);
+    h_radius.assign(n, 0.0);
+    h_type.assign(n, 0);
+    std::vector<double> th[3], df[6];
+    for (auto &v : th) v.assign(n, 0.0);
+    for (auto &v : df) v.assign(n, 0.0);
+    hdm_particles_host p{};
+    p.n = nParticles;
+    p.xCord = h_cords[0].data();
+    p.yCord = h_cords[1].data();
+    p.zCord = h_cords[2].data();
+    p.xTheta = th[0].data();
+    p.yTheta = th[1].data();
+    p.zTheta = th[2].data();
+    p.xDiff = df[0].data();
+    p.yDiff = df[1].data();
+    p.zDiff = df[2].data();
+    p.uDiff = df[3].data();
+    p.vDiff = df[4].data();
+    p.wDiff = df[5].data();
+    p.radius = h_radius.data();
+    p.type = h_type.data();
+    hidem::hdm_check(nullptr, hdm_gen_fcc_fill(&spec, &p), "hdm_gen_fcc_fill");
+    hidem::hdm_check(nullptr, hdm_gen_links_count(&p, info.linkGenScale, &nLinks), "hdm_gen_links_count");
+    size_t L = (size_t)nLinks;
+    h_a.assign(L, 0);
+    h_b.assign(L, 0);
+    h_len.assign(L, 0.0);
+    h_maxLoad.assign(L, 0.0);
+    std::vector<uint8_t> state(L, 1);
+    hdm_links_host l{};
+    l.n = nLinks;
+    l.localPartIndA = h_a.data();
+    l.localPartIndB = h_b.data();
+    l.state = state.data();
+    l.initialLen = h_len.data();
+    l.maxLoad = h_maxLoad.data();
+    hidem::hdm_check(nullptr, hdm_gen_links_fill(&p, info.linkGenScale, info.threshold, info.spread, info.linkSeed, &l),
+                     "hdm_gen_links_fill");
+
+    accel.uploadParticles(p);
+    accel.uploadLinks(l);
+    accel.generateLinks();
+
+    hidem::Domain d;
+    d.owner = 0;
+    d.lower = hidem::Real3(INFINITY);
+    d.upper = hidem::Real3(-INFINITY);
+    for (size_t i = 0; i < n; i++) {
+        d.lower = {std::fmin(d.lower.x, h_cords[0][i]), std::fmin(d.lower.y, h_cords[1][i]), std::fmin(d.lower.z, h_cords[2][i])};
+        d.upper = {std::fmax(d.upper.x, h_cords[0][i]), std::fmax(d.upper.y, h_cords[1][i]), std::fmax(d.upper.z, h_cords[2][i])};
+    }
+    d.upper = d.upper + radius;
+    d.lower = d.lower - radius;
+    d.setHalo(searchDistance);
+    world.domains = {d};
+    currentTimestep = startTimestep = 0;
+    built = false;
+}
+
+bool HiDEMLibHip::run() const {
+    auto mins = std::chrono::duration_cast<std::chrono::minutes>(std::chrono::steady_clock::now() - runStart).count();
+    return currentTimestep < endTimestep && (unsigned long long)mins < (unsigned long long)maxMinutes;
+}
+
+void HiDEMLibHip::rebuild() {
+    accumulatedDist = 0;
+    accel.generateNeighbours(searchDistance, interactDist);
+    hidem::hdm_check(accel.ctx, hdm_skin_snapshot(accel.ctx), "hdm_skin_snapshot");
+    built = true;
+    nRebuilds++;
+}
+
+// HiDEMLib::gridService (HiDEMLib.h:544-559, 589-627) with the build's exact
+// displacement test as an alternative policy.
+void HiDEMLibHip::gridService() {
+    bool needsGridRegen = currentTimestep == startTimestep || !built;
+    if (policy == HDM_REBUILD_INTERVAL) {
+        if (gridRegenInterval > 0 && (currentTimestep % gridRegenInterval) == 0) needsGridRegen = true;
+    } else if (policy == HDM_REBUILD_HEURISTIC) {
+        if (currentTimestep % iterationBlockSize == 0) {
+            double d = 0;
+            hidem::hdm_check(accel.ctx, hdm_skin_heuristic(accel.ctx, 10, &d), "hdm_skin_heuristic");
+            accumulatedDist += d;
+            if (accumulatedDist >= interactDist / 2.0) needsGridRegen = true;
+        }
+    } else if (built) {
+        double d = 0;
+        hidem::hdm_check(accel.ctx, hdm_skin_exact(accel.ctx, &d), "hdm_skin_exact");
+        if (2.0 * d >= interactDist - attractDistance) needsGridRegen = true;
+    }
+    lastRebuilt = needsGridRegen;
+    if (needsGridRegen) rebuild();
+}
+
+void HiDEMLibHip::computePartForce(const hidem::HiDEMDefNeighForceFunc &n, const hidem::HiDEMDefLinkForceFunc &l) {
+    accel.iterateCombinedPre(n, l);
+    accel.iterateCombinedPost(n, l);
+}
+
+void HiDEMLibHip::applyForce(const hdm_integ_params &ip) { accel.applyForce(ip); }
+
+void HiDEMLibHip::linkUpdate(const hidem::HiDEMDefLinkBreakFunc &b) { accel.iterateLinks(b); }
+
+void HiDEMLibHip::finalizeTs() { currentTimestep++; }
+
+}  // namespace hdm
+
+// ---------------------------------------------------------------------------
+// C entry points
+struct hdm_sim {
+    hdm::HiDEMLibHip lib;
+    hdm::ArgParser args;
+    hdm::GeneralParams gp;
+    hdm_integ_params integ;
+    std::string dump;
+    explicit hdm_sim(int32_t prec) : lib(prec) {}
+};
+
+static thread_local std::string g_sim_error;
+
+extern "C" {
+
+const char *hdm_sim_last_error(void) { return g_sim_error.c_str(); }
+
+int hdm_config_derive(int32_t argc, const char *const *argv, hdm_setup_info *out) {
+    if (!out || argc < 1 || !argv) return HDM_E_INVALID;
+    try {
+        std::vector<const char *> av{"hdm_config"};
+        for (int i = 0; i < argc; i++) av.push_back(argv[i]);
+        hdm::ArgParser args((int)av.size(), av.data());
+        *out = hdm::configure(args).info;
+        return HDM_OK;
+    } catch (const std::exception &e) {
+        g_sim_error = e.what();
+        return HDM_E_INVALID;
+    }
+}
+
+int hdm_sim_create(int32_t precision, int32_t argc, const char *const *argv, hdm_sim **out) {
+    if (!out || argc < 1 || !argv) return HDM_E_INVALID;
+    *out = nullptr;
+    hdm_ctx *probe = nullptr;
+    int rc = hdm_ctx_create(precision, 0, &probe);
+    if (rc != HDM_OK) {
+        g_sim_error = "no usable gfx950 device";
+        return rc;
+    }
+    hdm_ctx_destroy(probe);
+    try {
+        std::vector<const char *> av{"hdm_sim"};
+        for (int i = 0; i < argc; i++) av.push_back(argv[i]);
+        hdm::ArgParser args((int)av.size(), av.data());
+        hdm::SetupInfo si = hdm::configure(args);
+        hdm_sim *s = new hdm_sim(precision);
+        s->args = std::move(args);
+        s->gp = si.gp;
+        s->integ = si.info.run.integ;
+        s->lib.maxMinutes = s->args.getArg<unsigned int>("iteration.maxMinutes").value_or((unsigned int)-1);
+        s->lib.setup(si.info);
+        *out = s;
+        return HDM_OK;
+    } catch (const std::exception &e) {
+        g_sim_error = e.what();
+        return HDM_E_INVALID;
+    }
+}
+
+int hdm_sim_destroy(hdm_sim *s) {
+    delete s;
+    return HDM_OK;
+}
+
+int hdm_sim_ctx(hdm_sim *s, hdm_ctx **ctx) {
+    if (!s || !ctx) return HDM_E_INVALID;
+    *ctx = s->lib.accel.ctx;
+    return HDM_OK;
+}
+
+int hdm_sim_run_params(hdm_sim *s, hdm_run_params *rp) {
+    if (!s || !rp) return HDM_E_INVALID;
+    rp->force = hidem::make_force_params(s->gp.neighForce(), s->gp.linkForce());
+    rp->integ = s->integ;
+    rp->brk = hdm_break_params{s->gp.bendBreak, s->gp.maxLoad};
+    rp->searchDistance = s->lib.searchDistance;
+    rp->interactDist = s->lib.interactDist;
+    rp->dt = s->gp.dt;
+    rp->breakTime = s->gp.breakTime;
+    rp->policy = s->lib.policy;
+    rp->regenInterval = s->lib.gridRegenInterval;
+    return HDM_OK;
+}
+
+int hdm_sim_info(hdm_sim *s, int64_t *nParticles, int64_t *nLinks, int64_t *timestep, int64_t *endTimestep) {
+    if (!s) return HDM_E_INVALID;
+    if (nParticles) *nParticles = s->lib.numParticles();
+    if (nLinks) *nLinks = s->lib.numLinks();
+    if (timestep) *timestep = s->lib.getCurrentTimestep();
+    if (endTimestep) *endTimestep = s->lib.endTimestep;
+    return HDM_OK;
+}
+
+// The loop of MainIceBlock.cpp:409-447, nSteps iterations (or until run()
+// is false).  rebuildLog may be NULL.
+int hdm_sim_steps(hdm_sim *s, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats *stats) {
+    if (!s || nSteps < 0) return HDM_E_INVALID;
+    auto &h = s->lib;
+    auto nF = s->gp.neighForce();
+    auto lF = s->gp.linkForce();
+    auto bF = s->gp.linkBreak();
+    auto t0 = std::chrono::steady_clock::now();
+    int64_t r0 = h.rebuilds(), done = 0;
+    for (; done < nSteps && h.run(); done++) {
+        h.gridService();
+        if (rebuildLog) rebuildLog[done] = h.lastStepRebuilt() ? 1 : 0;
+        h.computePartForce(nF, lF);
+        h.applyForce(s->integ);
+        if (h.getCurrentTimestep() * s->gp.dt > s->gp.breakTime) h.linkUpdate(bF);
+        h.finalizeTs();
+    }
+    h.accel.sync();
+    auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        stats->timestep = h.getCurrentTimestep();
+        stats->steps = done;
+        stats->rebuilds = h.rebuilds() - r0;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+        int64_t b = 0;
+        int rc = hdm_count_broken(h.accel.ctx, &b);
+        if (rc != HDM_OK) return rc;
+        stats->brokenTotal = b;
+    }
+    return HDM_OK;
+}
+
+const char *hdm_sim_config_dump(hdm_sim *s) {
+    if (!s) return "";
+    s->dump = s->args.table.dump();
+    return s->dump.c_str();
+}
+
+}  // extern "C"

@@ -68,12 +68,12 @@ def derive(cfg: dict) -> Setup:
     scale = g["scale"]
     beamWidth = g["beamWidth"] * scale
     bedInteract = g["bedInteract"] * scale
-    bedStiffness = g["bedStiffness"] * scale * scale
-    waterDrag = g["waterDrag"] * scale * scale
-    airDrag = g["airDrag"] * scale * scale
-    bedDrag = g["bedDrag"] * scale * scale
-    damping = g["damping"] * scale * scale
-    dampingRotation = g["dampingRotation"] * scale * scale
+    bedStiffness = g["bedStiffness"] * (scale * scale)  # "x *= scale * scale"
+    waterDrag = g["waterDrag"] * (scale * scale)  # "x *= scale * scale"
+    airDrag = g["airDrag"] * (scale * scale)  # "x *= scale * scale"
+    bedDrag = g["bedDrag"] * (scale * scale)  # "x *= scale * scale"
+    damping = g["damping"] * (scale * scale)  # "x *= scale * scale"
+    dampingRotation = g["dampingRotation"] * (scale * scale)  # "x *= scale * scale"
     attractDistance = g["attractDistance"] * scale
     lift = (g["rhoWater"] - g["rhoIce"]) / g["rhoIce"] * g["g"]
     dt = g["dt"]

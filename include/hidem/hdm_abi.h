@@ -236,6 +236,37 @@ int  hdm_gen_links_count(const hdm_particles_host *p, double linkGenScale, int64
 int  hdm_gen_links_fill(const hdm_particles_host *p, double linkGenScale, double threshold, double spread,
                         uint64_t seed, hdm_links_host *out);
 
+/* ---- TOML-driven simulation (HiDEMLib::setup + the sample main loop) --------
+ * argv holds the command line after the program name, as the reference's
+ * ArgParser expects it (ArgParser.h:170-196): optional --section.key=value
+ * overrides, then the config path last.  The [general], [iteration], [grid]
+ * and [lattice] keys are the reference's; [hdm] holds the build-defined
+ * extras (block size, jitter, clamp, threshold spread, integrator, bed,
+ * rebuild policy).  hdm_sim_steps runs gridService + computePartForce +
+ * applyForce + linkUpdate + finalizeTs per step (MainIceBlock.cpp:409-447). */
+typedef struct hdm_sim hdm_sim;
+
+/* Everything setup derives from a config, without touching a GPU: the step
+ * parameters, the lattice spec and the link generation parameters. */
+typedef struct hdm_setup_info {
+    hdm_run_params run;
+    hdm_fcc_spec fcc;
+    double linkGenScale, threshold, spread;
+    uint64_t linkSeed;
+    int64_t endTimestep;
+    int32_t precision;
+} hdm_setup_info;
+int  hdm_config_derive(int32_t argc, const char *const *argv, hdm_setup_info *out);
+
+int  hdm_sim_create(int32_t precision, int32_t argc, const char *const *argv, hdm_sim **out);
+int  hdm_sim_destroy(hdm_sim *sim);
+const char *hdm_sim_last_error(void);
+int  hdm_sim_ctx(hdm_sim *sim, hdm_ctx **ctx);
+int  hdm_sim_run_params(hdm_sim *sim, hdm_run_params *rp);
+int  hdm_sim_info(hdm_sim *sim, int64_t *nParticles, int64_t *nLinks, int64_t *timestep, int64_t *endTimestep);
+int  hdm_sim_steps(hdm_sim *sim, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats *stats);
+const char *hdm_sim_config_dump(hdm_sim *sim);
+
 #ifdef __cplusplus
 }
 #endif
