@@ -96,6 +96,27 @@ def cpu_baseline(setup, p, l, steps, fast=True):
     return dict(value=p.n * steps / (t2 - t1), seconds=t2 - t1, setup_seconds=t1 - t0, threads=threads)
 
 
+def probe_traffic(setup, p, l, forms):
+    """Kernels of known size for calibrating the FETCH_SIZE / WRITE_SIZE
+    counters, then the force gather on the real workload."""
+    for prec in forms:
+        with A.Context(prec) as ctx:
+            ctx.upload(p, l)
+            ctx.links_build()
+            ctx.run(setup.run, 3)
+            if prec == forms[0]:
+                g = ctx.bw_probe(A.HDM_PROBE_COPY, 1 << 30, 1)  # k_probe_copy: 512 MiB read + 512 MiB written
+                log(f"copy probe 512 MiB read + 512 MiB written: {g:.0f} GB/s")
+            for _ in range(3):
+                ctx.force(setup.force)
+            ctx.sync()
+            counts, _, _ = ctx.neighbours_download()
+            b = force_bytes(p.n, int(counts.sum()), 2 * l.n, prec)
+            print(json.dumps({"precision": "fp64" if prec == A.HDM_PREC_FP64 else "mixed",
+                              "force_algorithmic_bytes": b, "neighbours": int(counts.sum()), "sides": 2 * l.n}),
+                  flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +126,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps in the CPU baseline sample (0: skip)")
     ap.add_argument("--cells", type=int, nargs=3, default=None, help="override unit cells (smaller runs)")
+    ap.add_argument("--probe-traffic", action="store_true",
+                    help="for rocprofv3 --pmc runs: a 512 MiB copy kernel of known traffic, then 3 force launches "
+                         "per precision form on the workload; prints the launch sizes")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -119,6 +143,9 @@ def main():
 
     forms = {"fp64": [A.HDM_PREC_FP64], "mixed": [A.HDM_PREC_MIXED],
              "both": [A.HDM_PREC_FP64, A.HDM_PREC_MIXED]}[args.precision]
+    if args.probe_traffic:
+        probe_traffic(setup, p, l, forms)
+        return
     results = {}
     for prec in forms:
         name = "fp64" if prec == A.HDM_PREC_FP64 else "mixed"

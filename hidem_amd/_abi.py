@@ -26,6 +26,8 @@ HDM_INTEG_GLACIER = 1
 HDM_REBUILD_HEURISTIC = 0
 HDM_REBUILD_EXACT = 1
 HDM_REBUILD_INTERVAL = 2
+HDM_PROBE_COPY = 0
+HDM_PROBE_CT_RMW = 1
 TIMER_NAMES = ("force", "integrate", "break", "skin", "grid", "neigh", "links")
 
 _dp = C.POINTER(C.c_double)
@@ -143,6 +145,7 @@ def _declare(L):
         "hdm_skin_snapshot": ([P], C.c_int),
         "hdm_skin_exact": ([P, _dp], C.c_int),
         "hdm_run": ([P, C.POINTER(HdmRunParams), C.c_int64, _u8p, C.POINTER(HdmRunStats)], C.c_int),
+        "hdm_bw_probe": ([P, C.c_int32, C.c_int64, C.c_int32, _dp], C.c_int),
         "hdm_set_timestep": ([P, C.c_int64], C.c_int),
         "hdm_count_broken": ([P, _i64p], C.c_int),
         "hdm_config_derive": ([C.c_int32, C.POINTER(C.c_char_p), C.POINTER(HdmSetupInfo)], C.c_int),
@@ -408,6 +411,11 @@ class Context:
         self._c(lib().hdm_run(self._h, C.byref(rp), n_steps, _ptr(log, C.c_uint8), C.byref(st)), "hdm_run")
         return (st, log[:n_steps]) if log_rebuilds else st
 
+    def bw_probe(self, kind, nbytes, iters=10):
+        g = C.c_double()
+        self._c(lib().hdm_bw_probe(self._h, kind, nbytes, iters, C.byref(g)), "hdm_bw_probe")
+        return g.value
+
     def set_timestep(self, ts):
         self._c(lib().hdm_set_timestep(self._h, ts), "hdm_set_timestep")
 
@@ -503,3 +511,67 @@ class Sim:
             self.close()
         except Exception:
             pass
+
+
+# ---- batched form ---------------------------------------------------------------
+class HdmBatchSpec(C.Structure):
+    _fields_ = [("nBlocks", C.c_int32), ("minParticles", C.c_int64), ("maxParticles", C.c_int64),
+                ("a", C.c_double), ("radius", C.c_double), ("jitter", C.c_double), ("gap", C.c_double),
+                ("thrLo", C.c_double), ("thrHi", C.c_double), ("spread", C.c_double),
+                ("waterLo", C.c_double), ("waterHi", C.c_double), ("clampX", C.c_double), ("seed", C.c_uint64)]
+
+
+class HdmBatchBlock(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("origin", C.c_double * 3),
+                ("particleStart", C.c_int64), ("particleCount", C.c_int64), ("threshold", C.c_double),
+                ("waterLevel", C.c_double)]
+
+
+def _declare_batch(L):
+    L.hdm_gen_batch_layout.argtypes = [C.POINTER(HdmBatchSpec), C.POINTER(HdmBatchBlock), _i64p]
+    L.hdm_gen_batch_fill.argtypes = [C.POINTER(HdmBatchSpec), C.POINTER(HdmBatchBlock), C.POINTER(HdmParticlesHost),
+                                     _i32p]
+    L.hdm_gen_batch_thresholds.argtypes = [C.POINTER(HdmBatchSpec), C.POINTER(HdmBatchBlock), _i32p,
+                                           C.POINTER(HdmLinksHost)]
+    L.hdm_set_blocks.argtypes = [C.c_void_p, _i32p, C.c_int32, _dp]
+    for f in (L.hdm_gen_batch_layout, L.hdm_gen_batch_fill, L.hdm_gen_batch_thresholds, L.hdm_set_blocks):
+        f.restype = C.c_int
+
+
+@dataclass
+class Batch:
+    spec: HdmBatchSpec
+    blocks: object          # ctypes array of HdmBatchBlock
+    block_id: np.ndarray    # per particle
+    water: np.ndarray       # per block (absolute z)
+    particles: Particles
+    links: Links
+
+
+def gen_batch(n_blocks=256, min_particles=5000, max_particles=200000, a=1.4707821048680188, radius=0.5,
+              jitter=0.01, gap=3.0, thr=(2e-4, 5e-4), spread=0.1, water=(0.2, 0.8), clamp_x=2.0, seed=11,
+              link_gen_scale=1.092, link_seed=7) -> Batch:
+    L = lib()
+    _declare_batch(L)
+    spec = HdmBatchSpec(n_blocks, min_particles, max_particles, a, radius, jitter, gap, thr[0], thr[1], spread,
+                        water[0], water[1], clamp_x, seed)
+    blocks = (HdmBatchBlock * n_blocks)()
+    n = C.c_int64()
+    check(L.hdm_gen_batch_layout(C.byref(spec), blocks, C.byref(n)), what="hdm_gen_batch_layout")
+    p = Particles.zeros(n.value)
+    bid = np.zeros(n.value, np.int32)
+    ps = p.struct()
+    check(L.hdm_gen_batch_fill(C.byref(spec), blocks, C.byref(ps), _ptr(bid, C.c_int32)), what="hdm_gen_batch_fill")
+    l = gen_links(p, link_gen_scale, 1.0, spread, link_seed)
+    ls = l.struct()
+    check(L.hdm_gen_batch_thresholds(C.byref(spec), blocks, _ptr(bid, C.c_int32), C.byref(ls)),
+          what="hdm_gen_batch_thresholds")
+    water_abs = np.array([blocks[b].waterLevel for b in range(n_blocks)])
+    return Batch(spec, blocks, bid, water_abs, p, l)
+
+
+def set_blocks(ctx: "Context", block_id: np.ndarray, water: np.ndarray):
+    L = lib()
+    _declare_batch(L)
+    ctx._c(L.hdm_set_blocks(ctx._h, _ptr(block_id, C.c_int32), len(water), _ptr(water, C.c_double)),
+           "hdm_set_blocks")

@@ -248,3 +248,100 @@ int hdm_gen_links_fill(const hdm_particles_host *p, double linkGenScale, double 
 }
 
 }  // extern "C"
+
+// ---- batched form ------------------------------------------------------------
+extern "C" {
+
+int hdm_gen_batch_layout(const hdm_batch_spec *spec, hdm_batch_block *blocks, int64_t *nParticles) {
+    if (!spec || !blocks || !nParticles || spec->nBlocks <= 0 || spec->minParticles < 4 ||
+        spec->maxParticles < spec->minParticles || !(spec->a > 0))
+        return HDM_E_INVALID;
+    const int B = spec->nBlocks;
+    const double lo = std::log((double)spec->minParticles), hi = std::log((double)spec->maxParticles);
+    for (int b = 0; b < B; b++) {
+        double u = uniform01(spec->seed ^ 0xB10CB10CB10Cull, (uint64_t)b * 4 + 0);
+        double target = std::exp(lo + (hi - lo) * u);
+        // nx = 2 ny = 2 nz unit cells, 4 particles each: 8 ny^3 = target/... -> ny = cbrt(target / 8)
+        int ny = std::max(2, (int)std::lround(std::cbrt(target / 8.0)));
+        hdm_batch_block &k = blocks[b];
+        k.nx = 2 * ny;
+        k.ny = ny;
+        k.nz = ny;
+        k.particleCount = (int64_t)k.nx * k.ny * k.nz * 4;
+        double ut = uniform01(spec->seed ^ 0xB10CB10CB10Cull, (uint64_t)b * 4 + 1);
+        double uw = uniform01(spec->seed ^ 0xB10CB10CB10Cull, (uint64_t)b * 4 + 2);
+        k.threshold = spec->thrLo + (spec->thrHi - spec->thrLo) * ut;
+        k.waterLevel = spec->waterLo + (spec->waterHi - spec->waterLo) * uw;  // fraction; made absolute below
+    }
+    // shelf packing: 16 columns; column width / row depth = largest block in it
+    const int cols = 16, rows = (B + cols - 1) / cols;
+    std::vector<double> colW((size_t)cols, 0.0), rowD((size_t)rows, 0.0);
+    for (int b = 0; b < B; b++) {
+        colW[(size_t)(b % cols)] = std::max(colW[(size_t)(b % cols)], blocks[b].nx * spec->a);
+        rowD[(size_t)(b / cols)] = std::max(rowD[(size_t)(b / cols)], blocks[b].ny * spec->a);
+    }
+    int64_t start = 0;
+    for (int b = 0; b < B; b++) {
+        hdm_batch_block &k = blocks[b];
+        double x = 0, y = 0;
+        for (int c = 0; c < b % cols; c++) x += colW[(size_t)c] + spec->gap;
+        for (int r = 0; r < b / cols; r++) y += rowD[(size_t)r] + spec->gap;
+        k.origin[0] = x;
+        k.origin[1] = y;
+        k.origin[2] = 0.0;
+        double height = k.nz * spec->a;
+        k.waterLevel = k.origin[2] + k.waterLevel * height;
+        k.particleStart = start;
+        start += k.particleCount;
+    }
+    *nParticles = start;
+    return HDM_OK;
+}
+
+int hdm_gen_batch_fill(const hdm_batch_spec *spec, const hdm_batch_block *blocks, hdm_particles_host *out,
+                       int32_t *blockId) {
+    if (!spec || !blocks || !out) return HDM_E_INVALID;
+    for (int b = 0; b < spec->nBlocks; b++) {
+        const hdm_batch_block &k = blocks[b];
+        hdm_fcc_spec fs{};
+        fs.nx = k.nx;
+        fs.ny = k.ny;
+        fs.nz = k.nz;
+        fs.a = spec->a;
+        for (int c = 0; c < 3; c++) fs.origin[c] = k.origin[c];
+        fs.radius = spec->radius;
+        fs.jitter = spec->jitter;
+        fs.seed = spec->seed + 1000003ull * (uint64_t)(b + 1);
+        fs.clampXBelow = k.origin[0] + spec->clampX;
+        hdm_particles_host sub{};
+        int64_t off = k.particleStart;
+        sub.n = k.particleCount;
+        auto at = [&](double *p) { return p ? p + off : nullptr; };
+        sub.xCord = at(out->xCord); sub.yCord = at(out->yCord); sub.zCord = at(out->zCord);
+        sub.xTheta = at(out->xTheta); sub.yTheta = at(out->yTheta); sub.zTheta = at(out->zTheta);
+        sub.xDiff = at(out->xDiff); sub.yDiff = at(out->yDiff); sub.zDiff = at(out->zDiff);
+        sub.uDiff = at(out->uDiff); sub.vDiff = at(out->vDiff); sub.wDiff = at(out->wDiff);
+        sub.radius = at(out->radius);
+        sub.xForce = at(out->xForce); sub.yForce = at(out->yForce); sub.zForce = at(out->zForce);
+        sub.uForce = at(out->uForce); sub.vForce = at(out->vForce); sub.wForce = at(out->wForce);
+        sub.type = out->type ? out->type + off : nullptr;
+        int r = hdm_gen_fcc_fill(&fs, &sub);
+        if (r != HDM_OK) return r;
+        if (blockId)
+            for (int64_t i = 0; i < k.particleCount; i++) blockId[off + i] = b;
+    }
+    return HDM_OK;
+}
+
+int hdm_gen_batch_thresholds(const hdm_batch_spec *spec, const hdm_batch_block *blocks, const int32_t *blockId,
+                             hdm_links_host *links) {
+    if (!spec || !blocks || !blockId || !links || !links->maxLoad) return HDM_E_INVALID;
+    for (int64_t l = 0; l < links->n; l++) {
+        int32_t b = blockId[links->localPartIndA[l]];
+        if (b < 0 || b >= spec->nBlocks || blockId[links->localPartIndB[l]] != b) return HDM_E_INVALID;
+        links->maxLoad[l] *= blocks[b].threshold;
+    }
+    return HDM_OK;
+}
+
+}  // extern "C"

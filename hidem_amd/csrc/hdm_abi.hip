@@ -238,6 +238,7 @@ int hdm_particles_upload(hdm_ctx *ctx, const hdm_particles_host *p) {
     if (p->n >= (int64_t)INT32_MAX) return fail(ctx, HDM_E_CAPACITY, "more than int32 particles");
     if (c.L > 0 && p->n != c.n) return fail(ctx, HDM_E_STATE, "particle count changed while links are bound");
     (void)hipSetDevice(c.device);
+    if (p->n != c.n) c.nBlocks = 0;
     c.n = p->n;
     hipStream_t s = c.stream;
     const double *dsrc[6] = {p->xDiff, p->yDiff, p->zDiff, p->uDiff, p->vDiff, p->wDiff};
@@ -264,6 +265,7 @@ int hdm_particles_upload(hdm_ctx *ctx, const hdm_particles_host *p) {
     c.grid.built = false;
     c.neighBuilt = false;
     c.snapshotValid = false;
+    c.skinFused = false;
     return HDM_OK;
 }
 
@@ -553,6 +555,7 @@ int hdm_integrate(hdm_ctx *ctx, const hdm_integ_params *ip) {
         return fail(ctx, HDM_E_INVALID, "unknown integrator variant");
     Timer t(ctx->c, HDM_T_INTEGRATE);
     HIP_CHECK(ctx, hdm::launch_integrate(ctx->c, *ip));
+    ctx->c.skinFused = false;
     return HDM_OK;
 }
 
@@ -593,6 +596,7 @@ int hdm_skin_snapshot(hdm_ctx *ctx) {
     if (!valid(ctx)) return HDM_E_INVALID;
     HIP_CHECK(ctx, hdm::launch_snapshot(ctx->c));
     ctx->c.snapshotValid = true;
+    ctx->c.skinFused = false;
     return HDM_OK;
 }
 
@@ -622,6 +626,49 @@ int hdm_count_broken(hdm_ctx *ctx, int64_t *broken) {
     return HDM_OK;
 }
 
+int hdm_set_blocks(hdm_ctx *ctx, const int32_t *blockId, int32_t nBlocks, const double *waterLevel) {
+    if (!valid(ctx) || nBlocks < 0) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (nBlocks == 0) {
+        c.nBlocks = 0;
+        return HDM_OK;
+    }
+    if (!blockId || !waterLevel || c.n == 0) return fail(ctx, HDM_E_INVALID, "hdm_set_blocks: missing arrays or particles");
+    for (int64_t i = 0; i < c.n; i++)
+        if (blockId[i] < 0 || blockId[i] >= nBlocks) return fail(ctx, HDM_E_INVALID, "hdm_set_blocks: block id out of range");
+    HIP_CHECK(ctx, c.blockId.ensure(sizeof(int32_t) * (size_t)c.n));
+    HIP_CHECK(ctx, c.blockWater.ensure(sizeof(double) * (size_t)nBlocks));
+    HIP_CHECK(ctx, hipMemcpyAsync(c.blockId.ptr, blockId, sizeof(int32_t) * c.n, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(ctx, hipMemcpyAsync(c.blockWater.ptr, waterLevel, sizeof(double) * nBlocks, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
+    c.nBlocks = nBlocks;
+    return HDM_OK;
+}
+
+int hdm_bw_probe(hdm_ctx *ctx, int32_t kind, int64_t bytes, int32_t iters, double *gbps) {
+    if (!valid(ctx) || (kind != HDM_PROBE_COPY && kind != HDM_PROBE_CT_RMW) || bytes < (1 << 20) || iters < 1)
+        return HDM_E_INVALID;
+    Context &c = ctx->c;
+    HIP_CHECK(ctx, c.probe.ensure((size_t)bytes));
+    HIP_CHECK(ctx, hdm::launch_probe_zero(c, c.probe.ptr, bytes));
+    HIP_CHECK(ctx, hdm::launch_probe(c, kind, c.probe.ptr, bytes));  // warm-up
+    hipEvent_t e0, e1;
+    HIP_CHECK(ctx, hipEventCreate(&e0));
+    HIP_CHECK(ctx, hipEventCreate(&e1));
+    HIP_CHECK(ctx, hipEventRecord(e0, c.stream));
+    for (int i = 0; i < iters; i++) HIP_CHECK(ctx, hdm::launch_probe(c, kind, c.probe.ptr, bytes));
+    HIP_CHECK(ctx, hipEventRecord(e1, c.stream));
+    HIP_CHECK(ctx, hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // copy: bytes/2 read + bytes/2 written; rmw6: bytes read + bytes written
+    double moved = (kind == HDM_PROBE_COPY) ? (double)(bytes / 2 / 32 * 32) * 2.0 : (double)(bytes / 48 * 48) * 2.0;
+    if (gbps) *gbps = moved * iters / ((double)ms * 1e-3) / 1e9;
+    return HDM_OK;
+}
+
 int hdm_set_timestep(hdm_ctx *ctx, int64_t ts) {
     if (!valid(ctx) || ts < 0) return HDM_E_INVALID;
     ctx->c.ts = ts;
@@ -630,14 +677,22 @@ int hdm_set_timestep(hdm_ctx *ctx, int64_t ts) {
 }
 
 // gridService (HiDEMLib.h:533-628) -> computePartForce -> applyForce ->
-// linkUpdate (ts*dt > breakTime) -> finalizeTs, nSteps times.
+// linkUpdate (ts*dt > breakTime) -> finalizeTs, nSteps times.  The integrator
+// of each step also reduces the rebuild-test quantities of the new state
+// (max+ Diff and max |x - x0|^2 into red[32..36]), so the next step's
+// decision costs one 40-byte read instead of a separate pass.
 int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats *stats) {
     if (!valid(ctx) || !rp || nSteps < 0) return HDM_E_INVALID;
     Context &c = ctx->c;
+    if (rp->integ.variant != HDM_INTEG_ICEBLOCK && rp->integ.variant != HDM_INTEG_GLACIER)
+        return fail(ctx, HDM_E_INVALID, "unknown integrator variant");
     if (!c.linksBuilt) {
         int r = hdm_links_build(ctx);
         if (r != HDM_OK) return r;
     }
+    uint64_t *skin = c.red.as<uint64_t>() + 32;
+    const bool wantSkin = rp->policy != HDM_REBUILD_INTERVAL;
+    const bool wantDisp = rp->policy == HDM_REBUILD_EXACT;
     auto t0 = std::chrono::steady_clock::now();
     int64_t rebuilds = 0;
     for (int64_t step = 0; step < nSteps; step++) {
@@ -647,15 +702,28 @@ int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *reb
         } else if (rp->policy == HDM_REBUILD_HEURISTIC) {
             if (c.ts % 10 == 0) {
                 double d = 0;
-                int r = hdm_skin_heuristic(ctx, 10, &d);
-                if (r != HDM_OK) return r;
+                if (c.skinFused) {
+                    uint64_t k[3];
+                    HIP_CHECK(ctx, d2h(k, skin, sizeof(k), c.stream));
+                    double xm = hdm::ordered_value(k[0]), ym = hdm::ordered_value(k[1]), zm = hdm::ordered_value(k[2]);
+                    d = 10 * std::sqrt((xm * xm) + (ym * ym) + (zm * zm));
+                } else {
+                    int r = hdm_skin_heuristic(ctx, 10, &d);
+                    if (r != HDM_OK) return r;
+                }
                 c.accumulatedDist += d;
                 if (c.accumulatedDist >= rp->interactDist / 2.0) regen = true;
             }
         } else if (!regen) {
             double d = 0;
-            int r = hdm_skin_exact(ctx, &d);
-            if (r != HDM_OK) return r;
+            if (c.skinFused && c.skinFusedDisp) {
+                uint64_t k = 0;
+                HIP_CHECK(ctx, d2h(&k, skin + 4, sizeof(k), c.stream));
+                d = std::sqrt(hdm::ordered_value(k));
+            } else {
+                int r = hdm_skin_exact(ctx, &d);
+                if (r != HDM_OK) return r;
+            }
             if (2.0 * d >= rp->interactDist - rp->force.attractDistance) regen = true;
         }
         if (rebuildLog) rebuildLog[step] = regen ? 1 : 0;
@@ -668,8 +736,14 @@ int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *reb
             rebuilds++;
         }
         int r = hdm_force(ctx, &rp->force);
-        if (r == HDM_OK) r = hdm_integrate(ctx, &rp->integ);
-        if (r == HDM_OK && (double)c.ts * rp->dt > rp->breakTime) r = hdm_break(ctx, &rp->brk, nullptr);
+        if (r != HDM_OK) return r;
+        {
+            Timer t(c, HDM_T_INTEGRATE);
+            HIP_CHECK(ctx, hdm::launch_integrate(c, rp->integ, wantSkin ? skin : nullptr, wantDisp));
+        }
+        c.skinFused = wantSkin;
+        c.skinFusedDisp = wantDisp;
+        if ((double)c.ts * rp->dt > rp->breakTime) r = hdm_break(ctx, &rp->brk, nullptr);
         if (r != HDM_OK) return r;
         c.ts++;
     }

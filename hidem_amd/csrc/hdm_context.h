@@ -72,6 +72,9 @@ struct Context {
     // particles
     int64_t n = 0;
     DevBuf x, y, z, tx, ty, tz, diff[6], radius, force[6], type;
+    // batched form: block id per particle, waterline per block (nBlocks = 0: unused)
+    int32_t nBlocks = 0;
+    DevBuf blockId, blockWater;
     // links
     int64_t L = 0;
     bool hasMaxLoad = false;
@@ -92,7 +95,10 @@ struct Context {
     // reductions / scalars
     DevBuf red;            // u64 scratch for bbox / max reductions and counters
     DevBuf x0, y0, z0;     // positions at the last neighbour-list build
+    DevBuf probe;          // bandwidth-probe buffer (hdm_bw_probe)
     bool snapshotValid = false;
+    bool skinFused = false;      // red[32..34] hold max+ Diff from the last integrate of hdm_run
+    bool skinFusedDisp = false;  // red[36] holds max |x - x0|^2 from that integrate
     // run state
     int64_t ts = 0;
     double accumulatedDist = 0;
@@ -126,12 +132,15 @@ hipError_t launch_link_fill(Context &c);
 hipError_t launch_ct_import(Context &c);   // ctStage (checkpoint order) -> ct (tiled)
 hipError_t launch_ct_export(Context &c);   // ct (tiled) -> ctStage (checkpoint order)
 hipError_t launch_force(Context &c, const hdm_force_params &fp);
-hipError_t launch_integrate(Context &c, const hdm_integ_params &ip);
+hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *skin = nullptr,
+                            bool withDisplacement = false);
 hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter);
 hipError_t launch_skin_heuristic(Context &c, uint64_t *out3);
 hipError_t launch_skin_exact(Context &c, uint64_t *out1);
 hipError_t launch_snapshot(Context &c);
 hipError_t launch_count_broken(Context &c, uint64_t *counter);
+hipError_t launch_probe(Context &c, int kind, void *buf, int64_t bytes);
+hipError_t launch_probe_zero(Context &c, void *buf, int64_t bytes);
 
 // Monotone map between doubles and unsigned 64-bit keys (used for
 // order-independent, hence deterministic, atomic min/max of doubles).

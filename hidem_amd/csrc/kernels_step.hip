@@ -216,6 +216,12 @@ struct IntegArgs {
     const uint8_t *type;
     int64_t n;
     hdm_integ_params ip;
+    // rebuild-test reductions of the new state (nullptr: none).  skin[0..2]:
+    // max(0, max Diff) per axis (HiDEMLib.h:424-461); skin[4]: max |x - x0|^2.
+    uint64_t *skin;
+    const double *x0, *y0, *z0;   // nullptr: no displacement reduction
+    const int32_t *blockId;       // batched form: waterline of the particle's block
+    const double *blockWater;     // (nullptr: ip.waterLevel for every particle)
 };
 
 // Damped central difference (HiDEMDefTimeIntegration, HiDEMDefaultFunctions.h
@@ -250,14 +256,16 @@ __device__ __forceinline__ void central_difference(double dragCoeff, double mass
 // applyForce: IceBlock (MainIceBlock.cpp:299-400, clamp = HDM_TYPE_FIXED) or
 // GlacierDem (MainDem.cpp:145-259 with an analytic bed, constant external
 // force, HiDEMDefVelocityLimit and HiDEMDefBoundaryCondition).
+// Integrates particle i; returns its new coordinates and translational Diff.
 template <class FO>
-__global__ void __launch_bounds__(TPB) k_integrate(IntegArgs<FO> a) {
-    const int64_t i = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
-    if (i >= a.n) return;
+__device__ __forceinline__ void integrate_particle(const IntegArgs<FO> &a, int64_t i, double newCord[3],
+                                                   double newDiff[3]) {
     const hdm_integ_params &ip = a.ip;
     if (ip.variant == HDM_INTEG_ICEBLOCK && (a.type[i] & HDM_TYPE_FIXED)) {
         a.d0[i] = 0; a.d1[i] = 0; a.d2[i] = 0;
         a.d3[i] = 0; a.d4[i] = 0; a.d5[i] = 0;
+        newCord[0] = a.x[i]; newCord[1] = a.y[i]; newCord[2] = a.z[i];
+        newDiff[0] = 0; newDiff[1] = 0; newDiff[2] = 0;
         return;
     }
     double cord[3] = {a.x[i], a.y[i], a.z[i]};
@@ -270,11 +278,12 @@ __global__ void __launch_bounds__(TPB) k_integrate(IntegArgs<FO> a) {
     const double zCord = cord[2];
     const double particleMass = ip.rhoIce * diameter * diameter * diameter;
     const double particleMomentInertia = particleMass * diameter * diameter / 6.0;
+    const double waterLevel = a.blockWater ? a.blockWater[a.blockId[i]] : ip.waterLevel;
     double dragCoeff;
     if (ip.variant == HDM_INTEG_ICEBLOCK) {
-        dragCoeff = (zCord > ip.waterLevel) ? ip.airDrag : ip.waterDrag;
-        if (zCord > ip.waterLevel) F[2] += -ip.g * particleMass;
-        if (zCord <= ip.waterLevel) F[2] += ip.lift * particleMass;
+        dragCoeff = (zCord > waterLevel) ? ip.airDrag : ip.waterDrag;
+        if (zCord > waterLevel) F[2] += -ip.g * particleMass;
+        if (zCord <= waterLevel) F[2] += ip.lift * particleMass;
         central_difference(dragCoeff, particleMass, particleMomentInertia, ip, F, diff, cord, cordPrev, theta,
                            thetaPrev, vel);
     } else {
@@ -283,13 +292,13 @@ __global__ void __launch_bounds__(TPB) k_integrate(IntegArgs<FO> a) {
         if ((zCord + radius) - bedzCord < ip.bedInteract) {
             double d = ip.bedDrag * fric;
             dragCoeff = (ip.frictionCap < d) ? ip.frictionCap : d;
-        } else if (zCord > ip.waterLevel) {
+        } else if (zCord > waterLevel) {
             dragCoeff = ip.airDrag;
         } else {
             dragCoeff = ip.waterDrag;
         }
-        if (zCord > ip.waterLevel) F[2] += -ip.g * particleMass;
-        if (zCord <= ip.waterLevel) F[2] += ip.lift * particleMass;
+        if (zCord > waterLevel) F[2] += -ip.g * particleMass;
+        if (zCord <= waterLevel) F[2] += ip.lift * particleMass;
         if (zCord + radius < bedzCord) F[2] += ip.bedStiffness * (bedzCord - (zCord + radius));
 #pragma unroll
         for (int c = 0; c < 6; c++) F[c] += ip.ext[c];
@@ -325,6 +334,36 @@ __global__ void __launch_bounds__(TPB) k_integrate(IntegArgs<FO> a) {
     a.d3[i] = theta[0] - thetaPrev[0];
     a.d4[i] = theta[1] - thetaPrev[1];
     a.d5[i] = theta[2] - thetaPrev[2];
+    for (int c = 0; c < 3; c++) {
+        newCord[c] = cord[c];
+        newDiff[c] = cord[c] - cordPrev[c];
+    }
+}
+
+template <class FO>
+__global__ void __launch_bounds__(TPB) k_integrate(IntegArgs<FO> a) {
+    const int64_t i = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    const bool active = i < a.n;
+    double cord[3] = {0, 0, 0}, diff[3] = {0, 0, 0};
+    if (active) integrate_particle(a, i, cord, diff);
+    if (a.skin == nullptr) return;
+    // reductions over the new state; inactive lanes contribute 0
+    double m[3] = {0.0, 0.0, 0.0}, d2 = 0.0;
+    if (active) {
+        for (int c = 0; c < 3; c++) m[c] = diff[c] > 0.0 ? diff[c] : 0.0;
+        if (a.x0) {
+            double ddx = cord[0] - a.x0[i], ddy = cord[1] - a.y0[i], ddz = cord[2] - a.z0[i];
+            d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) m[c] = wave_max(m[c]);
+    d2 = wave_max(d2);
+    if ((threadIdx.x & 63) == 0) {
+        for (int c = 0; c < 3; c++)
+            if (m[c] > 0.0) atomicMax((unsigned long long *)&a.skin[c], (unsigned long long)ordered_key(m[c]));
+        if (a.x0 && d2 > 0.0) atomicMax((unsigned long long *)&a.skin[4], (unsigned long long)ordered_key(d2));
+    }
 }
 
 // ---- link break --------------------------------------------------------------
@@ -431,8 +470,14 @@ hipError_t force_impl(Context &c, const hdm_force_params &fp) {
 }
 
 template <class FO>
-hipError_t integrate_impl(Context &c, const hdm_integ_params &ip) {
+hipError_t integrate_impl(Context &c, const hdm_integ_params &ip, uint64_t *skin, bool withDisplacement) {
     IntegArgs<FO> a;
+    a.blockId = c.nBlocks > 0 ? c.blockId.as<int32_t>() : nullptr;
+    a.blockWater = c.nBlocks > 0 ? c.blockWater.as<double>() : nullptr;
+    a.skin = skin;
+    a.x0 = withDisplacement ? c.x0.as<double>() : nullptr;
+    a.y0 = withDisplacement ? c.y0.as<double>() : nullptr;
+    a.z0 = withDisplacement ? c.z0.as<double>() : nullptr;
     a.x = c.x.as<double>(); a.y = c.y.as<double>(); a.z = c.z.as<double>();
     a.tx = c.tx.as<double>(); a.ty = c.ty.as<double>(); a.tz = c.tz.as<double>();
     a.d0 = c.diff[0].as<double>(); a.d1 = c.diff[1].as<double>(); a.d2 = c.diff[2].as<double>();
@@ -455,10 +500,14 @@ hipError_t launch_force(Context &c, const hdm_force_params &fp) {
     return force_impl<float, float, float>(c, fp);
 }
 
-hipError_t launch_integrate(Context &c, const hdm_integ_params &ip) {
+hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *skin, bool withDisplacement) {
     if (c.n == 0) return hipSuccess;
-    if (c.prec == HDM_PREC_FP64) return integrate_impl<double>(c, ip);
-    return integrate_impl<float>(c, ip);
+    if (skin) {
+        hipError_t e = launch_fill_u64(c.stream, skin, ordered_key(0.0), 5);
+        if (e != hipSuccess) return e;
+    }
+    if (c.prec == HDM_PREC_FP64) return integrate_impl<double>(c, ip, skin, withDisplacement);
+    return integrate_impl<float>(c, ip, skin, withDisplacement);
 }
 
 hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter) {

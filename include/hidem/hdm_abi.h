@@ -214,6 +214,14 @@ int  hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *re
 int  hdm_set_timestep(hdm_ctx *ctx, int64_t ts);
 int  hdm_count_broken(hdm_ctx *ctx, int64_t *broken);
 
+/* ---- HBM bandwidth probes (the idea of Benchmark/BenchMain.cpp:14-83):
+ *      kind 0 = streaming copy (read `bytes`, write `bytes`),
+ *      kind 1 = six-array read-modify-write, the access pattern of the CT
+ *      side update (reads and writes `bytes` in total each way).
+ *      Reports achieved GB/s (bytes read + written / time, HIP events). ------ */
+enum { HDM_PROBE_COPY = 0, HDM_PROBE_CT_RMW = 1 };
+int  hdm_bw_probe(hdm_ctx *ctx, int32_t kind, int64_t bytes, int32_t iters, double *gbps);
+
 /* ---- seeded FCC block and link generator (setup; build-defined, replaces the
  *      non-deterministic FCCParticleGenerator/LinkGenerator) ---------------- */
 typedef struct hdm_fcc_spec {
@@ -235,6 +243,43 @@ int  hdm_gen_fcc_fill(const hdm_fcc_spec *spec, hdm_particles_host *out);
 int  hdm_gen_links_count(const hdm_particles_host *p, double linkGenScale, int64_t *nLinks);
 int  hdm_gen_links_fill(const hdm_particles_host *p, double linkGenScale, double threshold, double spread,
                         uint64_t seed, hdm_links_host *out);
+
+/* ---- batched form: many independent ice blocks in one state --------------
+ * Block b gets N_b particles, log-uniform in [minParticles, maxParticles]
+ * (FCC, nx = 2*ny = 2*nz unit cells), its own fracture-threshold mean
+ * U[thrLo, thrHi] (per link: mean*(1 + spread*U(-1,1))), its own waterline
+ * U[waterLo, waterHi] x block height, and a clamp on its first clampX metres.
+ * Blocks are packed on a 16 x ceil(B/16) shelf grid in x-y with `gap` metres
+ * between them (gap > search distance: no cross-block neighbours or links).
+ * Particles of a block are contiguous; blockId[i] names the block. */
+typedef struct hdm_batch_spec {
+    int32_t nBlocks;
+    int64_t minParticles, maxParticles;
+    double a, radius, jitter, gap;
+    double thrLo, thrHi, spread;
+    double waterLo, waterHi;
+    double clampX;
+    uint64_t seed;
+} hdm_batch_spec;
+
+typedef struct hdm_batch_block {
+    int32_t nx, ny, nz;
+    double origin[3];
+    int64_t particleStart, particleCount;
+    double threshold;           /* mean fracture threshold of the block */
+    double waterLevel;          /* absolute z of the block's waterline */
+} hdm_batch_block;
+
+int  hdm_gen_batch_layout(const hdm_batch_spec *spec, hdm_batch_block *blocks, int64_t *nParticles);
+int  hdm_gen_batch_fill(const hdm_batch_spec *spec, const hdm_batch_block *blocks, hdm_particles_host *out,
+                        int32_t *blockId);
+/* Scales per-link thresholds (generated with threshold 1) by the mean of the
+ * block that owns end A. */
+int  hdm_gen_batch_thresholds(const hdm_batch_spec *spec, const hdm_batch_block *blocks, const int32_t *blockId,
+                              hdm_links_host *links);
+/* Per-particle block ids and per-block waterlines for the integrator
+ * (replaces hdm_integ_params.waterLevel per particle); nBlocks = 0 clears. */
+int  hdm_set_blocks(hdm_ctx *ctx, const int32_t *blockId, int32_t nBlocks, const double *waterLevel);
 
 /* ---- TOML-driven simulation (HiDEMLib::setup + the sample main loop) --------
  * argv holds the command line after the program name, as the reference's

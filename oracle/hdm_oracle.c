@@ -492,11 +492,12 @@ void hdmo_integrate(hdmo_particles *p, const hdmo_integ_params *ip)
         double dragCoeff;
         double particleMass = ip->rhoIce * diameter * diameter * diameter;
         double particleMomentInertia = particleMass * diameter * diameter / 6.0;
+        double waterLevel = ip->blockWater ? ip->blockWater[ip->blockId[i]] : ip->waterLevel;
 
         if (ip->variant == 0) {
-            if (zCord > ip->waterLevel) dragCoeff = ip->airDrag; else dragCoeff = ip->waterDrag;
-            if (zCord > ip->waterLevel) F[2] += -ip->g * particleMass;
-            if (zCord <= ip->waterLevel) F[2] += ip->lift * particleMass;
+            if (zCord > waterLevel) dragCoeff = ip->airDrag; else dragCoeff = ip->waterDrag;
+            if (zCord > waterLevel) F[2] += -ip->g * particleMass;
+            if (zCord <= waterLevel) F[2] += ip->lift * particleMass;
             central_difference(dragCoeff, particleMass, particleMomentInertia, ip, F, diff, cord, cordPrev, theta, thetaPrev, vel);
         } else {
             double bedzCord = (cord[0] < ip->bedXEnd) ? ip->bedZ : ip->deepZ;
@@ -504,13 +505,13 @@ void hdmo_integrate(hdmo_particles *p, const hdmo_integ_params *ip)
             if ((zCord + radius) - bedzCord < ip->bedInteract) {
                 double a = ip->bedDrag * fric;
                 dragCoeff = (ip->frictionCap < a) ? ip->frictionCap : a;   /* std::min(a, cap) */
-            } else if (zCord > ip->waterLevel) {
+            } else if (zCord > waterLevel) {
                 dragCoeff = ip->airDrag;
             } else {
                 dragCoeff = ip->waterDrag;
             }
-            if (zCord > ip->waterLevel) F[2] += -ip->g * particleMass;
-            if (zCord <= ip->waterLevel) F[2] += ip->lift * particleMass;
+            if (zCord > waterLevel) F[2] += -ip->g * particleMass;
+            if (zCord <= waterLevel) F[2] += ip->lift * particleMass;
             if (zCord + radius < bedzCord) F[2] += ip->bedStiffness * (bedzCord - (zCord + radius));
             for (int c = 0; c < 6; c++) F[c] += ip->ext[c];
             central_difference(dragCoeff, particleMass, particleMomentInertia, ip, F, diff, cord, cordPrev, theta, thetaPrev, vel);

@@ -112,6 +112,10 @@ typedef struct hdmo_integ_params {
     int32_t velocityLimit;
     int32_t bound[6];      /* xMin, xMax, yMin, yMax, zMin, zMax enabled */
     double lower[3], upper[3];
+    /* batched form (build-defined): waterline of particle i is
+     * blockWater[blockId[i]] when blockWater != NULL */
+    const int32_t *blockId;
+    const double *blockWater;
 } hdmo_integ_params;
 
 /* HiDEMDefLinkBreakFunc (HiDEMDefaultFunctions.h:207-242). */

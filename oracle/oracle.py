@@ -62,7 +62,8 @@ class IntegParams(C.Structure):
                                            "bedZ", "bedXEnd", "deepZ", "frictionXEnd")] +
                 [("ext", C.c_double * 6), ("maxVel", C.c_double), ("maxRot", C.c_double),
                  ("velocityLimit", C.c_int32), ("bound", C.c_int32 * 6),
-                 ("lower", C.c_double * 3), ("upper", C.c_double * 3)])
+                 ("lower", C.c_double * 3), ("upper", C.c_double * 3),
+                 ("blockId", C.POINTER(C.c_int32)), ("blockWater", C.POINTER(C.c_double))])
 
 
 class BreakParams(C.Structure):
@@ -161,9 +162,16 @@ def force_params(fp):
     return ForceParams(*[getattr(fp, f) for f, _ in ForceParams._fields_])
 
 
-def integ_params(ip):
+def integ_params(ip, block_id=None, block_water=None):
+    """Copies an hdm_integ_params; optional per-particle block ids and
+    per-block waterlines (numpy arrays kept alive by the caller)."""
     o = IntegParams()
+    if block_id is not None:
+        o.blockId = block_id.ctypes.data_as(C.POINTER(C.c_int32))
+        o.blockWater = block_water.ctypes.data_as(C.POINTER(C.c_double))
     for f, _ in IntegParams._fields_:
+        if f in ("blockId", "blockWater"):
+            continue
         v = getattr(ip, f)
         if f in ("ext", "bound", "lower", "upper"):
             arr = getattr(o, f)
@@ -237,8 +245,9 @@ def link_lists(n_particles, l):
 class OracleSim:
     """Owns copies of the particle/link arrays and steps them on the CPU."""
 
-    def __init__(self, p, l, setup, fast=False):
+    def __init__(self, p, l, setup, fast=False, block_id=None, block_water=None):
         self.L = lib(fast)
+        self._blocks = (block_id, block_water)
         self.p = p.copy()
         self.l = l.copy()
         self._ps = particles_struct(self.p)
@@ -247,7 +256,7 @@ class OracleSim:
         s.p = C.pointer(self._ps)
         s.l = C.pointer(self._ls)
         s.fp = force_params(setup.force)
-        s.ip = integ_params(setup.integ)
+        s.ip = integ_params(setup.integ, block_id, block_water)
         s.bp = break_params(setup.brk)
         s.searchDistance = setup.search_distance
         s.interactDist = setup.interact_dist
