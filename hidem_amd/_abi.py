@@ -26,6 +26,8 @@ HDM_INTEG_GLACIER = 1
 HDM_REBUILD_HEURISTIC = 0
 HDM_REBUILD_EXACT = 1
 HDM_REBUILD_INTERVAL = 2
+HDM_ORDER_LATTICE = 0
+HDM_ORDER_MORTON = 1
 HDM_PROBE_COPY = 0
 HDM_PROBE_CT_RMW = 1
 TIMER_NAMES = ("force", "integrate", "break", "skin", "grid", "neigh", "links")
@@ -90,7 +92,7 @@ class HdmRunStats(C.Structure):
 class HdmFccSpec(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("a", C.c_double),
                 ("origin", C.c_double * 3), ("radius", C.c_double), ("jitter", C.c_double),
-                ("seed", C.c_uint64), ("clampXBelow", C.c_double)]
+                ("seed", C.c_uint64), ("clampXBelow", C.c_double), ("order", C.c_int32)]
 
 
 class HdmSetupInfo(C.Structure):
@@ -263,8 +265,9 @@ def check(rc, ctx=None, what=""):
 
 
 # ---- generator -----------------------------------------------------------------
-def gen_fcc(nx, ny, nz, a, radius, jitter, seed, origin=(0.0, 0.0, 0.0), clamp_x_below=-np.inf) -> Particles:
-    spec = HdmFccSpec(nx, ny, nz, a, (C.c_double * 3)(*origin), radius, jitter, seed, clamp_x_below)
+def gen_fcc(nx, ny, nz, a, radius, jitter, seed, origin=(0.0, 0.0, 0.0), clamp_x_below=-np.inf,
+            order=HDM_ORDER_LATTICE) -> Particles:
+    spec = HdmFccSpec(nx, ny, nz, a, (C.c_double * 3)(*origin), radius, jitter, seed, clamp_x_below, order)
     n = C.c_int64()
     check(lib().hdm_gen_fcc_count(C.byref(spec), C.byref(n)), what="hdm_gen_fcc_count")
     p = Particles.zeros(n.value)

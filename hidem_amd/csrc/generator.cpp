@@ -7,9 +7,12 @@
 //   * FCC unit cell basis of FCCParticleGenerator::createUnitCell
 //     (FCCParticleGenerator.h:47-50): (0,0,0), (0,a/2,a/2), (a/2,0,a/2),
 //     (a/2,a/2,0);
-//   * particle index = global key = ((k*ny + j)*nx + i)*4 + basis;
+//   * particle index = global key = rank(unit cell)*4 + basis, where the unit
+//     cells are ranked in lattice order ((k*ny + j)*nx + i) or along a Morton
+//     (Z-order) curve, the cell-local order the reference's redistribution
+//     sort produces (HiDEMDistribution.h:150-169);
 //   * jitter: each coordinate + jitter*(2u-1), u from a counter-based
-//     SplitMix64 hash of (seed, 3*index + component);
+//     SplitMix64 hash of (seed, 3*site + component), site = lattice index;
 //   * links: pairs with dist <= (r_i + r_j)*linkGenScale, dist computed as in
 //     LinkGenerator::genNewLinks (LinkGenerator.h:83-89); end A is the smaller
 //     key (:64), links sorted by (A, B); initialLen = dist (:99);
@@ -37,6 +40,18 @@ inline double uniform01(uint64_t seed, uint64_t counter) {
     uint64_t z = splitmix64(seed ^ splitmix64(counter));
     return (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
+
+// Interleaves the low 21 bits of x, y, z (x in bit 0): Z-order curve key.
+uint64_t spread_bits3(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+uint64_t morton3(uint64_t x, uint64_t y, uint64_t z) { return spread_bits3(x) | (spread_bits3(y) << 1) | (spread_bits3(z) << 2); }
 
 int nthreads() {
     unsigned h = std::thread::hardware_concurrency();
@@ -151,17 +166,35 @@ int hdm_gen_fcc_fill(const hdm_fcc_spec *spec, hdm_particles_host *out) {
     if (!out || out->n != n || !out->xCord || !out->yCord || !out->zCord || !out->radius) return HDM_E_INVALID;
     const double a = spec->a, h = spec->a / 2.0;
     const double basis[4][3] = {{0, 0, 0}, {0, h, h}, {h, 0, h}, {h, h, 0}};
-    const int64_t nx = spec->nx, ny = spec->ny;
+    const int64_t nx = spec->nx, ny = spec->ny, nz = spec->nz;
+    // rank -> unit cell (lattice order: x fastest; Morton: sorted by Z-order code)
+    std::vector<int64_t> cellOfRank;
+    if (spec->order == HDM_ORDER_MORTON) {
+        cellOfRank.resize((size_t)(nx * ny * nz));
+        std::vector<std::pair<uint64_t, int64_t>> keyed((size_t)(nx * ny * nz));
+        parallel_for(nx * ny * nz, [&](int, int64_t b, int64_t e) {
+            for (int64_t c = b; c < e; c++) {
+                int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+                keyed[(size_t)c] = {morton3((uint64_t)i, (uint64_t)j, (uint64_t)k), c};
+            }
+        });
+        std::sort(keyed.begin(), keyed.end());
+        for (size_t r = 0; r < keyed.size(); r++) cellOfRank[r] = keyed[r].second;
+    } else if (spec->order != HDM_ORDER_LATTICE) {
+        return HDM_E_INVALID;
+    }
     parallel_for(n, [&](int, int64_t b, int64_t e) {
         for (int64_t idx = b; idx < e; idx++) {
             int64_t cell = idx >> 2;
+            if (!cellOfRank.empty()) cell = cellOfRank[(size_t)cell];
             int bi = (int)(idx & 3);
             int64_t i = cell % nx, j = (cell / nx) % ny, k = cell / (nx * ny);
             double pos[3] = {spec->origin[0] + (double)i * a + basis[bi][0],
                              spec->origin[1] + (double)j * a + basis[bi][1],
                              spec->origin[2] + (double)k * a + basis[bi][2]};
             for (int c = 0; c < 3; c++) {
-                double u = uniform01(spec->seed, (uint64_t)idx * 3u + (uint64_t)c);
+                // keyed by the lattice site, so both storage orders hold the same system
+                double u = uniform01(spec->seed, (uint64_t)(cell * 4 + bi) * 3u + (uint64_t)c);
                 pos[c] += spec->jitter * (2.0 * u - 1.0);
             }
             out->xCord[idx] = pos[0];
@@ -313,6 +346,7 @@ int hdm_gen_batch_fill(const hdm_batch_spec *spec, const hdm_batch_block *blocks
         fs.jitter = spec->jitter;
         fs.seed = spec->seed + 1000003ull * (uint64_t)(b + 1);
         fs.clampXBelow = k.origin[0] + spec->clampX;
+        fs.order = HDM_ORDER_MORTON;
         hdm_particles_host sub{};
         int64_t off = k.particleStart;
         sub.n = k.particleCount;

@@ -528,7 +528,7 @@ int hdm_linklists_download(hdm_ctx *ctx, int32_t *counts, int64_t *starts, int32
         for (int64_t p = 0; p < c.n; p++)
             for (int32_t k = 0; k < cnt[(size_t)p]; k++) {
                 size_t s = (size_t)(p >> 6) * 64 * w + (size_t)k * 64 + (size_t)(p & 63);
-                int32_t l = sl[s];
+                int32_t l = sl[s] & hdm::SLOT_LINK_MASK;
                 if (linkId) linkId[pos] = l;
                 if (other) other[pos] = so[s];
                 // the end is whichever of the link's two sides maps to this slot

@@ -114,6 +114,10 @@ struct Context {
     size_t forceBytes() const { return prec == HDM_PREC_FP64 ? 8 : 4; }
 };
 
+// Link slots hold the link id; the sign bit marks a broken link (state == 0).
+constexpr int32_t SLOT_BROKEN = (int32_t)0x80000000u;
+constexpr int32_t SLOT_LINK_MASK = 0x7fffffff;
+
 // Launch helpers shared by the kernel files.
 int xcd_grid(int64_t nBlocks);
 

@@ -153,6 +153,9 @@ SetupInfo configure(ArgParser &parser) {
     o.fcc.jitter = arg_or<double>(parser, "hdm.jitter", 0.0);
     o.fcc.seed = (uint64_t)arg_or<int64_t>(parser, "lattice.seed", 1);
     o.fcc.clampXBelow = arg_or<double>(parser, "hdm.clampXBelow", -std::numeric_limits<double>::infinity());
+    std::string order = arg_or<std::string>(parser, "hdm.order", "lattice");
+    if (order != "lattice" && order != "morton") throw TomlError("hdm.order must be \"lattice\" or \"morton\"");
+    o.fcc.order = order == "morton" ? HDM_ORDER_MORTON : HDM_ORDER_LATTICE;
     o.spread = arg_or<double>(parser, "hdm.maxLoadSpread", 0.0);
     o.linkSeed = (uint64_t)arg_or<int64_t>(parser, "hdm.linkSeed", 7);
     o.threshold = s.gp.maxLoad;

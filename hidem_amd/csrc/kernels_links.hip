@@ -55,6 +55,7 @@ __global__ void __launch_bounds__(TPB) k_link_fill(const int32_t *__restrict__ A
 }
 
 __global__ void __launch_bounds__(TPB) k_link_sort(const int32_t *__restrict__ A, const int32_t *__restrict__ B,
+                                                    const uint8_t *__restrict__ state,
                                                     int64_t n, int32_t width, const uint32_t *__restrict__ count,
                                                     int32_t *__restrict__ slotLink, int32_t *__restrict__ slotOther,
                                                     int32_t *__restrict__ sidePos) {
@@ -82,6 +83,9 @@ __global__ void __launch_bounds__(TPB) k_link_sort(const int32_t *__restrict__ A
         }
         slotOther[s] = other;
         sidePos[2 * (int64_t)l + end] = (int32_t)s;
+        // the slot word carries the link's broken flag in its sign bit, so the
+        // force gather needs no per-link state lookup
+        if (state[l] == 0) slotLink[s] = l | SLOT_BROKEN;
     }
 }
 
@@ -152,7 +156,7 @@ hipError_t launch_link_fill(Context &c) {
         hipLaunchKernelGGL(k_link_fill, dim3(blocks_for(c.L)), dim3(TPB), 0, c.stream, c.la.as<int32_t>(),
                            c.lb.as<int32_t>(), c.L, c.maxLinks, c.linkCursor.as<uint32_t>(), c.slotLink.as<int32_t>());
     hipLaunchKernelGGL(k_link_sort, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.la.as<int32_t>(),
-                       c.lb.as<int32_t>(), c.n, c.maxLinks, c.linkCount.as<uint32_t>(), c.slotLink.as<int32_t>(),
+                       c.lb.as<int32_t>(), c.state.as<uint8_t>(), c.n, c.maxLinks, c.linkCount.as<uint32_t>(), c.slotLink.as<int32_t>(),
                        c.slotOther.as<int32_t>(), c.sidePos.as<int32_t>());
     return hipGetLastError();
 }

@@ -14,9 +14,12 @@
 // The neighbour list and the link sides live in wave tiles: slot k of the 64
 // particles of a tile is one contiguous run, so every list/CT access of a
 // wavefront is a single coalesced 256 B / 512 B transaction.  Partner state
-// (coordinates, Diff, radius) is gathered through L2; particles are in
-// lattice order and blocks are renumbered per XCD so partners of a tile were
-// usually fetched by a neighbouring tile on the same XCD.
+// (coordinates, Diff, radius) is gathered through L2; particles are stored in
+// spatial (Morton) order and blocks are renumbered per XCD, so partners of a
+// tile were usually fetched by a neighbouring tile on the same XCD.  A link's
+// broken flag rides in the sign bit of its slot word (no per-link gather), and
+// contact pairs beyond the attraction range are skipped when that provably
+// leaves the force unchanged (contact_is_inert).
 #include "hdm_context.h"
 #include "kernel_common.h"
 
@@ -66,6 +69,33 @@ __device__ __forceinline__ void contact_law(R mask, R xDist, R yDist, R zDist, c
     F[3] += mask * ((-fp.dampingRotation) * ((self[3] - other[3]) * fp.invDt));
     F[4] += mask * ((-fp.dampingRotation) * ((self[4] - other[4]) * fp.invDt));
     F[5] += mask * ((-fp.dampingRotation) * ((self[5] - other[5]) * fp.invDt));
+}
+
+template <class R> __device__ __forceinline__ R huge_value();
+template <> __device__ __forceinline__ double huge_value<double>() { return 1e300; }
+template <> __device__ __forceinline__ float huge_value<float>() { return 1e37f; }
+
+// True when contact_law would add exactly +-0 to every force component, so the
+// pair can be skipped without changing a bit of the result: beyond
+// radSum + attractDistance the law multiplies its terms by mask = 0, and while
+// those terms are finite the product is +-0.  Adding +-0 leaves F unchanged,
+// because F starts at +0 and round-to-nearest sums never yield -0.  Pairs
+// with non-finite inputs take the full path, so NaN/inf propagate exactly as
+// in the sequential order of CombinedAccel::iterateCombined.  (The damping
+// terms are recomputed here in the law's own expression order.)
+template <class R>
+__device__ __forceinline__ bool contact_is_inert(R xDist, R yDist, R zDist, const R self[7], const R other[7],
+                                                 const ForceParams<R> &fp) {
+    R particleDistance2 = xDist * xDist + yDist * yDist + zDist * zDist;
+    R radSum = self[6] + other[6];
+    R activeDist2 = (radSum + fp.attractDistance) * (radSum + fp.attractDistance);
+    if (!(particleDistance2 >= activeDist2) || !(particleDistance2 <= huge_value<R>())) return false;
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 3; c++) ok &= fabs((-fp.damping) * ((self[c] - other[c]) * fp.invDt)) <= huge_value<R>();
+#pragma unroll
+    for (int c = 3; c < 6; c++) ok &= fabs((-fp.dampingRotation) * ((self[c] - other[c]) * fp.invDt)) <= huge_value<R>();
+    return ok;
 }
 
 // HiDEMDefLinkForceFunc::operator() (HiDEMDefaultFunctions.h:105-203).
@@ -150,7 +180,6 @@ struct ForceArgs {
     const uint32_t *linkCount;
     const int32_t *slotLink, *slotOther;
     int32_t linkWidth;
-    const uint8_t *state;
     CT *c0, *c1, *c2, *c3, *c4, *c5;
     FO *f0, *f1, *f2, *f3, *f4, *f5;
     ForceParams<R> fp;
@@ -173,6 +202,7 @@ __global__ void __launch_bounds__(TPB) k_force(ForceArgs<R, CT, FO> a) {
             R xDist = (R)(xCord - a.x[q]);
             R yDist = (R)(yCord - a.y[q]);
             R zDist = (R)(zCord - a.z[q]);
+            if (contact_is_inert<R>(xDist, yDist, zDist, self, other, a.fp)) continue;
             contact_law<R>(R(1), xDist, yDist, zDist, self, other, a.fp, F);
         }
     }
@@ -181,14 +211,13 @@ __global__ void __launch_bounds__(TPB) k_force(ForceArgs<R, CT, FO> a) {
         const int64_t base = (p >> 6) * (int64_t)64 * a.linkWidth + lane;
         for (int32_t k = 0; k < cnt; k++) {
             const int64_t sidx = base + (int64_t)k * 64;
-            const int32_t l = a.slotLink[sidx];
             const int32_t q = a.slotOther[sidx];
             R other[7] = {(R)a.d0[q], (R)a.d1[q], (R)a.d2[q], (R)a.d3[q], (R)a.d4[q], (R)a.d5[q], (R)a.radius[q]};
             R xDist = (R)(xCord - a.x[q]);
             R yDist = (R)(yCord - a.y[q]);
             R zDist = (R)(zCord - a.z[q]);
             R mask = 1;
-            if (a.state[l] == 0) mask = 0;
+            if (a.slotLink[sidx] < 0) mask = 0;  // broken link (state == 0)
             R ct[6] = {(R)a.c0[sidx], (R)a.c1[sidx], (R)a.c2[sidx], (R)a.c3[sidx], (R)a.c4[sidx], (R)a.c5[sidx]};
             beam_law<R>(mask, xDist, yDist, zDist, self, other, a.fp, ct, F);
             a.c0[sidx] = (CT)ct[0];
@@ -371,6 +400,8 @@ struct BreakArgs {
     const double *x, *y, *z, *tx, *ty, *tz;
     const int32_t *A, *B;
     uint8_t *state;
+    int32_t *slotLink;            // broken flag mirrored into both of the link's slots
+    const int32_t *sidePos;
     const double *initialLen, *maxLoad;
     int64_t L;
     double bendBreak, scalarMaxLoad;
@@ -399,6 +430,8 @@ __global__ void __launch_bounds__(TPB) k_break(BreakArgs a) {
         double maxLoad = a.maxLoad ? a.maxLoad[l] : a.scalarMaxLoad;
         if (load >= maxLoad) {
             a.state[l] = 0;
+            a.slotLink[a.sidePos[2 * l]] |= SLOT_BROKEN;
+            a.slotLink[a.sidePos[2 * l + 1]] |= SLOT_BROKEN;
             broke = 1;
         }
     }
@@ -459,7 +492,6 @@ hipError_t force_impl(Context &c, const hdm_force_params &fp) {
     a.nbrCount = c.nbrCount.as<int32_t>(); a.nbr = c.nbr.as<int32_t>(); a.nbrWidth = c.maxNeigh;
     a.linkCount = c.linkCount.as<uint32_t>(); a.slotLink = c.slotLink.as<int32_t>();
     a.slotOther = c.slotOther.as<int32_t>(); a.linkWidth = c.maxLinks;
-    a.state = c.state.as<uint8_t>();
     a.c0 = c.ct[0].as<CT>(); a.c1 = c.ct[1].as<CT>(); a.c2 = c.ct[2].as<CT>();
     a.c3 = c.ct[3].as<CT>(); a.c4 = c.ct[4].as<CT>(); a.c5 = c.ct[5].as<CT>();
     a.f0 = c.force[0].as<FO>(); a.f1 = c.force[1].as<FO>(); a.f2 = c.force[2].as<FO>();
@@ -519,6 +551,8 @@ hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counte
     a.state = c.state.as<uint8_t>();
     a.initialLen = c.initialLen.as<double>();
     a.maxLoad = c.hasMaxLoad ? c.maxLoad.as<double>() : nullptr;
+    a.slotLink = c.slotLink.as<int32_t>();
+    a.sidePos = c.sidePos.as<int32_t>();
     a.L = c.L;
     a.bendBreak = bp.bendBreak;
     a.scalarMaxLoad = bp.maxLoad;

@@ -144,8 +144,9 @@ def build_inputs(cfg: dict, cells=None):
     hdm = cfg.get("hdm", {})
     nx, ny, nz = cells if cells is not None else hdm["cells"]
     origin = hdm.get("origin", [0.0, 0.0, 0.0])
+    order = {"lattice": A.HDM_ORDER_LATTICE, "morton": A.HDM_ORDER_MORTON}[hdm.get("order", "lattice")]
     p = A.gen_fcc(nx, ny, nz, s.a, s.radius, hdm.get("jitter", 0.0), int(cfg["lattice"].get("seed", 1)),
-                  origin=tuple(origin), clamp_x_below=hdm.get("clampXBelow", -np.inf))
+                  origin=tuple(origin), clamp_x_below=hdm.get("clampXBelow", -np.inf), order=order)
     l = A.gen_links(p, s.link_gen_scale, cfg["general"].get("maxLoad", 0.35), hdm.get("maxLoadSpread", 0.0),
                     int(hdm.get("linkSeed", 7)))
     return s, p, l

@@ -232,7 +232,10 @@ typedef struct hdm_fcc_spec {
     double jitter;              /* uniform +-jitter per coordinate */
     uint64_t seed;
     double clampXBelow;         /* particles with x < clampXBelow get HDM_TYPE_FIXED (use -inf to disable) */
+    int32_t order;              /* HDM_ORDER_LATTICE or HDM_ORDER_MORTON (unit cells along a Z-order curve) */
 } hdm_fcc_spec;
+
+enum { HDM_ORDER_LATTICE = 0, HDM_ORDER_MORTON = 1 };
 
 int  hdm_gen_fcc_count(const hdm_fcc_spec *spec, int64_t *nParticles);
 int  hdm_gen_fcc_fill(const hdm_fcc_spec *spec, hdm_particles_host *out);

@@ -87,3 +87,17 @@ def test_glacier_config_size():
     assert nx * ny * nz * 4 == 8_000_000
     assert s.a == pytest.approx(1.0, rel=1e-6)
     assert 2 * s.radius == pytest.approx(1 / math.sqrt(2) - 0.04, rel=1e-4)
+
+
+def test_morton_order_is_a_permutation_of_lattice_order():
+    a = 1.4707821048680188
+    lat = A.gen_fcc(6, 5, 4, a, 0.5, 0.01, 3, order=A.HDM_ORDER_LATTICE)
+    mor = A.gen_fcc(6, 5, 4, a, 0.5, 0.01, 3, order=A.HDM_ORDER_MORTON)
+    P = lambda p: np.stack([p["xCord"], p["yCord"], p["zCord"]], 1)  # noqa: E731
+    pl, pm = P(lat), P(mor)
+    assert not np.array_equal(pl, pm)
+    key = lambda x: np.lexsort((x[:, 2], x[:, 1], x[:, 0]))  # noqa: E731
+    assert np.array_equal(pl[key(pl)], pm[key(pm)])  # same particles, bit for bit
+    # Morton order is spatially local: consecutive particles stay close
+    step = np.linalg.norm(np.diff(pm, axis=0), axis=1)
+    assert np.median(step) < 1.5 * a
