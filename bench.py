@@ -8,9 +8,10 @@ all 8,000,000 particles / ~48M links resident in HBM.  Prints one JSON line.
 value      whole-job particle-steps/s of the headline form (--precision):
            N_particles * K / wall time of the K timed steps, bracketed by
            device synchronisation.
-roofline   for the dominant kernel (force gather): algorithmic bytes per
+roofline   for the dominant kernel (k_step: link break of the previous step,
+           force gather and integration in one launch): algorithmic bytes per
            launch (compulsory traffic of the arrays it must touch, see
-           force_bytes) / its mean launch time from HIP events recorded on the
+           step_bytes) / its mean launch time from HIP events recorded on the
            context's stream during the timed steps, against 8 TB/s.
 cpu_baseline  the CPU oracle (a C/OpenMP restatement of the reference's CPU
            path, built with the reference's release flags) timed on this host
@@ -38,26 +39,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def force_bytes(n, n_neigh_total, n_sides, prec):
-    """Compulsory HBM bytes of one force-gather launch.
+def step_bytes(n, n_neigh_total, n_links, prec, exact=True):
+    """Compulsory HBM bytes of one fused step launch (k_step: link break of
+    the previous step, contact + beam force gather, integration).
 
-    Per particle: own coordinates (3 x 8), Diff (6 x 8), radius (8),
-    neighbour and link counts (2 x 4) read; force (6 x 8 FP64 / 6 x 4 mixed)
-    written.  Per neighbour-list entry: its 4 B index.  Per link side: CT0..5
-    read and written (2 x 6 x 8 FP64 / 2 x 6 x 4 mixed), link id and other
-    endpoint (2 x 4), link state (1).  Partner coordinates/Diff are gathered
-    from particles already counted once (cache reuse), as in the roofline model
-    of the hot path."""
+    Per particle: coordinates, rotations, Diff (12 x 8), radius (8), type (1),
+    neighbour and link counts (2 x 4), and for the exact rebuild test the
+    coordinates at the last rebuild (3 x 8) read; coordinates, rotations and
+    Diff (12 x 8) and the force (6 x 8 FP64 / 6 x 4 mixed) written.  Per
+    neighbour-list entry: its 4 B index.  Per link side: CT0..5 read and
+    written (2 x 6 x 8 FP64 / 2 x 6 x 4 mixed), link word and other endpoint
+    (2 x 4).  Per link: initialLen and maxLoad (2 x 8) for the break test.
+    Partner state is gathered from particles already counted once (cache
+    reuse); the rare writes of a breaking link are not counted."""
     fo = 8 if prec == A.HDM_PREC_FP64 else 4
-    per_particle = 24 + 48 + 8 + 8 + 6 * fo
-    per_side = 12 * fo + 8 + 1
-    return n * per_particle + 4 * n_neigh_total + n_sides * per_side
+    per_particle = 96 + 8 + 1 + 8 + (24 if exact else 0) + 96 + 6 * fo
+    per_side = 12 * fo + 8
+    return n * per_particle + 4 * n_neigh_total + 2 * n_links * per_side + 16 * n_links
 
 
-def integrate_bytes(n, prec):
-    fo = 8 if prec == A.HDM_PREC_FP64 else 4
-    # read cords, theta, Diff, radius, type, force; write cords, theta, Diff
-    return n * (24 + 24 + 48 + 8 + 1 + 6 * fo + 24 + 24 + 48)
+def break_bytes(n_links):
+    """Compulsory bytes of the standalone k_break: A, B, state, initialLen,
+    maxLoad per link (endpoint coordinates come through the cache)."""
+    return n_links * (4 + 4 + 1 + 8 + 8)
 
 
 def run_gpu(setup, p, l, prec, steps, warmup):
@@ -98,7 +102,7 @@ def cpu_baseline(setup, p, l, steps, fast=True):
 
 def probe_traffic(setup, p, l, forms):
     """Kernels of known size for calibrating the FETCH_SIZE / WRITE_SIZE
-    counters, then the force gather on the real workload."""
+    counters, then fused step launches on the real workload."""
     for prec in forms:
         with A.Context(prec) as ctx:
             ctx.upload(p, l)
@@ -107,13 +111,12 @@ def probe_traffic(setup, p, l, forms):
             if prec == forms[0]:
                 g = ctx.bw_probe(A.HDM_PROBE_COPY, 1 << 30, 1)  # k_probe_copy: 512 MiB read + 512 MiB written
                 log(f"copy probe 512 MiB read + 512 MiB written: {g:.0f} GB/s")
-            for _ in range(3):
-                ctx.force(setup.force)
+            ctx.run(setup.run, 3)
             ctx.sync()
             counts, _, _ = ctx.neighbours_download()
-            b = force_bytes(p.n, int(counts.sum()), 2 * l.n, prec)
+            b = step_bytes(p.n, int(counts.sum()), l.n, prec)
             print(json.dumps({"precision": "fp64" if prec == A.HDM_PREC_FP64 else "mixed",
-                              "force_algorithmic_bytes": b, "neighbours": int(counts.sum()), "sides": 2 * l.n}),
+                              "step_algorithmic_bytes": b, "neighbours": int(counts.sum()), "links": l.n}),
                   flush=True)
 
 
@@ -152,22 +155,19 @@ def main():
         log(f"running {name}: warmup {args.warmup}, timed {args.steps}")
         r = run_gpu(setup, p, l, prec, args.steps, args.warmup)
         r["value"] = p.n * args.steps / r["wall"]
-        fms, fcnt = r["kernels"]["force"]
-        r["force_avg_ms"] = fms / max(fcnt, 1)
-        r["force_bytes"] = force_bytes(p.n, r["n_neigh"], 2 * l.n, prec)
-        ims, icnt = r["kernels"]["integrate"]
-        r["integrate_avg_ms"] = ims / max(icnt, 1)
-        r["integrate_bytes"] = integrate_bytes(p.n, prec)
+        sms, scnt = r["kernels"]["step"]
+        r["step_avg_ms"] = sms / max(scnt, 1)
+        r["step_bytes"] = step_bytes(p.n, r["n_neigh"], l.n, prec)
         results[name] = r
         log(f"  {name}: {r['value']:.4e} p-steps/s, {1e3 * r['wall'] / args.steps:.3f} ms/step, "
-            f"force {r['force_avg_ms']:.3f} ms ({r['force_bytes'] / r['force_avg_ms'] / 1e6:.0f} GB/s), "
+            f"step kernel {r['step_avg_ms']:.3f} ms ({r['step_bytes'] / r['step_avg_ms'] / 1e6:.0f} GB/s), "
             f"rebuilds {r['stats'].rebuilds}, broken {r['broken']}")
 
     head = "mixed" if "mixed" in results else "fp64"
     h = results[head]
-    achieved = h["force_bytes"] / (h["force_avg_ms"] * 1e-3) / 1e9
+    achieved = h["step_bytes"] / (h["step_avg_ms"] * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"force_{head}_{args.config}_traffic.json")
+    tf = os.path.join(ROOT, "profiles", f"step_{head}_{args.config}_traffic.json")
     if os.path.exists(tf) and args.cells is None:
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -204,14 +204,13 @@ def main():
             "rebuilds_in_timed_steps": int(h["stats"].rebuilds),
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "force gather",
-                     "bytes_per_launch": h["force_bytes"], "avg_launch_ms": h["force_avg_ms"]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_step (link break + force gather + integration)",
+                     "bytes_per_launch": h["step_bytes"], "avg_launch_ms": h["step_avg_ms"]},
         "cpu_baseline": cpu,
         "forms": {name: {"value": r["value"], "ms_per_step": 1e3 * r["wall"] / args.steps,
-                         "force_avg_ms": r["force_avg_ms"],
-                         "force_GBps": r["force_bytes"] / (r["force_avg_ms"] * 1e-3) / 1e9,
-                         "integrate_avg_ms": r["integrate_avg_ms"],
-                         "integrate_GBps": r["integrate_bytes"] / (r["integrate_avg_ms"] * 1e-3) / 1e9,
+                         "step_avg_ms": r["step_avg_ms"],
+                         "step_GBps": r["step_bytes"] / (r["step_avg_ms"] * 1e-3) / 1e9,
                          "rebuilds": int(r["stats"].rebuilds), "broken_links": r["broken"],
                          "device_bytes": r["dev_bytes"], "kernels": kern[name]}
                   for name, r in results.items()},

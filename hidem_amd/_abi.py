@@ -30,7 +30,7 @@ HDM_ORDER_LATTICE = 0
 HDM_ORDER_MORTON = 1
 HDM_PROBE_COPY = 0
 HDM_PROBE_CT_RMW = 1
-TIMER_NAMES = ("force", "integrate", "break", "skin", "grid", "neigh", "links")
+TIMER_NAMES = ("force", "integrate", "break", "skin", "grid", "neigh", "links", "step")
 
 _dp = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)
@@ -437,10 +437,11 @@ class Context:
         self._c(lib().hdm_reset_kernel_times(self._h), "hdm_reset_kernel_times")
 
     def kernel_times(self):
-        ms = (C.c_double * 7)()
-        cnt = (C.c_int64 * 7)()
+        k = len(TIMER_NAMES)  # HDM_T_COUNT
+        ms = (C.c_double * k)()
+        cnt = (C.c_int64 * k)()
         self._c(lib().hdm_kernel_times(self._h, ms, cnt), "hdm_kernel_times")
-        return {TIMER_NAMES[k]: (ms[k], cnt[k]) for k in range(7)}
+        return {TIMER_NAMES[i]: (ms[i], cnt[i]) for i in range(k)}
 
     def device_bytes(self):
         b = C.c_int64()

@@ -221,9 +221,11 @@ int hdm_device_bytes(hdm_ctx *ctx, int64_t *bytes) {
     const hdm::DevBuf *bufs[] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz, &c.radius, &c.type, &c.la, &c.lb,
                                  &c.state, &c.initialLen, &c.maxLoad, &c.linkCount, &c.linkCursor, &c.slotLink,
                                  &c.slotOther, &c.sidePos, &c.nbrCount, &c.nbr, &c.colour, &c.order,
-                                 &c.cellStart, &c.cellCursor, &c.scanPartials, &c.red, &c.x0, &c.y0, &c.z0};
+                                 &c.cellStart, &c.cellCursor, &c.scanPartials, &c.red, &c.x0, &c.y0, &c.z0,
+                                 &c.skinPartials, &c.blockId, &c.blockWater, &c.probe};
     for (auto *b : bufs) t += b->bytes;
     for (int k = 0; k < 6; k++) t += c.diff[k].bytes + c.force[k].bytes + c.ct[k].bytes + c.ctStage[k].bytes;
+    for (int k = 0; k < 12; k++) t += c.alt[k].bytes;
     *bytes = (int64_t)t;
     return HDM_OK;
 }
@@ -695,6 +697,10 @@ int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *reb
     const bool wantDisp = rp->policy == HDM_REBUILD_EXACT;
     auto t0 = std::chrono::steady_clock::now();
     int64_t rebuilds = 0;
+    // The link break of step s runs inside step s+1's fused launch (on the
+    // same state, before any force uses the links); the last step's break is
+    // a standalone launch after the loop.
+    bool breakPending = false;
     for (int64_t step = 0; step < nSteps; step++) {
         bool regen = !c.neighBuilt || !c.snapshotValid;
         if (rp->policy == HDM_REBUILD_INTERVAL) {
@@ -735,17 +741,19 @@ int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *reb
             if (r != HDM_OK) return r;
             rebuilds++;
         }
-        int r = hdm_force(ctx, &rp->force);
-        if (r != HDM_OK) return r;
         {
-            Timer t(c, HDM_T_INTEGRATE);
-            HIP_CHECK(ctx, hdm::launch_integrate(c, rp->integ, wantSkin ? skin : nullptr, wantDisp));
+            Timer t(c, HDM_T_STEP);
+            HIP_CHECK(ctx, hdm::launch_step(c, rp->force, rp->integ, wantSkin ? skin : nullptr, wantDisp,
+                                            breakPending ? &rp->brk : nullptr));
         }
         c.skinFused = wantSkin;
         c.skinFusedDisp = wantDisp;
-        if ((double)c.ts * rp->dt > rp->breakTime) r = hdm_break(ctx, &rp->brk, nullptr);
-        if (r != HDM_OK) return r;
+        breakPending = (double)c.ts * rp->dt > rp->breakTime;
         c.ts++;
+    }
+    if (breakPending) {
+        int r = hdm_break(ctx, &rp->brk, nullptr);
+        if (r != HDM_OK) return r;
     }
     HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
     auto t1 = std::chrono::steady_clock::now();

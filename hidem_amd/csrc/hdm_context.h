@@ -5,6 +5,8 @@
 // Device layout (every array is one hipMalloc, structure of arrays):
 //   particles  x/y/zCord, x/y/zTheta, x..wDiff, radius: f64;  x..wForce: f64
 //              (FP64 form) or f32 (mixed form); type: u8.  Index = pInd.
+//              Coordinates, rotations and Diff have a second buffer each that
+//              the fused step kernel integrates into before the two swap.
 //   links      localPartIndA/B: i32, state: u8, initialLen, maxLoad: f64.
 //   slots      "wave tiles": particle p owns slot k of tile p/64 at
 //              tile*64*width + k*64 + p%64, so the 64 lanes of a wavefront
@@ -48,6 +50,15 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
+inline void swap_buf(DevBuf &a, DevBuf &b) {
+    void *p = a.ptr;
+    size_t s = a.bytes;
+    a.ptr = b.ptr;
+    a.bytes = b.bytes;
+    b.ptr = p;
+    b.bytes = s;
+}
+
 struct TimedLaunch {
     int cls;
     hipEvent_t start, stop;
@@ -72,6 +83,9 @@ struct Context {
     // particles
     int64_t n = 0;
     DevBuf x, y, z, tx, ty, tz, diff[6], radius, force[6], type;
+    // second state buffer of the fused step (x, y, z, x..zTheta, Diff0..5):
+    // each hdm_run step integrates into it and then swaps it with the above
+    DevBuf alt[12];
     // batched form: block id per particle, waterline per block (nBlocks = 0: unused)
     int32_t nBlocks = 0;
     DevBuf blockId, blockWater;
@@ -94,6 +108,7 @@ struct Context {
     DevBuf colour, order, cellStart, cellCursor, scanPartials;
     // reductions / scalars
     DevBuf red;            // u64 scratch for bbox / max reductions and counters
+    DevBuf skinPartials;   // f64 [workgroups][4]: rebuild-test maxima per integrating workgroup
     DevBuf x0, y0, z0;     // positions at the last neighbour-list build
     DevBuf probe;          // bandwidth-probe buffer (hdm_bw_probe)
     bool snapshotValid = false;
@@ -138,6 +153,12 @@ hipError_t launch_ct_export(Context &c);   // ct (tiled) -> ctStage (checkpoint 
 hipError_t launch_force(Context &c, const hdm_force_params &fp);
 hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *skin = nullptr,
                             bool withDisplacement = false);
+// Force + integration of one timestep in one launch (state ping-pong; the
+// integrated state is current afterwards).  skin as for launch_integrate.
+// brk != nullptr: first apply the link break that the previous timestep's
+// launch_break would have applied to the current state.
+hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_params &ip, uint64_t *skin,
+                       bool withDisplacement, const hdm_break_params *brk);
 hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter);
 hipError_t launch_skin_heuristic(Context &c, uint64_t *out3);
 hipError_t launch_skin_exact(Context &c, uint64_t *out1);

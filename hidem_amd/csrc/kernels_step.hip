@@ -20,6 +20,8 @@
 // broken flag rides in the sign bit of its slot word (no per-link gather), and
 // contact pairs beyond the attraction range are skipped when that provably
 // leaves the force unchanged (contact_is_inert).
+#include <type_traits>
+
 #include "hdm_context.h"
 #include "kernel_common.h"
 
@@ -183,15 +185,51 @@ struct ForceArgs {
     CT *c0, *c1, *c2, *c3, *c4, *c5;
     FO *f0, *f1, *f2, *f3, *f4, *f5;
     ForceParams<R> fp;
+    // Fused link break of the previous timestep (fused step only; breakCheck
+    // = 0 elsewhere): before its beam law, every intact side evaluates
+    // HiDEMDefLinkBreakFunc on the current state, exactly as k_break does
+    // after the previous integration.
+    int32_t breakCheck;
+    int32_t *slotLinkW;
+    const double *tx, *ty, *tz, *initialLen, *maxLoad;
+    uint8_t *state;
+    double bendBreak, scalarMaxLoad;
 };
 
+// Force on particle p (CombinedAccel::iterateCombined body): contacts in list
+// order, then link sides in ascending link id, CT updated in place.
+// HiDEMDefLinkBreakFunc (HiDEMDefaultFunctions.h:217-241) for link l seen
+// from one end: d = this end's coordinates minus the other's, t = this end's
+// rotations, q = the other end.  Both ends compute the same bits: IEEE
+// subtraction is antisymmetric (a - b == -(b - a)), so the squared distances
+// and the absolute rotation differences do not depend on which end asks.
 template <class R, class CT, class FO>
-__global__ void __launch_bounds__(TPB) k_force(ForceArgs<R, CT, FO> a) {
-    const int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
-    if (p >= a.n) return;
+__device__ __forceinline__ bool link_breaks(const ForceArgs<R, CT, FO> &a, int32_t l, int32_t q, double xDist,
+                                            double yDist, double zDist, const double t[3]) {
+    double stretchLength = sqrt((xDist * xDist) + (yDist * yDist) + (zDist * zDist));
+    double bx = fabs(t[0] - a.tx[q]);
+    double by = fabs(t[1] - a.ty[q]);
+    double bz = fabs(t[2] - a.tz[q]);
+    double inner = (by < bz) ? bz : by;
+    double maxBend = (bx < inner) ? inner : bx;
+    double initialLen = a.initialLen[l];
+    double load = (stretchLength - initialLen) / initialLen + a.bendBreak * maxBend;
+    double maxLoad = a.maxLoad ? a.maxLoad[l] : a.scalarMaxLoad;
+    return load >= maxLoad;
+}
+
+template <class R, class CT, class FO>
+__device__ __forceinline__ void force_particle(const ForceArgs<R, CT, FO> &a, int64_t p, R F[6]) {
     const double xCord = a.x[p], yCord = a.y[p], zCord = a.z[p];
+    double theta[3] = {0, 0, 0};
+    if (a.breakCheck) {
+        theta[0] = a.tx[p];
+        theta[1] = a.ty[p];
+        theta[2] = a.tz[p];
+    }
     R self[7] = {(R)a.d0[p], (R)a.d1[p], (R)a.d2[p], (R)a.d3[p], (R)a.d4[p], (R)a.d5[p], (R)a.radius[p]};
-    R F[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 6; c++) F[c] = 0;
     const int64_t lane = p & 63;
     {
         const int32_t cnt = a.nbrCount[p];
@@ -213,11 +251,20 @@ __global__ void __launch_bounds__(TPB) k_force(ForceArgs<R, CT, FO> a) {
             const int64_t sidx = base + (int64_t)k * 64;
             const int32_t q = a.slotOther[sidx];
             R other[7] = {(R)a.d0[q], (R)a.d1[q], (R)a.d2[q], (R)a.d3[q], (R)a.d4[q], (R)a.d5[q], (R)a.radius[q]};
-            R xDist = (R)(xCord - a.x[q]);
-            R yDist = (R)(yCord - a.y[q]);
-            R zDist = (R)(zCord - a.z[q]);
+            const double dx = xCord - a.x[q], dy = yCord - a.y[q], dz = zCord - a.z[q];
+            R xDist = (R)dx;
+            R yDist = (R)dy;
+            R zDist = (R)dz;
+            const int32_t word = a.slotLink[sidx];
             R mask = 1;
-            if (a.slotLink[sidx] < 0) mask = 0;  // broken link (state == 0)
+            if (word < 0) {
+                mask = 0;  // broken link (state == 0)
+            } else if (a.breakCheck && link_breaks(a, word, q, dx, dy, dz, theta)) {
+                // both ends reach the same verdict; each flags its own slot
+                mask = 0;
+                a.slotLinkW[sidx] = word | SLOT_BROKEN;
+                a.state[word] = 0;
+            }
             R ct[6] = {(R)a.c0[sidx], (R)a.c1[sidx], (R)a.c2[sidx], (R)a.c3[sidx], (R)a.c4[sidx], (R)a.c5[sidx]};
             beam_law<R>(mask, xDist, yDist, zDist, self, other, a.fp, ct, F);
             a.c0[sidx] = (CT)ct[0];
@@ -228,6 +275,10 @@ __global__ void __launch_bounds__(TPB) k_force(ForceArgs<R, CT, FO> a) {
             a.c5[sidx] = (CT)ct[5];
         }
     }
+}
+
+template <class R, class CT, class FO>
+__device__ __forceinline__ void store_force(const ForceArgs<R, CT, FO> &a, int64_t p, const R F[6]) {
     a.f0[p] = (FO)F[0];
     a.f1[p] = (FO)F[1];
     a.f2[p] = (FO)F[2];
@@ -236,18 +287,31 @@ __global__ void __launch_bounds__(TPB) k_force(ForceArgs<R, CT, FO> a) {
     a.f5[p] = (FO)F[5];
 }
 
+template <class R, class CT, class FO>
+__global__ void __launch_bounds__(TPB) k_force(ForceArgs<R, CT, FO> a) {
+    const int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    if (p >= a.n) return;
+    R F[6];
+    force_particle(a, p, F);
+    store_force(a, p, F);
+}
+
 // ---- time integration --------------------------------------------------------
-template <class FO>
+// Particle state read by the integrator (in) and written by it (out).  The
+// standalone integrator updates in place (out == in); the fused step kernel
+// writes the other buffer of a ping-pong pair, because other lanes of the same
+// launch still gather the old state for their forces.
 struct IntegArgs {
-    double *x, *y, *z, *tx, *ty, *tz, *d0, *d1, *d2, *d3, *d4, *d5;
+    const double *x, *y, *z, *tx, *ty, *tz, *d0, *d1, *d2, *d3, *d4, *d5;
+    double *ox, *oy, *oz, *otx, *oty, *otz, *od0, *od1, *od2, *od3, *od4, *od5;
     const double *radius;
-    const FO *f0, *f1, *f2, *f3, *f4, *f5;
     const uint8_t *type;
     int64_t n;
     hdm_integ_params ip;
-    // rebuild-test reductions of the new state (nullptr: none).  skin[0..2]:
-    // max(0, max Diff) per axis (HiDEMLib.h:424-461); skin[4]: max |x - x0|^2.
-    uint64_t *skin;
+    // rebuild-test reductions of the new state (nullptr: none): one row of 4
+    // doubles per workgroup, max(0, max Diff) per axis (HiDEMLib.h:424-461)
+    // and max |x - x0|^2, folded into the skin slots by k_skin_finalize.
+    double *partials;
     const double *x0, *y0, *z0;   // nullptr: no displacement reduction
     const int32_t *blockId;       // batched form: waterline of the particle's block
     const double *blockWater;     // (nullptr: ip.waterLevel for every particle)
@@ -285,22 +349,26 @@ __device__ __forceinline__ void central_difference(double dragCoeff, double mass
 // applyForce: IceBlock (MainIceBlock.cpp:299-400, clamp = HDM_TYPE_FIXED) or
 // GlacierDem (MainDem.cpp:145-259 with an analytic bed, constant external
 // force, HiDEMDefVelocityLimit and HiDEMDefBoundaryCondition).
-// Integrates particle i; returns its new coordinates and translational Diff.
-template <class FO>
-__device__ __forceinline__ void integrate_particle(const IntegArgs<FO> &a, int64_t i, double newCord[3],
-                                                   double newDiff[3]) {
+// Integrates particle i under force Fin; returns its new coordinates and
+// translational Diff.
+__device__ __forceinline__ void integrate_particle(const IntegArgs &a, int64_t i, const double Fin[6],
+                                                   double newCord[3], double newDiff[3]) {
     const hdm_integ_params &ip = a.ip;
-    if (ip.variant == HDM_INTEG_ICEBLOCK && (a.type[i] & HDM_TYPE_FIXED)) {
-        a.d0[i] = 0; a.d1[i] = 0; a.d2[i] = 0;
-        a.d3[i] = 0; a.d4[i] = 0; a.d5[i] = 0;
-        newCord[0] = a.x[i]; newCord[1] = a.y[i]; newCord[2] = a.z[i];
-        newDiff[0] = 0; newDiff[1] = 0; newDiff[2] = 0;
-        return;
-    }
     double cord[3] = {a.x[i], a.y[i], a.z[i]};
     double theta[3] = {a.tx[i], a.ty[i], a.tz[i]};
+    if (ip.variant == HDM_INTEG_ICEBLOCK && (a.type[i] & HDM_TYPE_FIXED)) {
+        a.ox[i] = cord[0]; a.oy[i] = cord[1]; a.oz[i] = cord[2];
+        a.otx[i] = theta[0]; a.oty[i] = theta[1]; a.otz[i] = theta[2];
+        a.od0[i] = 0; a.od1[i] = 0; a.od2[i] = 0;
+        a.od3[i] = 0; a.od4[i] = 0; a.od5[i] = 0;
+        for (int c = 0; c < 3; c++) {
+            newCord[c] = cord[c];
+            newDiff[c] = 0;
+        }
+        return;
+    }
     double diff[6] = {a.d0[i], a.d1[i], a.d2[i], a.d3[i], a.d4[i], a.d5[i]};
-    double F[6] = {(double)a.f0[i], (double)a.f1[i], (double)a.f2[i], (double)a.f3[i], (double)a.f4[i], (double)a.f5[i]};
+    double F[6] = {Fin[0], Fin[1], Fin[2], Fin[3], Fin[4], Fin[5]};
     double cordPrev[3], thetaPrev[3], vel[6];
     const double radius = a.radius[i];
     const double diameter = radius * 2.0;
@@ -351,47 +419,101 @@ __device__ __forceinline__ void integrate_particle(const IntegArgs<FO> &a, int64
             if (cord[c] - radius > ip.upper[c] && ip.bound[2 * c + 1]) cord[c] = cordPrev[c];
         }
     }
-    a.x[i] = cord[0];
-    a.y[i] = cord[1];
-    a.z[i] = cord[2];
-    a.tx[i] = theta[0];
-    a.ty[i] = theta[1];
-    a.tz[i] = theta[2];
-    a.d0[i] = cord[0] - cordPrev[0];
-    a.d1[i] = cord[1] - cordPrev[1];
-    a.d2[i] = cord[2] - cordPrev[2];
-    a.d3[i] = theta[0] - thetaPrev[0];
-    a.d4[i] = theta[1] - thetaPrev[1];
-    a.d5[i] = theta[2] - thetaPrev[2];
+    a.ox[i] = cord[0];
+    a.oy[i] = cord[1];
+    a.oz[i] = cord[2];
+    a.otx[i] = theta[0];
+    a.oty[i] = theta[1];
+    a.otz[i] = theta[2];
+    a.od0[i] = cord[0] - cordPrev[0];
+    a.od1[i] = cord[1] - cordPrev[1];
+    a.od2[i] = cord[2] - cordPrev[2];
+    a.od3[i] = theta[0] - thetaPrev[0];
+    a.od4[i] = theta[1] - thetaPrev[1];
+    a.od5[i] = theta[2] - thetaPrev[2];
     for (int c = 0; c < 3; c++) {
         newCord[c] = cord[c];
         newDiff[c] = cord[c] - cordPrev[c];
     }
 }
 
-template <class FO>
-__global__ void __launch_bounds__(TPB) k_integrate(IntegArgs<FO> a) {
-    const int64_t i = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
-    const bool active = i < a.n;
-    double cord[3] = {0, 0, 0}, diff[3] = {0, 0, 0};
-    if (active) integrate_particle(a, i, cord, diff);
-    if (a.skin == nullptr) return;
-    // reductions over the new state; inactive lanes contribute 0
-    double m[3] = {0.0, 0.0, 0.0}, d2 = 0.0;
+// Workgroup maxima of the rebuild-test quantities of the particles this
+// workgroup integrated, written as one row of a.partials (every quantity is
+// >= 0, so the maxima start from 0 and inactive lanes contribute 0).
+__device__ __forceinline__ void skin_partials(const IntegArgs &a, bool active, int64_t i, const double cord[3],
+                                              const double diff[3]) {
+    __shared__ double rows[TPB / 64][4];
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
     if (active) {
         for (int c = 0; c < 3; c++) m[c] = diff[c] > 0.0 ? diff[c] : 0.0;
         if (a.x0) {
             double ddx = cord[0] - a.x0[i], ddy = cord[1] - a.y0[i], ddz = cord[2] - a.z0[i];
-            d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            m[3] = ddx * ddx + ddy * ddy + ddz * ddz;
         }
     }
 #pragma unroll
-    for (int c = 0; c < 3; c++) m[c] = wave_max(m[c]);
-    d2 = wave_max(d2);
+    for (int c = 0; c < 4; c++) m[c] = wave_max(m[c]);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 4; c++) rows[w][c] = m[c];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double v = rows[0][threadIdx.x];
+        for (int k = 1; k < TPB / 64; k++) v = rows[k][threadIdx.x] > v ? rows[k][threadIdx.x] : v;
+        a.partials[(int64_t)blockIdx.x * 4 + threadIdx.x] = v;
+    }
+}
+
+template <class FO>
+__global__ void __launch_bounds__(TPB) k_integrate(IntegArgs a, const FO *__restrict__ f0, const FO *__restrict__ f1,
+                                                   const FO *__restrict__ f2, const FO *__restrict__ f3,
+                                                   const FO *__restrict__ f4, const FO *__restrict__ f5) {
+    const int64_t i = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    const bool active = i < a.n;
+    double cord[3] = {0, 0, 0}, diff[3] = {0, 0, 0};
+    if (active) {
+        const double F[6] = {(double)f0[i], (double)f1[i], (double)f2[i], (double)f3[i], (double)f4[i], (double)f5[i]};
+        integrate_particle(a, i, F, cord, diff);
+    }
+    if (a.partials) skin_partials(a, active, i, cord, diff);
+}
+
+// One timestep of hdm_run in one launch: the link break of the previous
+// timestep (when f.breakCheck), force on particle p from the current state
+// (k_force), then its integration into the other state buffer (k_integrate),
+// with the rebuild-test partials of the new state.  The force is also stored,
+// as the separate kernels would leave it.
+template <class R, class CT>
+__global__ void __launch_bounds__(TPB) k_step(ForceArgs<R, CT, R> f, IntegArgs g) {
+    const int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    const bool active = p < f.n;
+    double cord[3] = {0, 0, 0}, diff[3] = {0, 0, 0};
+    if (active) {
+        R F[6];
+        force_particle(f, p, F);
+        store_force(f, p, F);
+        const double Fd[6] = {(double)F[0], (double)F[1], (double)F[2], (double)F[3], (double)F[4], (double)F[5]};
+        integrate_particle(g, p, Fd, cord, diff);
+    }
+    if (g.partials) skin_partials(g, active, p, cord, diff);
+}
+
+// Folds the per-workgroup rows into skin[0..2] (max+ Diff) and skin[4]
+// (max |x - x0|^2) as ordered keys.
+__global__ void __launch_bounds__(TPB) k_skin_finalize(const double *__restrict__ partials, int64_t rows,
+                                                        uint64_t *skin) {
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t r = (int64_t)blockIdx.x * TPB + threadIdx.x; r < rows; r += (int64_t)gridDim.x * TPB)
+        for (int c = 0; c < 4; c++) {
+            double v = partials[r * 4 + c];
+            m[c] = v > m[c] ? v : m[c];
+        }
+#pragma unroll
+    for (int c = 0; c < 4; c++) m[c] = wave_max(m[c]);
     if ((threadIdx.x & 63) == 0) {
         for (int c = 0; c < 3; c++)
-            if (m[c] > 0.0) atomicMax((unsigned long long *)&a.skin[c], (unsigned long long)ordered_key(m[c]));
-        if (a.x0 && d2 > 0.0) atomicMax((unsigned long long *)&a.skin[4], (unsigned long long)ordered_key(d2));
+            if (m[c] > 0.0) atomicMax((unsigned long long *)&skin[c], (unsigned long long)ordered_key(m[c]));
+        if (m[3] > 0.0) atomicMax((unsigned long long *)&skin[4], (unsigned long long)ordered_key(m[3]));
     }
 }
 
@@ -481,8 +603,34 @@ inline int reduce_grid(const Context &c, int64_t n) {
     return g < 1 ? 1 : g;
 }
 
+// State pointers for the integrator: in place (alt == nullptr) or from the
+// current buffers into alt[0..11] (x, y, z, xTheta.., Diff0..5).
+IntegArgs integ_args(Context &c, const hdm_integ_params &ip, bool skin, bool withDisplacement, DevBuf *alt) {
+    IntegArgs a;
+    a.x = c.x.as<double>(); a.y = c.y.as<double>(); a.z = c.z.as<double>();
+    a.tx = c.tx.as<double>(); a.ty = c.ty.as<double>(); a.tz = c.tz.as<double>();
+    a.d0 = c.diff[0].as<double>(); a.d1 = c.diff[1].as<double>(); a.d2 = c.diff[2].as<double>();
+    a.d3 = c.diff[3].as<double>(); a.d4 = c.diff[4].as<double>(); a.d5 = c.diff[5].as<double>();
+    double *o[12];
+    const double *in[12] = {a.x, a.y, a.z, a.tx, a.ty, a.tz, a.d0, a.d1, a.d2, a.d3, a.d4, a.d5};
+    for (int k = 0; k < 12; k++) o[k] = alt ? alt[k].as<double>() : const_cast<double *>(in[k]);
+    a.ox = o[0]; a.oy = o[1]; a.oz = o[2]; a.otx = o[3]; a.oty = o[4]; a.otz = o[5];
+    a.od0 = o[6]; a.od1 = o[7]; a.od2 = o[8]; a.od3 = o[9]; a.od4 = o[10]; a.od5 = o[11];
+    a.radius = c.radius.as<double>();
+    a.type = c.type.as<uint8_t>();
+    a.n = c.n;
+    a.ip = ip;
+    a.partials = skin ? c.skinPartials.as<double>() : nullptr;
+    a.x0 = withDisplacement ? c.x0.as<double>() : nullptr;
+    a.y0 = withDisplacement ? c.y0.as<double>() : nullptr;
+    a.z0 = withDisplacement ? c.z0.as<double>() : nullptr;
+    a.blockId = c.nBlocks > 0 ? c.blockId.as<int32_t>() : nullptr;
+    a.blockWater = c.nBlocks > 0 ? c.blockWater.as<double>() : nullptr;
+    return a;
+}
+
 template <class R, class CT, class FO>
-hipError_t force_impl(Context &c, const hdm_force_params &fp) {
+ForceArgs<R, CT, FO> force_args(Context &c, const hdm_force_params &fp) {
     ForceArgs<R, CT, FO> a;
     a.x = c.x.as<double>(); a.y = c.y.as<double>(); a.z = c.z.as<double>();
     a.d0 = c.diff[0].as<double>(); a.d1 = c.diff[1].as<double>(); a.d2 = c.diff[2].as<double>();
@@ -497,32 +645,35 @@ hipError_t force_impl(Context &c, const hdm_force_params &fp) {
     a.f0 = c.force[0].as<FO>(); a.f1 = c.force[1].as<FO>(); a.f2 = c.force[2].as<FO>();
     a.f3 = c.force[3].as<FO>(); a.f4 = c.force[4].as<FO>(); a.f5 = c.force[5].as<FO>();
     a.fp = cast_params<R>(fp);
-    hipLaunchKernelGGL((k_force<R, CT, FO>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, a);
+    a.breakCheck = 0;
+    a.slotLinkW = c.slotLink.as<int32_t>();
+    a.tx = c.tx.as<double>(); a.ty = c.ty.as<double>(); a.tz = c.tz.as<double>();
+    a.initialLen = c.initialLen.as<double>();
+    a.maxLoad = c.hasMaxLoad ? c.maxLoad.as<double>() : nullptr;
+    a.state = c.state.as<uint8_t>();
+    a.bendBreak = 0;
+    a.scalarMaxLoad = 0;
+    return a;
+}
+
+template <class R, class CT, class FO>
+hipError_t force_impl(Context &c, const hdm_force_params &fp) {
+    hipLaunchKernelGGL((k_force<R, CT, FO>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, force_args<R, CT, FO>(c, fp));
     return hipGetLastError();
 }
 
-template <class FO>
-hipError_t integrate_impl(Context &c, const hdm_integ_params &ip, uint64_t *skin, bool withDisplacement) {
-    IntegArgs<FO> a;
-    a.blockId = c.nBlocks > 0 ? c.blockId.as<int32_t>() : nullptr;
-    a.blockWater = c.nBlocks > 0 ? c.blockWater.as<double>() : nullptr;
-    a.skin = skin;
-    a.x0 = withDisplacement ? c.x0.as<double>() : nullptr;
-    a.y0 = withDisplacement ? c.y0.as<double>() : nullptr;
-    a.z0 = withDisplacement ? c.z0.as<double>() : nullptr;
-    a.x = c.x.as<double>(); a.y = c.y.as<double>(); a.z = c.z.as<double>();
-    a.tx = c.tx.as<double>(); a.ty = c.ty.as<double>(); a.tz = c.tz.as<double>();
-    a.d0 = c.diff[0].as<double>(); a.d1 = c.diff[1].as<double>(); a.d2 = c.diff[2].as<double>();
-    a.d3 = c.diff[3].as<double>(); a.d4 = c.diff[4].as<double>(); a.d5 = c.diff[5].as<double>();
-    a.radius = c.radius.as<double>();
-    a.f0 = c.force[0].as<FO>(); a.f1 = c.force[1].as<FO>(); a.f2 = c.force[2].as<FO>();
-    a.f3 = c.force[3].as<FO>(); a.f4 = c.force[4].as<FO>(); a.f5 = c.force[5].as<FO>();
-    a.type = c.type.as<uint8_t>();
-    a.n = c.n;
-    a.ip = ip;
-    hipLaunchKernelGGL((k_integrate<FO>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, a);
+// Skin slots start at ordered_key(0.0); k_skin_finalize folds the rows of the
+// workgroups that ran (one per block of the integrating launch).
+hipError_t skin_finalize(Context &c, uint64_t *skin) {
+    hipError_t e = launch_fill_u64(c.stream, skin, ordered_key(0.0), 5);
+    if (e != hipSuccess) return e;
+    const int64_t rows = blocks_for(c.n);
+    hipLaunchKernelGGL(k_skin_finalize, dim3(reduce_grid(c, rows)), dim3(TPB), 0, c.stream,
+                       c.skinPartials.as<double>(), rows, skin);
     return hipGetLastError();
 }
+
+hipError_t ensure_partials(Context &c) { return c.skinPartials.ensure(sizeof(double) * 4 * (size_t)blocks_for(c.n)); }
 
 }  // namespace
 
@@ -535,11 +686,58 @@ hipError_t launch_force(Context &c, const hdm_force_params &fp) {
 hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *skin, bool withDisplacement) {
     if (c.n == 0) return hipSuccess;
     if (skin) {
-        hipError_t e = launch_fill_u64(c.stream, skin, ordered_key(0.0), 5);
+        hipError_t e = ensure_partials(c);
         if (e != hipSuccess) return e;
     }
-    if (c.prec == HDM_PREC_FP64) return integrate_impl<double>(c, ip, skin, withDisplacement);
-    return integrate_impl<float>(c, ip, skin, withDisplacement);
+    IntegArgs a = integ_args(c, ip, skin != nullptr, withDisplacement, nullptr);
+    if (c.prec == HDM_PREC_FP64)
+        hipLaunchKernelGGL(k_integrate<double>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, a,
+                           c.force[0].as<double>(), c.force[1].as<double>(), c.force[2].as<double>(),
+                           c.force[3].as<double>(), c.force[4].as<double>(), c.force[5].as<double>());
+    else
+        hipLaunchKernelGGL(k_integrate<float>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, a,
+                           c.force[0].as<float>(), c.force[1].as<float>(), c.force[2].as<float>(),
+                           c.force[3].as<float>(), c.force[4].as<float>(), c.force[5].as<float>());
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !skin) return e;
+    return skin_finalize(c, skin);
+}
+
+hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_params &ip, uint64_t *skin,
+                       bool withDisplacement, const hdm_break_params *brk) {
+    if (c.n == 0) return hipSuccess;
+    const size_t b = sizeof(double) * (size_t)c.n;
+    for (int k = 0; k < 12; k++) {
+        hipError_t e = c.alt[k].ensure(b);
+        if (e != hipSuccess) return e;
+    }
+    if (skin) {
+        hipError_t e = ensure_partials(c);
+        if (e != hipSuccess) return e;
+    }
+    IntegArgs g = integ_args(c, ip, skin != nullptr, withDisplacement, c.alt);
+    auto launch = [&](auto f) {
+        if (brk && c.L > 0) {
+            f.breakCheck = 1;
+            f.bendBreak = brk->bendBreak;
+            f.scalarMaxLoad = brk->maxLoad;
+        }
+        using A = decltype(f);
+        if constexpr (std::is_same<A, ForceArgs<double, double, double>>::value)
+            hipLaunchKernelGGL((k_step<double, double>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, f, g);
+        else
+            hipLaunchKernelGGL((k_step<float, float>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, f, g);
+    };
+    if (c.prec == HDM_PREC_FP64) launch(force_args<double, double, double>(c, fp));
+    else launch(force_args<float, float, float>(c, fp));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // the new state becomes current; the old one is the next step's target
+    DevBuf *cur[12] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz,
+                       &c.diff[0], &c.diff[1], &c.diff[2], &c.diff[3], &c.diff[4], &c.diff[5]};
+    for (int k = 0; k < 12; k++) swap_buf(*cur[k], c.alt[k]);
+    if (!skin) return hipSuccess;
+    return skin_finalize(c, skin);
 }
 
 hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter) {

@@ -149,7 +149,8 @@ typedef struct hdm_run_stats {
 
 /* Per-kernel-class device time accumulated with HIP events on the context's
  * stream since the last reset (milliseconds, and launch counts). */
-enum { HDM_T_FORCE = 0, HDM_T_INTEGRATE, HDM_T_BREAK, HDM_T_SKIN, HDM_T_GRID, HDM_T_NEIGH, HDM_T_LINKS, HDM_T_COUNT };
+enum { HDM_T_FORCE = 0, HDM_T_INTEGRATE, HDM_T_BREAK, HDM_T_SKIN, HDM_T_GRID, HDM_T_NEIGH, HDM_T_LINKS,
+       HDM_T_STEP, HDM_T_COUNT };
 
 /* ---- context ------------------------------------------------------------ */
 int  hdm_ctx_create(int32_t precision, int32_t device, hdm_ctx **out);

@@ -3,6 +3,7 @@ header declares, rejects use without a gfx950 device, and its (host-side)
 seeded generator produces the documented lattice and links."""
 import ctypes as C
 import math
+import re
 import subprocess
 
 import numpy as np
@@ -101,3 +102,11 @@ def test_morton_order_is_a_permutation_of_lattice_order():
     # Morton order is spatially local: consecutive particles stay close
     step = np.linalg.norm(np.diff(pm, axis=0), axis=1)
     assert np.median(step) < 1.5 * a
+
+
+def test_timer_names_match_header_enum():
+    hdr = open(A.HEADER_PATH).read()
+    body = re.search(r"enum \{ HDM_T_FORCE = 0,(.*?)HDM_T_COUNT \};", hdr, re.S).group(1)
+    names = ["HDM_T_FORCE"] + [s.strip() for s in body.split(",") if s.strip()]
+    assert len(names) == len(A.TIMER_NAMES)
+    assert [n[6:].lower() for n in names] == list(A.TIMER_NAMES)
