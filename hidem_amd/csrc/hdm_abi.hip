@@ -222,7 +222,8 @@ int hdm_device_bytes(hdm_ctx *ctx, int64_t *bytes) {
                                  &c.state, &c.initialLen, &c.maxLoad, &c.linkCount, &c.linkCursor, &c.slotLink,
                                  &c.slotOther, &c.sidePos, &c.nbrCount, &c.nbr, &c.colour, &c.order,
                                  &c.cellStart, &c.cellCursor, &c.scanPartials, &c.red, &c.x0, &c.y0, &c.z0,
-                                 &c.skinPartials, &c.blockId, &c.blockWater, &c.probe};
+                                 &c.skinPartials, &c.blockId, &c.blockWater, &c.probe, &c.rec, &c.recAlt,
+                                 &c.drec, &c.drecAlt, &c.slotBrk};
     for (auto *b : bufs) t += b->bytes;
     for (int k = 0; k < 6; k++) t += c.diff[k].bytes + c.force[k].bytes + c.ct[k].bytes + c.ctStage[k].bytes;
     for (int k = 0; k < 12; k++) t += c.alt[k].bytes;
@@ -268,6 +269,7 @@ int hdm_particles_upload(hdm_ctx *ctx, const hdm_particles_host *p) {
     c.neighBuilt = false;
     c.snapshotValid = false;
     c.skinFused = false;
+    c.recCurrent = false;
     return HDM_OK;
 }
 
@@ -495,8 +497,10 @@ int hdm_links_build(hdm_ctx *ctx) {
     HIP_CHECK(ctx, c.slotOther.ensure(sizeof(int32_t) * slots));
     HIP_CHECK(ctx, c.sidePos.ensure(sizeof(int32_t) * (size_t)(2 * c.L > 0 ? 2 * c.L : 1)));
     for (int k = 0; k < 6; k++) HIP_CHECK(ctx, c.ct[k].ensure(c.ctBytes() * slots));
+    HIP_CHECK(ctx, c.slotBrk.ensure(sizeof(float) * 2 * slots));
     HIP_CHECK(ctx, hdm::launch_link_fill(c));
     HIP_CHECK(ctx, hdm::launch_ct_import(c));
+    HIP_CHECK(ctx, hdm::launch_side_thresholds(c));
     HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
     c.linksBuilt = true;
     return HDM_OK;
@@ -558,6 +562,7 @@ int hdm_integrate(hdm_ctx *ctx, const hdm_integ_params *ip) {
     Timer t(ctx->c, HDM_T_INTEGRATE);
     HIP_CHECK(ctx, hdm::launch_integrate(ctx->c, *ip));
     ctx->c.skinFused = false;
+    ctx->c.recCurrent = false;
     return HDM_OK;
 }
 

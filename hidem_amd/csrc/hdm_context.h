@@ -86,6 +86,14 @@ struct Context {
     // second state buffer of the fused step (x, y, z, x..zTheta, Diff0..5):
     // each hdm_run step integrates into it and then swaps it with the above
     DevBuf alt[12];
+    // gather records of the fused step, ping-pong like the state:
+    //   rec   double4 {x, y, z, radius}; the radius reads NaN when a Diff
+    //         component exceeds recTame (then the contact skip test fails)
+    //   drec  8 x (f64 | f32 in the mixed form): Diff0..5, rotation bound
+    //         |xTheta| + |yTheta| + |zTheta| (as a float), 0
+    DevBuf rec, recAlt, drec, drecAlt;
+    bool recCurrent = false;     // rec/drec hold the current SoA state
+    double recTame = 0;          // Diff bound the radius flags were made with
     // batched form: block id per particle, waterline per block (nBlocks = 0: unused)
     int32_t nBlocks = 0;
     DevBuf blockId, blockWater;
@@ -99,6 +107,7 @@ struct Context {
     bool linksBuilt = false;
     DevBuf linkCount, linkCursor, slotLink, slotOther, ct[6];
     DevBuf sidePos;        // 2L: tiled slot of side (l, end)
+    DevBuf slotBrk;        // float2 per slot: (initialLen, maxLoad) of the side's link, for the break prefilter
     // neighbour list (wave tiles)
     int32_t maxNeigh = 0;
     bool neighBuilt = false;
@@ -150,6 +159,8 @@ hipError_t launch_link_count(Context &c);
 hipError_t launch_link_fill(Context &c);
 hipError_t launch_ct_import(Context &c);   // ctStage (checkpoint order) -> ct (tiled)
 hipError_t launch_ct_export(Context &c);   // ct (tiled) -> ctStage (checkpoint order)
+hipError_t launch_side_thresholds(Context &c);  // initialLen / maxLoad -> slotBrk (after link_fill)
+hipError_t launch_pack_records(Context &c, double tame);  // SoA state -> rec / drec
 hipError_t launch_force(Context &c, const hdm_force_params &fp);
 hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *skin = nullptr,
                             bool withDisplacement = false);

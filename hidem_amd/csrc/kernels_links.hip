@@ -126,6 +126,19 @@ __global__ void __launch_bounds__(TPB) k_ct_export(const int32_t *__restrict__ s
     s5[i] = (double)c5[d];
 }
 
+// Per-side copy of the link's fracture parameters in float, in the side's
+// wave-tile slot, so the fused step can prefilter the break test with one
+// coalesced 8 B load per side (the exact test still reads the FP64 values).
+__global__ void __launch_bounds__(TPB) k_side_thresholds(const int32_t *__restrict__ sidePos, int64_t nSides,
+                                                          const double *__restrict__ initialLen,
+                                                          const double *__restrict__ maxLoad,
+                                                          float2 *__restrict__ slotBrk) {
+    int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= nSides) return;
+    int64_t l = i >> 1;
+    slotBrk[sidePos[i]] = make_float2((float)initialLen[l], maxLoad ? (float)maxLoad[l] : 0.0f);
+}
+
 __global__ void __launch_bounds__(TPB) k_count_broken(const uint8_t *__restrict__ state, int64_t L, uint64_t *out) {
     uint32_t c = 0;
     for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < L; i += (int64_t)gridDim.x * TPB)
@@ -200,6 +213,15 @@ hipError_t launch_ct_export(Context &c) {
                            c.sidePos.as<int32_t>(), nSides, d[0], d[1], d[2], d[3], d[4], d[5], s[0], s[1], s[2],
                            s[3], s[4], s[5]);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_side_thresholds(Context &c) {
+    int64_t nSides = 2 * c.L;
+    if (nSides == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_side_thresholds, dim3(blocks_for(nSides)), dim3(TPB), 0, c.stream, c.sidePos.as<int32_t>(),
+                       nSides, c.initialLen.as<double>(), c.hasMaxLoad ? c.maxLoad.as<double>() : nullptr,
+                       c.slotBrk.as<float2>());
     return hipGetLastError();
 }
 

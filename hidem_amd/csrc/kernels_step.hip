@@ -20,6 +20,7 @@
 // broken flag rides in the sign bit of its slot word (no per-link gather), and
 // contact pairs beyond the attraction range are skipped when that provably
 // leaves the force unchanged (contact_is_inert).
+#include <cmath>
 #include <type_traits>
 
 #include "hdm_context.h"
@@ -35,12 +36,22 @@ template <class R>
 struct ForceParams {
     R invDt, attractDistance, attractScaleForce, damping, dampingRotation, scaleE;
     R E, W2, W4, G;
+    int32_t skipInert;  // contact constants small enough for contact_is_inert's argument
 };
+
+// The inert-contact skip relies on every term of the contact law staying
+// finite in R for radius sums up to 1e6; these bounds on the law's constants
+// guarantee it (Cnst <= sqrt(1e18) * 1e15 or 1e12 * 1e6 * 1e15 < FLT_MAX).
+inline bool inert_skip_allowed(const hdm_force_params &f) {
+    return std::fabs(f.attractDistance) <= 1e6 && std::fabs(f.attractScaleForce) <= 1e12 &&
+           std::fabs(f.scaleE) <= 1e15;
+}
 
 template <class R>
 ForceParams<R> cast_params(const hdm_force_params &f) {
     return ForceParams<R>{(R)f.invDt, (R)f.attractDistance, (R)f.attractScaleForce, (R)f.damping,
-                          (R)f.dampingRotation, (R)f.scaleE, (R)f.E, (R)f.W2, (R)f.W4, (R)f.G};
+                          (R)f.dampingRotation, (R)f.scaleE, (R)f.E, (R)f.W2, (R)f.W4, (R)f.G,
+                          inert_skip_allowed(f) ? 1 : 0};
 }
 
 // HiDEMDefNeighForceFunc::operator() (HiDEMDefaultFunctions.h:38-86).
@@ -81,17 +92,21 @@ template <> __device__ __forceinline__ float huge_value<float>() { return 1e37f;
 // pair can be skipped without changing a bit of the result: beyond
 // radSum + attractDistance the law multiplies its terms by mask = 0, and while
 // those terms are finite the product is +-0.  Adding +-0 leaves F unchanged,
-// because F starts at +0 and round-to-nearest sums never yield -0.  Pairs
-// with non-finite inputs take the full path, so NaN/inf propagate exactly as
-// in the sequential order of CombinedAccel::iterateCombined.  (The damping
-// terms are recomputed here in the law's own expression order.)
+// because F starts at +0 and round-to-nearest sums never yield -0.  The terms
+// are finite when the distance is positive and at most sqrt(huge), the radius
+// sum at most 1e6, the constants within inert_skip_allowed and the damping
+// terms at most huge (recomputed here in the law's own expression order).
+// Other pairs take the full path, so NaN/inf propagate exactly as in the
+// sequential order of CombinedAccel::iterateCombined.
 template <class R>
 __device__ __forceinline__ bool contact_is_inert(R xDist, R yDist, R zDist, const R self[7], const R other[7],
                                                  const ForceParams<R> &fp) {
     R particleDistance2 = xDist * xDist + yDist * yDist + zDist * zDist;
     R radSum = self[6] + other[6];
     R activeDist2 = (radSum + fp.attractDistance) * (radSum + fp.attractDistance);
-    if (!(particleDistance2 >= activeDist2) || !(particleDistance2 <= huge_value<R>())) return false;
+    if (!fp.skipInert || !(particleDistance2 >= activeDist2) || !(particleDistance2 <= huge_value<R>()) ||
+        !(particleDistance2 > R(0)) || !(fabs(radSum) <= R(1e6)))
+        return false;
     bool ok = true;
 #pragma unroll
     for (int c = 0; c < 3; c++) ok &= fabs((-fp.damping) * ((self[c] - other[c]) * fp.invDt)) <= huge_value<R>();
@@ -348,34 +363,23 @@ __device__ __forceinline__ void central_difference(double dragCoeff, double mass
 
 // applyForce: IceBlock (MainIceBlock.cpp:299-400, clamp = HDM_TYPE_FIXED) or
 // GlacierDem (MainDem.cpp:145-259 with an analytic bed, constant external
-// force, HiDEMDefVelocityLimit and HiDEMDefBoundaryCondition).
-// Integrates particle i under force Fin; returns its new coordinates and
-// translational Diff.
-__device__ __forceinline__ void integrate_particle(const IntegArgs &a, int64_t i, const double Fin[6],
-                                                   double newCord[3], double newDiff[3]) {
-    const hdm_integ_params &ip = a.ip;
-    double cord[3] = {a.x[i], a.y[i], a.z[i]};
-    double theta[3] = {a.tx[i], a.ty[i], a.tz[i]};
-    if (ip.variant == HDM_INTEG_ICEBLOCK && (a.type[i] & HDM_TYPE_FIXED)) {
-        a.ox[i] = cord[0]; a.oy[i] = cord[1]; a.oz[i] = cord[2];
-        a.otx[i] = theta[0]; a.oty[i] = theta[1]; a.otz[i] = theta[2];
-        a.od0[i] = 0; a.od1[i] = 0; a.od2[i] = 0;
-        a.od3[i] = 0; a.od4[i] = 0; a.od5[i] = 0;
-        for (int c = 0; c < 3; c++) {
-            newCord[c] = cord[c];
-            newDiff[c] = 0;
-        }
+// force, HiDEMDefVelocityLimit and HiDEMDefBoundaryCondition), on one
+// particle's state in registers: cord/theta/diff are replaced by the new
+// coordinates, rotations and Diff.
+__device__ __forceinline__ void integrate_state(const hdm_integ_params &ip, bool fixed, double radius,
+                                                double waterLevel, const double Fin[6], double cord[3],
+                                                double theta[3], double diff[6]) {
+    if (ip.variant == HDM_INTEG_ICEBLOCK && fixed) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) diff[c] = 0;
         return;
     }
-    double diff[6] = {a.d0[i], a.d1[i], a.d2[i], a.d3[i], a.d4[i], a.d5[i]};
     double F[6] = {Fin[0], Fin[1], Fin[2], Fin[3], Fin[4], Fin[5]};
     double cordPrev[3], thetaPrev[3], vel[6];
-    const double radius = a.radius[i];
     const double diameter = radius * 2.0;
     const double zCord = cord[2];
     const double particleMass = ip.rhoIce * diameter * diameter * diameter;
     const double particleMomentInertia = particleMass * diameter * diameter / 6.0;
-    const double waterLevel = a.blockWater ? a.blockWater[a.blockId[i]] : ip.waterLevel;
     double dragCoeff;
     if (ip.variant == HDM_INTEG_ICEBLOCK) {
         dragCoeff = (zCord > waterLevel) ? ip.airDrag : ip.waterDrag;
@@ -419,21 +423,39 @@ __device__ __forceinline__ void integrate_particle(const IntegArgs &a, int64_t i
             if (cord[c] - radius > ip.upper[c] && ip.bound[2 * c + 1]) cord[c] = cordPrev[c];
         }
     }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        diff[c] = cord[c] - cordPrev[c];
+        diff[3 + c] = theta[c] - thetaPrev[c];
+    }
+}
+
+// Integrates particle i under force Fin from the IntegArgs state arrays;
+// returns its new coordinates and translational Diff.
+__device__ __forceinline__ void integrate_particle(const IntegArgs &a, int64_t i, const double Fin[6],
+                                                   double newCord[3], double newDiff[3]) {
+    double cord[3] = {a.x[i], a.y[i], a.z[i]};
+    double theta[3] = {a.tx[i], a.ty[i], a.tz[i]};
+    double diff[6] = {a.d0[i], a.d1[i], a.d2[i], a.d3[i], a.d4[i], a.d5[i]};
+    const bool fixed = (a.type[i] & HDM_TYPE_FIXED) != 0;
+    const double waterLevel = a.blockWater ? a.blockWater[a.blockId[i]] : a.ip.waterLevel;
+    integrate_state(a.ip, fixed, a.radius[i], waterLevel, Fin, cord, theta, diff);
     a.ox[i] = cord[0];
     a.oy[i] = cord[1];
     a.oz[i] = cord[2];
     a.otx[i] = theta[0];
     a.oty[i] = theta[1];
     a.otz[i] = theta[2];
-    a.od0[i] = cord[0] - cordPrev[0];
-    a.od1[i] = cord[1] - cordPrev[1];
-    a.od2[i] = cord[2] - cordPrev[2];
-    a.od3[i] = theta[0] - thetaPrev[0];
-    a.od4[i] = theta[1] - thetaPrev[1];
-    a.od5[i] = theta[2] - thetaPrev[2];
+    a.od0[i] = diff[0];
+    a.od1[i] = diff[1];
+    a.od2[i] = diff[2];
+    a.od3[i] = diff[3];
+    a.od4[i] = diff[4];
+    a.od5[i] = diff[5];
+#pragma unroll
     for (int c = 0; c < 3; c++) {
         newCord[c] = cord[c];
-        newDiff[c] = cord[c] - cordPrev[c];
+        newDiff[c] = diff[c];
     }
 }
 
@@ -478,24 +500,217 @@ __global__ void __launch_bounds__(TPB) k_integrate(IntegArgs a, const FO *__rest
     if (a.partials) skin_partials(a, active, i, cord, diff);
 }
 
+// ---- fused step ------------------------------------------------------------------
+// Gather records (see Context::rec / drec) and the break prefilter inputs.
+template <class R>
+struct StepRecs {
+    const double4 *rec;        // current state
+    const R *drec;
+    double4 *recOut;           // integrated state
+    R *drecOut;
+    double tame;               // Diff bound behind the radius flag of rec
+    const float2 *slotBrk;     // per side (initialLen, maxLoad) in float
+    int32_t prefilter;         // 0: every intact side takes the exact break test
+    int32_t perLinkMaxLoad;
+    float scalarMaxLoad, bendBreak;
+};
+
+// Upper bound of max |theta_p - theta_q| over components, as a float: the sum
+// of absolute values, NaN/inf when any component is (then the prefilter
+// defers to the exact test).
+__device__ __forceinline__ float rotation_bound(const double t[3]) {
+    return (float)(fabs(t[0]) + fabs(t[1]) + fabs(t[2]));
+}
+
+template <class R>
+__device__ __forceinline__ void load_drec(const R *drec, int64_t q, R o[6], float &rot);
+template <>
+__device__ __forceinline__ void load_drec<double>(const double *drec, int64_t q, double o[6], float &rot) {
+    const double4 *v = reinterpret_cast<const double4 *>(drec + q * 8);
+    const double4 a = v[0], b = v[1];
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; o[4] = b.x; o[5] = b.y;
+    rot = (float)b.z;
+}
+template <>
+__device__ __forceinline__ void load_drec<float>(const float *drec, int64_t q, float o[6], float &rot) {
+    const float4 *v = reinterpret_cast<const float4 *>(drec + q * 8);
+    const float4 a = v[0], b = v[1];
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; o[4] = b.x; o[5] = b.y;
+    rot = b.z;
+}
+
+template <class R>
+__device__ __forceinline__ void store_records(const StepRecs<R> &r, int64_t p, const double cord[3],
+                                              const double theta[3], const double diff[6], double radius) {
+    bool tame = fabs(radius) <= 5e5;
+#pragma unroll
+    for (int c = 0; c < 6; c++) tame &= fabs(diff[c]) <= r.tame;
+    r.recOut[p] = make_double4(cord[0], cord[1], cord[2], tame ? radius : __builtin_nan(""));
+    R *d = r.drecOut + p * 8;
+#pragma unroll
+    for (int c = 0; c < 6; c++) d[c] = (R)diff[c];
+    d[6] = (R)rotation_bound(theta);
+    d[7] = 0;
+}
+
+// Break prefilter: true when the link certainly stays intact, i.e. the float
+// estimate of (s - L0) + bendBreak * bound * L0 is below maxLoad * L0 by more
+// than 1e-5 of the magnitudes involved.  Float rounding of the inputs and
+// operations is below 1e-6 of those magnitudes, the rotation bound is at
+// least the exact maximum bend, and the exact test is
+// (s - L0) / L0 + bendBreak * maxBend >= maxLoad with L0 > 0 and
+// bendBreak >= 0 (the host disables the prefilter otherwise), so a link the
+// exact FP64 test would break never passes.  NaN/inf inputs fail the
+// comparison and go to the exact test.
+__device__ __forceinline__ bool certainly_intact(float2 brk, float maxLoad, float bendBreak, double dx, double dy,
+                                                 double dz, float rotBound) {
+    const float L0 = brk.x;
+    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+    const float s = sqrtf(fx * fx + fy * fy + fz * fz);
+    const float bend = bendBreak * rotBound * L0;
+    const float lhs = (s - L0) + bend;
+    const float rhs = maxLoad * L0;
+    const float margin = 1e-5f * (s + L0 + bend + fabsf(rhs) + fabsf(lhs));
+    return L0 >= 1e-30f && lhs + margin < rhs;
+}
+
 // One timestep of hdm_run in one launch: the link break of the previous
 // timestep (when f.breakCheck), force on particle p from the current state
-// (k_force), then its integration into the other state buffer (k_integrate),
-// with the rebuild-test partials of the new state.  The force is also stored,
-// as the separate kernels would leave it.
+// (CombinedAccel::iterateCombined, as k_force), then its integration into the
+// other state buffer (as k_integrate), with the rebuild-test partials and the
+// gather records of the new state.  The force is also stored, as the
+// separate kernels would leave it.
+//
+// Partner state comes from the records: one 32 B load of rec decides a
+// contact pair that is provably inert (contact_is_inert: beyond
+// radSum + attractDistance with every Diff bounded by tame, so every term the
+// law adds is +-0); only other pairs, and link partners, load drec.  The
+// break test of an intact side runs exactly (FP64, as k_break) unless the
+// float prefilter proves the link stays intact.
+// The FP32 form fits 4 waves per SIMD (<= 128 VGPRs) without spilling.
 template <class R, class CT>
-__global__ void __launch_bounds__(TPB) k_step(ForceArgs<R, CT, R> f, IntegArgs g) {
+__global__ void __launch_bounds__(TPB, sizeof(R) == 4 ? 4 : 1) k_step(ForceArgs<R, CT, R> f, IntegArgs g, StepRecs<R> r) {
     const int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
     const bool active = p < f.n;
-    double cord[3] = {0, 0, 0}, diff[3] = {0, 0, 0};
+    double cordN[3] = {0, 0, 0}, diffN[3] = {0, 0, 0};
     if (active) {
-        R F[6];
-        force_particle(f, p, F);
+        const double4 P = r.rec[p];
+        const double xCord = P.x, yCord = P.y, zCord = P.z;
+        double diff[6] = {g.d0[p], g.d1[p], g.d2[p], g.d3[p], g.d4[p], g.d5[p]};
+        const double radius = g.radius[p];
+        double theta[3] = {g.tx[p], g.ty[p], g.tz[p]};
+        const float rotSelf = rotation_bound(theta);
+        R self[7] = {(R)diff[0], (R)diff[1], (R)diff[2], (R)diff[3], (R)diff[4], (R)diff[5], (R)radius};
+        R F[6] = {0, 0, 0, 0, 0, 0};
+        const int64_t lane = p & 63;
+        {
+            const int32_t cnt = f.nbrCount[p];
+            const int32_t *nl = f.nbr + (p >> 6) * (int64_t)64 * f.nbrWidth + lane;
+            for (int32_t k = 0; k < cnt; k++) {
+                const int32_t q = nl[(int64_t)k * 64];
+                const double4 Q = r.rec[q];
+                const R xDist = (R)(xCord - Q.x);
+                const R yDist = (R)(yCord - Q.y);
+                const R zDist = (R)(zCord - Q.z);
+                {
+                    // contact_is_inert with contact_law's own expressions; the
+                    // record radius is NaN unless both the Diff (damping terms)
+                    // and the radius are within their bounds
+                    const R particleDistance2 = xDist * xDist + yDist * yDist + zDist * zDist;
+                    const R radSum = (R)P.w + (R)Q.w;
+                    const R activeDist = radSum + f.fp.attractDistance;
+                    if (particleDistance2 >= activeDist * activeDist && particleDistance2 <= huge_value<R>() &&
+                        particleDistance2 > R(0))
+                        continue;
+                }
+                R other[7];
+                float rotOther;
+                load_drec<R>(r.drec, q, other, rotOther);
+                other[6] = (R)f.radius[q];
+                contact_law<R>(R(1), xDist, yDist, zDist, self, other, f.fp, F);
+            }
+        }
+        {
+            const int32_t cnt = (int32_t)f.linkCount[p];
+            const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + lane;
+            for (int32_t k = 0; k < cnt; k++) {
+                const int64_t sidx = base + (int64_t)k * 64;
+                const int32_t q = f.slotOther[sidx];
+                const double4 Q = r.rec[q];
+                const double dx = xCord - Q.x, dy = yCord - Q.y, dz = zCord - Q.z;
+                R other[7];
+                float rotOther;
+                load_drec<R>(r.drec, q, other, rotOther);
+                other[6] = 0;  // beam_law reads no radius
+                const int32_t word = f.slotLink[sidx];
+                R mask = 1;
+                if (word < 0) {
+                    mask = 0;  // broken link (state == 0)
+                } else if (f.breakCheck) {
+                    const float maxLoadF = r.perLinkMaxLoad ? r.slotBrk[sidx].y : r.scalarMaxLoad;
+                    const bool skip = r.prefilter && certainly_intact(r.slotBrk[sidx], maxLoadF, r.bendBreak, dx, dy,
+                                                                      dz, rotSelf + rotOther);
+                    if (!skip && link_breaks(f, word, q, dx, dy, dz, theta)) {
+                        // both ends reach the same verdict; each flags its own slot
+                        mask = 0;
+                        f.slotLinkW[sidx] = word | SLOT_BROKEN;
+                        f.state[word] = 0;
+                    }
+                }
+                R ct[6] = {(R)f.c0[sidx], (R)f.c1[sidx], (R)f.c2[sidx], (R)f.c3[sidx], (R)f.c4[sidx], (R)f.c5[sidx]};
+                beam_law<R>(mask, (R)dx, (R)dy, (R)dz, self, other, f.fp, ct, F);
+                f.c0[sidx] = (CT)ct[0];
+                f.c1[sidx] = (CT)ct[1];
+                f.c2[sidx] = (CT)ct[2];
+                f.c3[sidx] = (CT)ct[3];
+                f.c4[sidx] = (CT)ct[4];
+                f.c5[sidx] = (CT)ct[5];
+            }
+        }
         store_force(f, p, F);
         const double Fd[6] = {(double)F[0], (double)F[1], (double)F[2], (double)F[3], (double)F[4], (double)F[5]};
-        integrate_particle(g, p, Fd, cord, diff);
+        double cord[3] = {xCord, yCord, zCord};
+        const bool fixed = (g.type[p] & HDM_TYPE_FIXED) != 0;
+        const double waterLevel = g.blockWater ? g.blockWater[g.blockId[p]] : g.ip.waterLevel;
+        integrate_state(g.ip, fixed, radius, waterLevel, Fd, cord, theta, diff);
+        g.ox[p] = cord[0];
+        g.oy[p] = cord[1];
+        g.oz[p] = cord[2];
+        g.otx[p] = theta[0];
+        g.oty[p] = theta[1];
+        g.otz[p] = theta[2];
+        g.od0[p] = diff[0];
+        g.od1[p] = diff[1];
+        g.od2[p] = diff[2];
+        g.od3[p] = diff[3];
+        g.od4[p] = diff[4];
+        g.od5[p] = diff[5];
+        store_records(r, p, cord, theta, diff, radius);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            cordN[c] = cord[c];
+            diffN[c] = diff[c];
+        }
     }
-    if (g.partials) skin_partials(g, active, p, cord, diff);
+    if (g.partials) skin_partials(g, active, p, cordN, diffN);
+}
+
+// Gather records of the current SoA state (hdm_run entry).
+template <class R>
+__global__ void __launch_bounds__(TPB) k_pack_records(const double *__restrict__ x, const double *__restrict__ y,
+                                                       const double *__restrict__ z, const double *__restrict__ tx,
+                                                       const double *__restrict__ ty, const double *__restrict__ tz,
+                                                       const double *__restrict__ d0, const double *__restrict__ d1,
+                                                       const double *__restrict__ d2, const double *__restrict__ d3,
+                                                       const double *__restrict__ d4, const double *__restrict__ d5,
+                                                       const double *__restrict__ radius, int64_t n,
+                                                       StepRecs<R> r) {
+    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (p >= n) return;
+    const double cord[3] = {x[p], y[p], z[p]};
+    const double theta[3] = {tx[p], ty[p], tz[p]};
+    const double diff[6] = {d0[p], d1[p], d2[p], d3[p], d4[p], d5[p]};
+    store_records(r, p, cord, theta, diff, radius[p]);
 }
 
 // Folds the per-workgroup rows into skin[0..2] (max+ Diff) and skin[4]
@@ -703,6 +918,83 @@ hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *sk
     return skin_finalize(c, skin);
 }
 
+namespace {
+
+// Bound on every Diff component under which a pair beyond the attraction
+// range provably adds +-0 (contact_is_inert): the damping terms then stay
+// below 1e300 (FP64) / 1e37 (FP32) in magnitude.  -1: no Diff qualifies.
+double tame_bound(const Context &c, const hdm_force_params &fp) {
+    if (!inert_skip_allowed(fp)) return -1.0;  // no Diff is tame: no pair is skipped
+    double k = 1.0;
+    const double a = fabs(fp.damping * fp.invDt), b = fabs(fp.dampingRotation * fp.invDt);
+    k = a > k ? a : k;
+    k = b > k ? b : k;
+    if (!(k < 1e30)) return -1.0;  // also NaN
+    return (c.prec == HDM_PREC_FP64 ? 1e299 : 1e36) / (4.0 * k);
+}
+
+template <class R>
+StepRecs<R> step_recs(Context &c, double tame, const hdm_break_params *brk) {
+    StepRecs<R> r;
+    r.rec = c.rec.as<double4>();
+    r.drec = c.drec.as<R>();
+    r.recOut = c.recAlt.as<double4>();
+    r.drecOut = c.drecAlt.as<R>();
+    r.tame = tame;
+    r.slotBrk = c.slotBrk.as<float2>();
+    r.perLinkMaxLoad = c.hasMaxLoad ? 1 : 0;
+    r.scalarMaxLoad = brk ? (float)brk->maxLoad : 0.0f;
+    r.bendBreak = brk ? (float)brk->bendBreak : 0.0f;
+    r.prefilter = (brk && brk->bendBreak >= 0.0 && brk->bendBreak < 1e30) ? 1 : 0;
+    return r;
+}
+
+hipError_t ensure_records(Context &c) {
+    const size_t n = (size_t)(c.n > 0 ? c.n : 1);
+    for (DevBuf *b : {&c.rec, &c.recAlt}) {
+        hipError_t e = b->ensure(32 * n);
+        if (e != hipSuccess) return e;
+    }
+    for (DevBuf *b : {&c.drec, &c.drecAlt}) {
+        hipError_t e = b->ensure(8 * c.forceBytes() * n);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t launch_pack_records(Context &c, double tame) {
+    hipError_t e = ensure_records(c);
+    if (e != hipSuccess || c.n == 0) return e;
+    // pack writes the "out" side; swap so the packed records are current
+    swap_buf(c.rec, c.recAlt);
+    swap_buf(c.drec, c.drecAlt);
+    auto go = [&](auto r) {
+        hipLaunchKernelGGL(k_pack_records, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
+                           c.y.as<double>(), c.z.as<double>(), c.tx.as<double>(), c.ty.as<double>(),
+                           c.tz.as<double>(), c.diff[0].as<double>(), c.diff[1].as<double>(), c.diff[2].as<double>(),
+                           c.diff[3].as<double>(), c.diff[4].as<double>(), c.diff[5].as<double>(),
+                           c.radius.as<double>(), c.n, r);
+    };
+    if (c.prec == HDM_PREC_FP64) {
+        StepRecs<double> r = step_recs<double>(c, tame, nullptr);
+        r.recOut = c.rec.as<double4>();
+        r.drecOut = c.drec.as<double>();
+        go(r);
+    } else {
+        StepRecs<float> r = step_recs<float>(c, tame, nullptr);
+        r.recOut = c.rec.as<double4>();
+        r.drecOut = c.drec.as<float>();
+        go(r);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    c.recCurrent = true;
+    c.recTame = tame;
+    return hipSuccess;
+}
+
 hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_params &ip, uint64_t *skin,
                        bool withDisplacement, const hdm_break_params *brk) {
     if (c.n == 0) return hipSuccess;
@@ -715,8 +1007,13 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
         hipError_t e = ensure_partials(c);
         if (e != hipSuccess) return e;
     }
+    const double tame = tame_bound(c, fp);
+    if (!c.recCurrent || c.recTame != tame) {
+        hipError_t e = launch_pack_records(c, tame);
+        if (e != hipSuccess) return e;
+    }
     IntegArgs g = integ_args(c, ip, skin != nullptr, withDisplacement, c.alt);
-    auto launch = [&](auto f) {
+    auto launch = [&](auto f, auto r) {
         if (brk && c.L > 0) {
             f.breakCheck = 1;
             f.bendBreak = brk->bendBreak;
@@ -724,18 +1021,20 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
         }
         using A = decltype(f);
         if constexpr (std::is_same<A, ForceArgs<double, double, double>>::value)
-            hipLaunchKernelGGL((k_step<double, double>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, f, g);
+            hipLaunchKernelGGL((k_step<double, double>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, f, g, r);
         else
-            hipLaunchKernelGGL((k_step<float, float>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, f, g);
+            hipLaunchKernelGGL((k_step<float, float>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, f, g, r);
     };
-    if (c.prec == HDM_PREC_FP64) launch(force_args<double, double, double>(c, fp));
-    else launch(force_args<float, float, float>(c, fp));
+    if (c.prec == HDM_PREC_FP64) launch(force_args<double, double, double>(c, fp), step_recs<double>(c, tame, brk));
+    else launch(force_args<float, float, float>(c, fp), step_recs<float>(c, tame, brk));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // the new state becomes current; the old one is the next step's target
     DevBuf *cur[12] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz,
                        &c.diff[0], &c.diff[1], &c.diff[2], &c.diff[3], &c.diff[4], &c.diff[5]};
     for (int k = 0; k < 12; k++) swap_buf(*cur[k], c.alt[k]);
+    swap_buf(c.rec, c.recAlt);
+    swap_buf(c.drec, c.drecAlt);
     if (!skin) return hipSuccess;
     return skin_finalize(c, skin);
 }
