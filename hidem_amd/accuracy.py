@@ -1,0 +1,103 @@
+#!/usr/bin/env python3
+"""Error of the mixed-precision form against the plain FP64 form.
+
+Runs both forms side by side on one GPU from the same seeded inputs (FP32
+contact and beam arithmetic and CT storage; FP64 positions, integration and
+link strain) and reports, at each checkpoint step:
+
+  position   max |x_mixed - x_fp64| over particles (Euclidean, metres), that
+             maximum relative to the largest coordinate magnitude, and the
+             largest per-particle error relative to that particle's
+             displacement since step 0 (floored at 1 mm)
+  force      ||F_mixed - F_fp64||_2 / ||F_fp64||_2 over all particles
+             (translational components, forces of the checkpoint step)
+  broken     broken-link counts of both forms and their relative difference
+
+Usage: python -m hidem_amd.accuracy [--config glacier_8m] [--checkpoints 100 1000 10000]
+Prints one JSON line per checkpoint and a final summary line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+if __package__ in (None, ""):
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from hidem_amd import _abi as A, params  # noqa: E402
+else:
+    from . import _abi as A, params
+
+
+def _xyz(p):
+    return np.stack([p["xCord"], p["yCord"], p["zCord"]], 1)
+
+
+def _force(p):
+    return np.stack([p["xForce"], p["yForce"], p["zForce"]], 1)
+
+
+def compare(p64, pmx, x0, broken64, brokenmx):
+    x64, xmx = _xyz(p64), _xyz(pmx)
+    err = np.linalg.norm(xmx - x64, axis=1)
+    scale = float(np.abs(x64).max())
+    disp = np.maximum(np.linalg.norm(x64 - x0, axis=1), 1e-3)
+    f64, fmx = _force(p64), _force(pmx)
+    fnorm = float(np.linalg.norm(f64))
+    return {
+        "pos_max_abs_m": float(err.max()),
+        "pos_max_rel": float(err.max() / scale),
+        "pos_max_rel_to_displacement": float((err / disp).max()),
+        "pos_rms_m": float(np.sqrt(np.mean(err ** 2))),
+        "force_rel_l2": float(np.linalg.norm(fmx - f64) / fnorm) if fnorm > 0 else 0.0,
+        "broken_fp64": int(broken64),
+        "broken_mixed": int(brokenmx),
+        "broken_rel_diff": float(abs(brokenmx - broken64) / broken64) if broken64 else float(brokenmx != 0),
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--config", default="glacier_8m")
+    ap.add_argument("--cells", type=int, nargs=3, default=None, help="override unit cells (smaller runs)")
+    ap.add_argument("--checkpoints", type=int, nargs="+", default=[100, 1000, 10000])
+    ap.add_argument("--chunk", type=int, default=500, help="steps per hdm_run call between progress lines")
+    args = ap.parse_args(argv)
+
+    cfg = params.load_config(args.config)
+    setup, p, l = params.build_inputs(cfg, args.cells)
+    x0 = _xyz(p)
+    print(f"# {args.config}: {p.n} particles, {l.n} links", file=sys.stderr, flush=True)
+    out = []
+    t0 = time.perf_counter()
+    with A.Context(A.HDM_PREC_FP64) as c64, A.Context(A.HDM_PREC_MIXED) as cmx:
+        for c in (c64, cmx):
+            c.upload(p, l)
+            c.links_build()
+        done = 0
+        rebuilds = [0, 0]
+        for cp in sorted(args.checkpoints):
+            while done < cp:
+                k = min(args.chunk, cp - done)
+                for i, c in enumerate((c64, cmx)):
+                    st = c.run(setup.run, k)
+                    rebuilds[i] += st.rebuilds
+                done += k
+                print(f"# step {done}  ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+            r = compare(c64.download_particles(), cmx.download_particles(), x0, c64.count_broken(),
+                        cmx.count_broken())
+            r.update(step=cp, rebuilds_fp64=rebuilds[0], rebuilds_mixed=rebuilds[1])
+            out.append(r)
+            print(json.dumps(r), flush=True)
+    summary = {"config": args.config, "particles": p.n, "links": l.n, "checkpoints": out,
+               "seconds": time.perf_counter() - t0}
+    print(json.dumps(summary), flush=True)
+    return summary
+
+
+if __name__ == "__main__":
+    main()

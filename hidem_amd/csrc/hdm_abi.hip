@@ -749,7 +749,7 @@ int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *reb
         {
             Timer t(c, HDM_T_STEP);
             HIP_CHECK(ctx, hdm::launch_step(c, rp->force, rp->integ, wantSkin ? skin : nullptr, wantDisp,
-                                            breakPending ? &rp->brk : nullptr));
+                                            breakPending ? &rp->brk : nullptr, step == nSteps - 1));
         }
         c.skinFused = wantSkin;
         c.skinFusedDisp = wantDisp;

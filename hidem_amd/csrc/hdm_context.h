@@ -167,9 +167,10 @@ hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *sk
 // Force + integration of one timestep in one launch (state ping-pong; the
 // integrated state is current afterwards).  skin as for launch_integrate.
 // brk != nullptr: first apply the link break that the previous timestep's
-// launch_break would have applied to the current state.
+// launch_break would have applied to the current state.  storeForce: write
+// the force arrays (else they keep their previous contents).
 hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_params &ip, uint64_t *skin,
-                       bool withDisplacement, const hdm_break_params *brk);
+                       bool withDisplacement, const hdm_break_params *brk, bool storeForce);
 hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter);
 hipError_t launch_skin_heuristic(Context &c, uint64_t *out3);
 hipError_t launch_skin_exact(Context &c, uint64_t *out1);

@@ -513,6 +513,7 @@ struct StepRecs {
     int32_t prefilter;         // 0: every intact side takes the exact break test
     int32_t perLinkMaxLoad;
     float scalarMaxLoad, bendBreak;
+    int32_t storeForce;        // write x..wForce (the last step of a run; earlier ones are overwritten unseen)
 };
 
 // Upper bound of max |theta_p - theta_q| over components, as a float: the sum
@@ -667,7 +668,7 @@ __global__ void __launch_bounds__(TPB, sizeof(R) == 4 ? 4 : 1) k_step(ForceArgs<
                 f.c5[sidx] = (CT)ct[5];
             }
         }
-        store_force(f, p, F);
+        if (r.storeForce) store_force(f, p, F);
         const double Fd[6] = {(double)F[0], (double)F[1], (double)F[2], (double)F[3], (double)F[4], (double)F[5]};
         double cord[3] = {xCord, yCord, zCord};
         const bool fixed = (g.type[p] & HDM_TYPE_FIXED) != 0;
@@ -946,6 +947,7 @@ StepRecs<R> step_recs(Context &c, double tame, const hdm_break_params *brk) {
     r.scalarMaxLoad = brk ? (float)brk->maxLoad : 0.0f;
     r.bendBreak = brk ? (float)brk->bendBreak : 0.0f;
     r.prefilter = (brk && brk->bendBreak >= 0.0 && brk->bendBreak < 1e30) ? 1 : 0;
+    r.storeForce = 1;
     return r;
 }
 
@@ -996,7 +998,7 @@ hipError_t launch_pack_records(Context &c, double tame) {
 }
 
 hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_params &ip, uint64_t *skin,
-                       bool withDisplacement, const hdm_break_params *brk) {
+                       bool withDisplacement, const hdm_break_params *brk, bool storeForce) {
     if (c.n == 0) return hipSuccess;
     const size_t b = sizeof(double) * (size_t)c.n;
     for (int k = 0; k < 12; k++) {
@@ -1014,6 +1016,7 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
     }
     IntegArgs g = integ_args(c, ip, skin != nullptr, withDisplacement, c.alt);
     auto launch = [&](auto f, auto r) {
+        r.storeForce = storeForce ? 1 : 0;
         if (brk && c.L > 0) {
             f.breakCheck = 1;
             f.bendBreak = brk->bendBreak;
