@@ -13,6 +13,11 @@ link strain) and reports, at each checkpoint step:
              (translational components, forces of the checkpoint step)
   broken     broken-link counts of both forms and their relative difference
 
+With --perturb, a third run (FP64, every coordinate moved by one ulp towards
++inf at step 0) is compared with the FP64 run the same way: the divergence a
+one-ulp difference grows into, i.e. the floor below which no change of
+arithmetic can stay once the dynamics amplify differences (fracture).
+
 Usage: python -m hidem_amd.accuracy [--config glacier_8m] [--checkpoints 100 1000 10000]
 Prints one JSON line per checkpoint and a final summary line.
 """
@@ -66,6 +71,7 @@ def main(argv=None):
     ap.add_argument("--cells", type=int, nargs=3, default=None, help="override unit cells (smaller runs)")
     ap.add_argument("--checkpoints", type=int, nargs="+", default=[100, 1000, 10000])
     ap.add_argument("--chunk", type=int, default=500, help="steps per hdm_run call between progress lines")
+    ap.add_argument("--perturb", action="store_true", help="also run FP64 from one-ulp perturbed coordinates")
     args = ap.parse_args(argv)
 
     cfg = params.load_config(args.config)
@@ -74,25 +80,40 @@ def main(argv=None):
     print(f"# {args.config}: {p.n} particles, {l.n} links", file=sys.stderr, flush=True)
     out = []
     t0 = time.perf_counter()
-    with A.Context(A.HDM_PREC_FP64) as c64, A.Context(A.HDM_PREC_MIXED) as cmx:
-        for c in (c64, cmx):
-            c.upload(p, l)
+    ctxs = [A.Context(A.HDM_PREC_FP64), A.Context(A.HDM_PREC_MIXED)]
+    if args.perturb:
+        ctxs.append(A.Context(A.HDM_PREC_FP64))
+    try:
+        for i, c in enumerate(ctxs):
+            if i == 2:
+                q = p.copy()
+                for f in ("xCord", "yCord", "zCord"):
+                    q.arrays[f] = np.nextafter(q[f], np.inf)
+                c.upload(q, l)
+            else:
+                c.upload(p, l)
             c.links_build()
         done = 0
-        rebuilds = [0, 0]
+        rebuilds = [0] * len(ctxs)
         for cp in sorted(args.checkpoints):
             while done < cp:
                 k = min(args.chunk, cp - done)
-                for i, c in enumerate((c64, cmx)):
+                for i, c in enumerate(ctxs):
                     st = c.run(setup.run, k)
                     rebuilds[i] += st.rebuilds
                 done += k
                 print(f"# step {done}  ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
-            r = compare(c64.download_particles(), cmx.download_particles(), x0, c64.count_broken(),
-                        cmx.count_broken())
+            p64 = ctxs[0].download_particles()
+            b64 = ctxs[0].count_broken()
+            r = compare(p64, ctxs[1].download_particles(), x0, b64, ctxs[1].count_broken())
             r.update(step=cp, rebuilds_fp64=rebuilds[0], rebuilds_mixed=rebuilds[1])
+            if args.perturb:
+                r["fp64_ulp_perturbed"] = compare(p64, ctxs[2].download_particles(), x0, b64, ctxs[2].count_broken())
             out.append(r)
             print(json.dumps(r), flush=True)
+    finally:
+        for c in ctxs:
+            c.close()
     summary = {"config": args.config, "particles": p.n, "links": l.n, "checkpoints": out,
                "seconds": time.perf_counter() - t0}
     print(json.dumps(summary), flush=True)

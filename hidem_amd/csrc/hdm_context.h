@@ -89,7 +89,8 @@ struct Context {
     // gather records of the fused step, ping-pong like the state:
     //   rec   double4 {x, y, z, radius}; the radius reads NaN when a Diff
     //         component exceeds recTame (then the contact skip test fails)
-    //   drec  8 x (f64 | f32 in the mixed form): Diff0..5, rotation bound
+    //   drec  8 x f64: Diff0..5 (FP64 in both forms: pair differences are
+    //         formed before any rounding to FP32), rotation bound
     //         |xTheta| + |yTheta| + |zTheta| (as a float), 0
     DevBuf rec, recAlt, drec, drecAlt;
     bool recCurrent = false;     // rec/drec hold the current SoA state

@@ -54,10 +54,21 @@ ForceParams<R> cast_params(const hdm_force_params &f) {
                           inert_skip_allowed(f) ? 1 : 0};
 }
 
-// HiDEMDefNeighForceFunc::operator() (HiDEMDefaultFunctions.h:38-86).
+// Diff differences of a pair, s - o: in the FP64 form exactly the law's own
+// (self[c] - other[c]); in the FP32 form the FP64 difference rounded once,
+// because neighbouring Diffs are nearly equal and rounding each to FP32 first
+// would cancel most of the significant bits of their difference.
+template <class R>
+__device__ __forceinline__ void diff_delta(const double s[6], const double o[6], R d[6]) {
+#pragma unroll
+    for (int c = 0; c < 6; c++) d[c] = (R)(s[c] - o[c]);
+}
+
+// HiDEMDefNeighForceFunc::operator() (HiDEMDefaultFunctions.h:38-86); the
+// damping terms take the Diff differences d (diff_delta).
 template <class R>
 __device__ __forceinline__ void contact_law(R mask, R xDist, R yDist, R zDist, const R self[7], const R other[7],
-                                            const ForceParams<R> &fp, R F[6]) {
+                                            const R d[6], const ForceParams<R> &fp, R F[6]) {
     R xDist2 = xDist * xDist;
     R yDist2 = yDist * yDist;
     R zDist2 = zDist * zDist;
@@ -76,12 +87,12 @@ __device__ __forceinline__ void contact_law(R mask, R xDist, R yDist, R zDist, c
     Cnst = (particleDistance >= radSum && particleDistance < radSum + fp.attractDistance)
                ? fp.attractScaleForce * dimDist : Cnst;
     Cnst *= fp.scaleE;
-    F[0] += mask * ((-fp.damping) * ((self[0] - other[0]) * fp.invDt) + (Cnst * xComp));
-    F[1] += mask * ((-fp.damping) * ((self[1] - other[1]) * fp.invDt) + (Cnst * yComp));
-    F[2] += mask * ((-fp.damping) * ((self[2] - other[2]) * fp.invDt) + (Cnst * zComp));
-    F[3] += mask * ((-fp.dampingRotation) * ((self[3] - other[3]) * fp.invDt));
-    F[4] += mask * ((-fp.dampingRotation) * ((self[4] - other[4]) * fp.invDt));
-    F[5] += mask * ((-fp.dampingRotation) * ((self[5] - other[5]) * fp.invDt));
+    F[0] += mask * ((-fp.damping) * (d[0] * fp.invDt) + (Cnst * xComp));
+    F[1] += mask * ((-fp.damping) * (d[1] * fp.invDt) + (Cnst * yComp));
+    F[2] += mask * ((-fp.damping) * (d[2] * fp.invDt) + (Cnst * zComp));
+    F[3] += mask * ((-fp.dampingRotation) * (d[3] * fp.invDt));
+    F[4] += mask * ((-fp.dampingRotation) * (d[4] * fp.invDt));
+    F[5] += mask * ((-fp.dampingRotation) * (d[5] * fp.invDt));
 }
 
 template <class R> __device__ __forceinline__ R huge_value();
@@ -100,7 +111,7 @@ template <> __device__ __forceinline__ float huge_value<float>() { return 1e37f;
 // sequential order of CombinedAccel::iterateCombined.
 template <class R>
 __device__ __forceinline__ bool contact_is_inert(R xDist, R yDist, R zDist, const R self[7], const R other[7],
-                                                 const ForceParams<R> &fp) {
+                                                 const R d[6], const ForceParams<R> &fp) {
     R particleDistance2 = xDist * xDist + yDist * yDist + zDist * zDist;
     R radSum = self[6] + other[6];
     R activeDist2 = (radSum + fp.attractDistance) * (radSum + fp.attractDistance);
@@ -109,22 +120,26 @@ __device__ __forceinline__ bool contact_is_inert(R xDist, R yDist, R zDist, cons
         return false;
     bool ok = true;
 #pragma unroll
-    for (int c = 0; c < 3; c++) ok &= fabs((-fp.damping) * ((self[c] - other[c]) * fp.invDt)) <= huge_value<R>();
+    for (int c = 0; c < 3; c++) ok &= fabs((-fp.damping) * (d[c] * fp.invDt)) <= huge_value<R>();
 #pragma unroll
-    for (int c = 3; c < 6; c++) ok &= fabs((-fp.dampingRotation) * ((self[c] - other[c]) * fp.invDt)) <= huge_value<R>();
+    for (int c = 3; c < 6; c++) ok &= fabs((-fp.dampingRotation) * (d[c] * fp.invDt)) <= huge_value<R>();
     return ok;
 }
 
-// HiDEMDefLinkForceFunc::operator() (HiDEMDefaultFunctions.h:105-203).
+// HiDEMDefLinkForceFunc::operator() (HiDEMDefaultFunctions.h:105-203).  The
+// FP64 form evaluates the reference expressions in their order.  The FP32
+// form groups the same terms by pair differences, B0 - B6 = T0 . (s - o) etc.,
+// with s - o = d from diff_delta, so the CT increments keep the relative
+// motion of the two ends instead of the difference of two rounded values.
 template <class R>
 __device__ __forceinline__ void beam_law(R mask, R xDist, R yDist, R zDist, const R s[7], const R o[7],
-                                         const ForceParams<R> &fp, R ct[6], R F[6]) {
-    F[0] += (-fp.damping * ((s[0] - o[0]) * fp.invDt)) * mask;
-    F[1] += (-fp.damping * ((s[1] - o[1]) * fp.invDt)) * mask;
-    F[2] += (-fp.damping * ((s[2] - o[2]) * fp.invDt)) * mask;
-    F[3] += (-fp.dampingRotation * ((s[3] - o[3]) * fp.invDt)) * mask;
-    F[4] += (-fp.dampingRotation * ((s[4] - o[4]) * fp.invDt)) * mask;
-    F[5] += (-fp.dampingRotation * ((s[5] - o[5]) * fp.invDt)) * mask;
+                                         const R d[6], const ForceParams<R> &fp, R ct[6], R F[6]) {
+    F[0] += (-fp.damping * (d[0] * fp.invDt)) * mask;
+    F[1] += (-fp.damping * (d[1] * fp.invDt)) * mask;
+    F[2] += (-fp.damping * (d[2] * fp.invDt)) * mask;
+    F[3] += (-fp.dampingRotation * (d[3] * fp.invDt)) * mask;
+    F[4] += (-fp.dampingRotation * (d[4] * fp.invDt)) * mask;
+    F[5] += (-fp.dampingRotation * (d[5] * fp.invDt)) * mask;
 
     R xDist2 = xDist * xDist;
     R yDist2 = yDist * yDist;
@@ -145,17 +160,6 @@ __device__ __forceinline__ void beam_law(R mask, R xDist, R yDist, R zDist, cons
     R T21 = -(zDist * yDist) * invLILP;
     R T22 = (xDist2 + yDist2) * invLILP;
 
-    R B0 = T00 * s[0] + T01 * s[1] + T02 * s[2];
-    R B1 = T10 * s[0] + T11 * s[1];
-    R B2 = T20 * s[0] + T21 * s[1] + T22 * s[2];
-    R B4 = T10 * s[3] + T11 * s[4];
-    R B5 = T20 * s[3] + T21 * s[4] + T22 * s[5];
-    R B6 = T00 * o[0] + T01 * o[1] + T02 * o[2];
-    R B7 = T10 * o[0] + T11 * o[1];
-    R B8 = T20 * o[0] + T21 * o[1] + T22 * o[2];
-    R B10 = T10 * o[3] + T11 * o[4];
-    R B11 = T20 * o[3] + T21 * o[4] + T22 * o[5];
-
     R L2 = LI * LI;
     R L3 = LI * LI * LI;
     R LIDiv6 = (R(1) / R(6)) * invLI;
@@ -163,25 +167,55 @@ __device__ __forceinline__ void beam_law(R mask, R xDist, R yDist, R zDist, cons
     R K0 = fp.E * fp.W2 * invLI;
     R K1 = fp.E * fp.W4 / L3;
     R K5 = fp.E * fp.W4 / (R(2) * L2);
-
-    ct[0] += K0 * B0 + (-K0) * B6;
-    ct[1] += K1 * B1 + K5 * B5 + (-K1) * B7 + K5 * B11;
-    ct[2] += K1 * B2 + (-K5) * B4 + (-K1) * B8 + (-K5) * B10;
-
-    F[0] += (-(T00 * ct[0] + T10 * ct[1] + T20 * ct[2])) * mask;
-    F[1] += (-(T01 * ct[0] + T11 * ct[1] + T21 * ct[2])) * mask;
-    F[2] += (-(T02 * ct[0] + T22 * ct[2])) * mask;
-
     R K2 = fp.G * fp.W4 * LIDiv6;
     R K3 = fp.E * fp.W4 * LIDiv6 * R(2);
     R K4 = fp.E * fp.W4 * LIDiv6;
 
-    R B3 = T00 * s[3] + T01 * s[4] + T02 * s[5];
-    R B9 = T00 * o[3] + T01 * o[4] + T02 * o[5];
+    R B4 = T10 * s[3] + T11 * s[4];
+    R B5 = T20 * s[3] + T21 * s[4] + T22 * s[5];
+    R B10 = T10 * o[3] + T11 * o[4];
+    R B11 = T20 * o[3] + T21 * o[4] + T22 * o[5];
 
-    ct[3] += K2 * B3 + (-K2) * B9;
-    ct[4] += (-K5) * B2 + K3 * B4 + K5 * B8 + K4 * B10;
-    ct[5] += K5 * B1 + K3 * B5 + (-K5) * B7 + K4 * B11;
+    if constexpr (sizeof(R) == 8) {
+        R B0 = T00 * s[0] + T01 * s[1] + T02 * s[2];
+        R B1 = T10 * s[0] + T11 * s[1];
+        R B2 = T20 * s[0] + T21 * s[1] + T22 * s[2];
+        R B6 = T00 * o[0] + T01 * o[1] + T02 * o[2];
+        R B7 = T10 * o[0] + T11 * o[1];
+        R B8 = T20 * o[0] + T21 * o[1] + T22 * o[2];
+
+        ct[0] += K0 * B0 + (-K0) * B6;
+        ct[1] += K1 * B1 + K5 * B5 + (-K1) * B7 + K5 * B11;
+        ct[2] += K1 * B2 + (-K5) * B4 + (-K1) * B8 + (-K5) * B10;
+
+        F[0] += (-(T00 * ct[0] + T10 * ct[1] + T20 * ct[2])) * mask;
+        F[1] += (-(T01 * ct[0] + T11 * ct[1] + T21 * ct[2])) * mask;
+        F[2] += (-(T02 * ct[0] + T22 * ct[2])) * mask;
+
+        R B3 = T00 * s[3] + T01 * s[4] + T02 * s[5];
+        R B9 = T00 * o[3] + T01 * o[4] + T02 * o[5];
+
+        ct[3] += K2 * B3 + (-K2) * B9;
+        ct[4] += (-K5) * B2 + K3 * B4 + K5 * B8 + K4 * B10;
+        ct[5] += K5 * B1 + K3 * B5 + (-K5) * B7 + K4 * B11;
+    } else {
+        R D0 = T00 * d[0] + T01 * d[1] + T02 * d[2];   // B0 - B6
+        R D1 = T10 * d[0] + T11 * d[1];                 // B1 - B7
+        R D2 = T20 * d[0] + T21 * d[1] + T22 * d[2];   // B2 - B8
+        R D3 = T00 * d[3] + T01 * d[4] + T02 * d[5];   // B3 - B9
+
+        ct[0] += K0 * D0;
+        ct[1] += K1 * D1 + K5 * (B5 + B11);
+        ct[2] += K1 * D2 + (-K5) * (B4 + B10);
+
+        F[0] += (-(T00 * ct[0] + T10 * ct[1] + T20 * ct[2])) * mask;
+        F[1] += (-(T01 * ct[0] + T11 * ct[1] + T21 * ct[2])) * mask;
+        F[2] += (-(T02 * ct[0] + T22 * ct[2])) * mask;
+
+        ct[3] += K2 * D3;
+        ct[4] += (-K5) * D2 + K3 * B4 + K4 * B10;
+        ct[5] += K5 * D1 + K3 * B5 + K4 * B11;
+    }
 
     F[3] += (-(T00 * ct[3] + T10 * ct[4] + T20 * ct[5])) * mask;
     F[4] += (-(T01 * ct[3] + T11 * ct[4] + T21 * ct[5])) * mask;
@@ -236,13 +270,8 @@ __device__ __forceinline__ bool link_breaks(const ForceArgs<R, CT, FO> &a, int32
 template <class R, class CT, class FO>
 __device__ __forceinline__ void force_particle(const ForceArgs<R, CT, FO> &a, int64_t p, R F[6]) {
     const double xCord = a.x[p], yCord = a.y[p], zCord = a.z[p];
-    double theta[3] = {0, 0, 0};
-    if (a.breakCheck) {
-        theta[0] = a.tx[p];
-        theta[1] = a.ty[p];
-        theta[2] = a.tz[p];
-    }
-    R self[7] = {(R)a.d0[p], (R)a.d1[p], (R)a.d2[p], (R)a.d3[p], (R)a.d4[p], (R)a.d5[p], (R)a.radius[p]};
+    const double sd[6] = {a.d0[p], a.d1[p], a.d2[p], a.d3[p], a.d4[p], a.d5[p]};
+    R self[7] = {(R)sd[0], (R)sd[1], (R)sd[2], (R)sd[3], (R)sd[4], (R)sd[5], (R)a.radius[p]};
 #pragma unroll
     for (int c = 0; c < 6; c++) F[c] = 0;
     const int64_t lane = p & 63;
@@ -251,12 +280,15 @@ __device__ __forceinline__ void force_particle(const ForceArgs<R, CT, FO> &a, in
         const int32_t *nl = a.nbr + (p >> 6) * (int64_t)64 * a.nbrWidth + lane;
         for (int32_t k = 0; k < cnt; k++) {
             const int32_t q = nl[(int64_t)k * 64];
-            R other[7] = {(R)a.d0[q], (R)a.d1[q], (R)a.d2[q], (R)a.d3[q], (R)a.d4[q], (R)a.d5[q], (R)a.radius[q]};
+            const double od[6] = {a.d0[q], a.d1[q], a.d2[q], a.d3[q], a.d4[q], a.d5[q]};
+            R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], (R)a.radius[q]};
+            R d[6];
+            diff_delta<R>(sd, od, d);
             R xDist = (R)(xCord - a.x[q]);
             R yDist = (R)(yCord - a.y[q]);
             R zDist = (R)(zCord - a.z[q]);
-            if (contact_is_inert<R>(xDist, yDist, zDist, self, other, a.fp)) continue;
-            contact_law<R>(R(1), xDist, yDist, zDist, self, other, a.fp, F);
+            if (contact_is_inert<R>(xDist, yDist, zDist, self, other, d, a.fp)) continue;
+            contact_law<R>(R(1), xDist, yDist, zDist, self, other, d, a.fp, F);
         }
     }
     {
@@ -265,23 +297,17 @@ __device__ __forceinline__ void force_particle(const ForceArgs<R, CT, FO> &a, in
         for (int32_t k = 0; k < cnt; k++) {
             const int64_t sidx = base + (int64_t)k * 64;
             const int32_t q = a.slotOther[sidx];
-            R other[7] = {(R)a.d0[q], (R)a.d1[q], (R)a.d2[q], (R)a.d3[q], (R)a.d4[q], (R)a.d5[q], (R)a.radius[q]};
-            const double dx = xCord - a.x[q], dy = yCord - a.y[q], dz = zCord - a.z[q];
-            R xDist = (R)dx;
-            R yDist = (R)dy;
-            R zDist = (R)dz;
-            const int32_t word = a.slotLink[sidx];
+            const double od[6] = {a.d0[q], a.d1[q], a.d2[q], a.d3[q], a.d4[q], a.d5[q]};
+            R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], R(0)};
+            R d[6];
+            diff_delta<R>(sd, od, d);
+            R xDist = (R)(xCord - a.x[q]);
+            R yDist = (R)(yCord - a.y[q]);
+            R zDist = (R)(zCord - a.z[q]);
             R mask = 1;
-            if (word < 0) {
-                mask = 0;  // broken link (state == 0)
-            } else if (a.breakCheck && link_breaks(a, word, q, dx, dy, dz, theta)) {
-                // both ends reach the same verdict; each flags its own slot
-                mask = 0;
-                a.slotLinkW[sidx] = word | SLOT_BROKEN;
-                a.state[word] = 0;
-            }
+            if (a.slotLink[sidx] < 0) mask = 0;  // broken link (state == 0)
             R ct[6] = {(R)a.c0[sidx], (R)a.c1[sidx], (R)a.c2[sidx], (R)a.c3[sidx], (R)a.c4[sidx], (R)a.c5[sidx]};
-            beam_law<R>(mask, xDist, yDist, zDist, self, other, a.fp, ct, F);
+            beam_law<R>(mask, xDist, yDist, zDist, self, other, d, a.fp, ct, F);
             a.c0[sidx] = (CT)ct[0];
             a.c1[sidx] = (CT)ct[1];
             a.c2[sidx] = (CT)ct[2];
@@ -505,9 +531,9 @@ __global__ void __launch_bounds__(TPB) k_integrate(IntegArgs a, const FO *__rest
 template <class R>
 struct StepRecs {
     const double4 *rec;        // current state
-    const R *drec;
+    const double *drec;
     double4 *recOut;           // integrated state
-    R *drecOut;
+    double *drecOut;
     double tame;               // Diff bound behind the radius flag of rec
     const float2 *slotBrk;     // per side (initialLen, maxLoad) in float
     int32_t prefilter;         // 0: every intact side takes the exact break test
@@ -523,21 +549,11 @@ __device__ __forceinline__ float rotation_bound(const double t[3]) {
     return (float)(fabs(t[0]) + fabs(t[1]) + fabs(t[2]));
 }
 
-template <class R>
-__device__ __forceinline__ void load_drec(const R *drec, int64_t q, R o[6], float &rot);
-template <>
-__device__ __forceinline__ void load_drec<double>(const double *drec, int64_t q, double o[6], float &rot) {
+__device__ __forceinline__ void load_drec(const double *drec, int64_t q, double o[6], float &rot) {
     const double4 *v = reinterpret_cast<const double4 *>(drec + q * 8);
     const double4 a = v[0], b = v[1];
     o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; o[4] = b.x; o[5] = b.y;
     rot = (float)b.z;
-}
-template <>
-__device__ __forceinline__ void load_drec<float>(const float *drec, int64_t q, float o[6], float &rot) {
-    const float4 *v = reinterpret_cast<const float4 *>(drec + q * 8);
-    const float4 a = v[0], b = v[1];
-    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; o[4] = b.x; o[5] = b.y;
-    rot = b.z;
 }
 
 template <class R>
@@ -547,11 +563,9 @@ __device__ __forceinline__ void store_records(const StepRecs<R> &r, int64_t p, c
 #pragma unroll
     for (int c = 0; c < 6; c++) tame &= fabs(diff[c]) <= r.tame;
     r.recOut[p] = make_double4(cord[0], cord[1], cord[2], tame ? radius : __builtin_nan(""));
-    R *d = r.drecOut + p * 8;
-#pragma unroll
-    for (int c = 0; c < 6; c++) d[c] = (R)diff[c];
-    d[6] = (R)rotation_bound(theta);
-    d[7] = 0;
+    double4 *d = reinterpret_cast<double4 *>(r.drecOut + p * 8);
+    d[0] = make_double4(diff[0], diff[1], diff[2], diff[3]);
+    d[1] = make_double4(diff[4], diff[5], (double)rotation_bound(theta), 0.0);
 }
 
 // Break prefilter: true when the link certainly stays intact, i.e. the float
@@ -624,11 +638,13 @@ __global__ void __launch_bounds__(TPB, sizeof(R) == 4 ? 4 : 1) k_step(ForceArgs<
                         particleDistance2 > R(0))
                         continue;
                 }
-                R other[7];
+                double od[6];
                 float rotOther;
-                load_drec<R>(r.drec, q, other, rotOther);
-                other[6] = (R)f.radius[q];
-                contact_law<R>(R(1), xDist, yDist, zDist, self, other, f.fp, F);
+                load_drec(r.drec, q, od, rotOther);
+                R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], (R)f.radius[q]};
+                R d[6];
+                diff_delta<R>(diff, od, d);
+                contact_law<R>(R(1), xDist, yDist, zDist, self, other, d, f.fp, F);
             }
         }
         {
@@ -639,10 +655,12 @@ __global__ void __launch_bounds__(TPB, sizeof(R) == 4 ? 4 : 1) k_step(ForceArgs<
                 const int32_t q = f.slotOther[sidx];
                 const double4 Q = r.rec[q];
                 const double dx = xCord - Q.x, dy = yCord - Q.y, dz = zCord - Q.z;
-                R other[7];
+                double od[6];
                 float rotOther;
-                load_drec<R>(r.drec, q, other, rotOther);
-                other[6] = 0;  // beam_law reads no radius
+                load_drec(r.drec, q, od, rotOther);
+                R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], R(0)};  // no radius needed
+                R d[6];
+                diff_delta<R>(diff, od, d);
                 const int32_t word = f.slotLink[sidx];
                 R mask = 1;
                 if (word < 0) {
@@ -659,7 +677,7 @@ __global__ void __launch_bounds__(TPB, sizeof(R) == 4 ? 4 : 1) k_step(ForceArgs<
                     }
                 }
                 R ct[6] = {(R)f.c0[sidx], (R)f.c1[sidx], (R)f.c2[sidx], (R)f.c3[sidx], (R)f.c4[sidx], (R)f.c5[sidx]};
-                beam_law<R>(mask, (R)dx, (R)dy, (R)dz, self, other, f.fp, ct, F);
+                beam_law<R>(mask, (R)dx, (R)dy, (R)dz, self, other, d, f.fp, ct, F);
                 f.c0[sidx] = (CT)ct[0];
                 f.c1[sidx] = (CT)ct[1];
                 f.c2[sidx] = (CT)ct[2];
@@ -938,9 +956,9 @@ template <class R>
 StepRecs<R> step_recs(Context &c, double tame, const hdm_break_params *brk) {
     StepRecs<R> r;
     r.rec = c.rec.as<double4>();
-    r.drec = c.drec.as<R>();
+    r.drec = c.drec.as<double>();
     r.recOut = c.recAlt.as<double4>();
-    r.drecOut = c.drecAlt.as<R>();
+    r.drecOut = c.drecAlt.as<double>();
     r.tame = tame;
     r.slotBrk = c.slotBrk.as<float2>();
     r.perLinkMaxLoad = c.hasMaxLoad ? 1 : 0;
@@ -958,7 +976,7 @@ hipError_t ensure_records(Context &c) {
         if (e != hipSuccess) return e;
     }
     for (DevBuf *b : {&c.drec, &c.drecAlt}) {
-        hipError_t e = b->ensure(8 * c.forceBytes() * n);
+        hipError_t e = b->ensure(64 * n);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -987,7 +1005,7 @@ hipError_t launch_pack_records(Context &c, double tame) {
     } else {
         StepRecs<float> r = step_recs<float>(c, tame, nullptr);
         r.recOut = c.rec.as<double4>();
-        r.drecOut = c.drec.as<float>();
+        r.drecOut = c.drec.as<double>();
         go(r);
     }
     e = hipGetLastError();
