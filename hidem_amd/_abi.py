@@ -26,6 +26,9 @@ HDM_INTEG_GLACIER = 1
 HDM_REBUILD_HEURISTIC = 0
 HDM_REBUILD_EXACT = 1
 HDM_REBUILD_INTERVAL = 2
+HDM_REBUILD_LEARNED = 3
+SCHED_FEATURES = 16
+SCHED_HIDDEN = 64
 HDM_ORDER_LATTICE = 0
 HDM_ORDER_MORTON = 1
 HDM_PROBE_COPY = 0
@@ -36,6 +39,7 @@ _dp = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
 _u8p = C.POINTER(C.c_uint8)
+_f32p = C.POINTER(C.c_float)
 
 
 class HdmParticlesHost(C.Structure):
@@ -101,6 +105,33 @@ class HdmSetupInfo(C.Structure):
                 ("endTimestep", C.c_int64), ("precision", C.c_int32)]
 
 
+class HdmMlpWeights(C.Structure):
+    _fields_ = [("W1", C.c_float * (16 * 64)), ("b1", C.c_float * 64), ("W2", C.c_float * (64 * 64)),
+                ("b2", C.c_float * 64), ("w3", C.c_float * 64), ("b3", C.c_float),
+                ("featMean", C.c_float * 16), ("featStd", C.c_float * 16)]
+
+    N_PARAMS = 16 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+
+    def to_numpy(self):
+        """Flat float32 copy in struct order (parameters first, then standardisation)."""
+        return np.frombuffer(bytes(self), dtype=np.float32).copy()
+
+    @classmethod
+    def from_numpy(cls, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        assert flat.size * 4 == C.sizeof(cls)
+        return cls.from_buffer_copy(flat.tobytes())
+
+
+class HdmSchedParams(C.Structure):
+    _fields_ = [("blockCells", C.c_int32), ("minBatch", C.c_int32), ("maxBatch", C.c_int32), ("safety", C.c_float)]
+
+
+class HdmMlpTrainParams(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float)]
+
+
 _lib = None
 
 
@@ -159,6 +190,16 @@ def _declare(L):
         "hdm_sim_info": ([P, _i64p, _i64p, _i64p, _i64p], C.c_int),
         "hdm_sim_steps": ([P, C.c_int64, _u8p, C.POINTER(HdmRunStats)], C.c_int),
         "hdm_sim_config_dump": ([P], C.c_char_p),
+        "hdm_sched_configure": ([P, C.POINTER(HdmSchedParams)], C.c_int),
+        "hdm_sched_set_model": ([P, C.POINTER(HdmMlpWeights)], C.c_int),
+        "hdm_sched_features": ([P, C.POINTER(HdmRunParams), C.c_int64, C.c_int32, _i32p, _f32p, _f32p, _u8p],
+                               C.c_int),
+        "hdm_sched_predict": ([P, C.c_int32, _f32p, _dp], C.c_int),
+        "hdm_mlp_forward": ([P, C.POINTER(HdmMlpWeights), _f32p, C.c_int64, _f32p], C.c_int),
+        "hdm_mlp_gradient": ([P, C.POINTER(HdmMlpWeights), _f32p, _f32p, _u8p, C.c_int64, _f32p, _dp], C.c_int),
+        "hdm_mlp_train": ([P, C.POINTER(HdmMlpWeights), _f32p, _f32p, _u8p, C.c_int64,
+                           C.POINTER(HdmMlpTrainParams), _dp], C.c_int),
+        "hdm_sched_stats": ([P, _i64p, _i64p, _i64p], C.c_int),
         "hdm_gen_fcc_count": ([C.POINTER(HdmFccSpec), _i64p], C.c_int),
         "hdm_gen_fcc_fill": ([C.POINTER(HdmFccSpec), C.POINTER(HdmParticlesHost)], C.c_int),
         "hdm_gen_links_count": ([C.POINTER(HdmParticlesHost), C.c_double, _i64p], C.c_int),
@@ -426,6 +467,75 @@ class Context:
         b = C.c_int64()
         self._c(lib().hdm_count_broken(self._h, C.byref(b)), "hdm_count_broken")
         return b.value
+
+    # ---- learned rebuild scheduler
+    def sched_configure(self, block_cells=4, min_batch=8, max_batch=256, safety=0.5):
+        sp = HdmSchedParams(block_cells, min_batch, max_batch, safety)
+        self._c(lib().hdm_sched_configure(self._h, C.byref(sp)), "hdm_sched_configure")
+        self._block_cells = block_cells
+
+    def sched_set_model(self, w):
+        self._c(lib().hdm_sched_set_model(self._h, C.byref(w) if w is not None else None), "hdm_sched_set_model")
+
+    def sched_blocks(self):
+        """Number of scheduler blocks over the current grid (hdm_sched_features' nBlocks)."""
+        g = self.grid_info()
+        bc = getattr(self, "_block_cells", 4)
+        return (-(-g.X // bc)) * (-(-g.Y // bc)) * (-(-g.Z // bc))
+
+    def sched_features(self, rp, steps_since_rebuild):
+        """Features [nBlocks][16], per-block max |x - x0| and validity of the current state."""
+        n = self.sched_blocks()
+        f = np.zeros((n, SCHED_FEATURES), np.float32)
+        d = np.zeros(n, np.float32)
+        v = np.zeros(n, np.uint8)
+        nb = C.c_int32()
+        self._c(lib().hdm_sched_features(self._h, C.byref(rp), steps_since_rebuild, n, C.byref(nb),
+                                         _ptr(f, C.c_float), _ptr(d, C.c_float), _ptr(v, C.c_uint8)),
+                "hdm_sched_features")
+        assert nb.value == n
+        return f, d, v
+
+    def sched_predict(self, nblocks):
+        y = np.zeros(nblocks, np.float32)
+        m = C.c_double()
+        self._c(lib().hdm_sched_predict(self._h, nblocks, _ptr(y, C.c_float), C.byref(m)), "hdm_sched_predict")
+        return y, m.value
+
+    def mlp_forward(self, w, X):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.zeros(X.shape[0], np.float32)
+        self._c(lib().hdm_mlp_forward(self._h, C.byref(w), _ptr(X, C.c_float), X.shape[0], _ptr(y, C.c_float)),
+                "hdm_mlp_forward")
+        return y
+
+    def mlp_gradient(self, w, X, target, censored=None):
+        X = np.ascontiguousarray(X, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        cen = None if censored is None else np.ascontiguousarray(censored, np.uint8)
+        g = np.zeros(HdmMlpWeights.N_PARAMS, np.float32)
+        loss = C.c_double()
+        self._c(lib().hdm_mlp_gradient(self._h, C.byref(w), _ptr(X, C.c_float), _ptr(target, C.c_float),
+                                       None if cen is None else _ptr(cen, C.c_uint8), X.shape[0],
+                                       _ptr(g, C.c_float), C.byref(loss)), "hdm_mlp_gradient")
+        return g, loss.value
+
+    def mlp_train(self, w, X, target, censored=None, epochs=10, batch=4096, lr=1e-3, beta1=0.9, beta2=0.999,
+                  eps=1e-8):
+        X = np.ascontiguousarray(X, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        cen = None if censored is None else np.ascontiguousarray(censored, np.uint8)
+        tp = HdmMlpTrainParams(epochs, batch, lr, beta1, beta2, eps)
+        loss = C.c_double()
+        self._c(lib().hdm_mlp_train(self._h, C.byref(w), _ptr(X, C.c_float), _ptr(target, C.c_float),
+                                    None if cen is None else _ptr(cen, C.c_uint8), X.shape[0], C.byref(tp),
+                                    C.byref(loss)), "hdm_mlp_train")
+        return loss.value
+
+    def sched_stats(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self._c(lib().hdm_sched_stats(self._h, C.byref(a), C.byref(b), C.byref(c)), "hdm_sched_stats")
+        return {"host_checks": a.value, "steps_launched": b.value, "steps_executed": c.value}
 
     def sync(self):
         self._c(lib().hdm_sync(self._h), "hdm_sync")

@@ -688,15 +688,221 @@ int hdm_set_timestep(hdm_ctx *ctx, int64_t ts) {
 // of each step also reduces the rebuild-test quantities of the new state
 // (max+ Diff and max |x - x0|^2 into red[32..36]), so the next step's
 // decision costs one 40-byte read instead of a separate pass.
+}  // extern "C"
+
+namespace {
+
+// gridService rebuild (HiDEMLib.h:589-627): grid, neighbour list, snapshot
+// for the exact test; the scheduler's blocks follow the new grid.
+int rebuild(hdm_ctx *ctx, const hdm_run_params *rp) {
+    Context &c = ctx->c;
+    c.accumulatedDist = 0;
+    int r = hdm_grid_build(ctx, rp->searchDistance);
+    if (r == HDM_OK) r = hdm_neighbours_build(ctx, rp->interactDist);
+    if (r == HDM_OK) r = hdm_skin_snapshot(ctx);
+    if (r != HDM_OK) return r;
+    c.stepsSinceRebuild = 0;
+    c.schedLastEval = 0;
+    c.schedAssigned = false;
+    return HDM_OK;
+}
+
+hdm::SchedGeometry sched_geometry(const Context &c) {
+    hdm::SchedGeometry g;
+    const int32_t bc = c.sched.blockCells > 0 ? c.sched.blockCells : 4;
+    g.ox = c.grid.xMin;
+    g.oy = c.grid.yMin;
+    g.oz = c.grid.zMin;
+    g.edge = c.grid.spacing * bc;
+    g.BX = (c.grid.X + bc - 1) / bc;
+    g.BY = (c.grid.Y + bc - 1) / bc;
+    g.BZ = (c.grid.Z + bc - 1) / bc;
+    return g;
+}
+
+// Features of the current state (assigning blocks first when the grid changed).
+int sched_eval_features(hdm_ctx *ctx, const hdm_run_params *rp, int64_t stepsSince, int32_t *nBlocks) {
+    Context &c = ctx->c;
+    if (!c.grid.built) return fail(ctx, HDM_E_STATE, "scheduler features need a grid build");
+    if (!c.snapshotValid) return fail(ctx, HDM_E_STATE, "scheduler features need a position snapshot");
+    hdm::SchedGeometry g = sched_geometry(c);
+    if (!c.schedAssigned) {
+        HIP_CHECK(ctx, hdm::sched_assign_blocks(c, g));
+        c.schedAssigned = true;
+        c.schedLastEval = 0;
+    }
+    hdm::SchedFeatureInputs in;
+    in.budget = 0.5 * (rp->interactDist - rp->force.attractDistance);
+    in.waterLevel = rp->integ.waterLevel;
+    in.stepsSince = stepsSince;
+    in.stepsSincePrev = stepsSince - c.schedLastEval;
+    HIP_CHECK(ctx, hdm::sched_features(c, g, in));
+    c.schedLastEval = stepsSince;
+    c.schedNb = g.BX * g.BY * g.BZ;
+    if (nBlocks) *nBlocks = c.schedNb;
+    return HDM_OK;
+}
+
+int sched_use_model(hdm_ctx *ctx) {
+    Context &c = ctx->c;
+    if (!c.mlpHoldsModel) {
+        HIP_CHECK(ctx, hdm::mlp_upload(c, c.schedModel));
+        c.mlpHoldsModel = true;
+    }
+    return HDM_OK;
+}
+
+// Weights for a standalone network call (no longer the scheduler's).
+int mlp_use_weights(hdm_ctx *ctx, const hdm_mlp_weights *w) {
+    Context &c = ctx->c;
+    for (int k = 0; k < HDM_SCHED_FEATURES; k++)
+        if (!(w->featStd[k] > 0.0f) || !std::isfinite(w->featMean[k]))
+            return fail(ctx, HDM_E_INVALID, "hdm_mlp_weights: featStd must be > 0 and featMean finite");
+    HIP_CHECK(ctx, hdm::mlp_upload(c, *w));
+    c.mlpHoldsModel = false;
+    return HDM_OK;
+}
+
+template <class T>
+hipError_t h2d_buf(hdm::DevBuf &b, const T *src, int64_t count, hipStream_t s) {
+    hipError_t e = b.ensure(sizeof(T) * (size_t)(count > 0 ? count : 1));
+    if (e != hipSuccess || count == 0 || src == nullptr) return e;
+    return hipMemcpyAsync(b.ptr, src, sizeof(T) * (size_t)count, hipMemcpyHostToDevice, s);
+}
+
+// Steps to enqueue before the next host check.
+int sched_horizon(hdm_ctx *ctx, const hdm_run_params *rp, int64_t *H) {
+    Context &c = ctx->c;
+    const int64_t lo = c.sched.minBatch > 0 ? c.sched.minBatch : 1;
+    const int64_t hi = c.sched.maxBatch >= lo ? c.sched.maxBatch : lo;
+    if (!c.haveModel) {
+        *H = hi;
+        return HDM_OK;
+    }
+    int32_t nb = 0;
+    int r = sched_eval_features(ctx, rp, c.stepsSinceRebuild, &nb);
+    if (r == HDM_OK) r = sched_use_model(ctx);
+    if (r != HDM_OK) return r;
+    HIP_CHECK(ctx, hdm::mlp_forward(c, c.schedFeat.as<float>(), nb, c.schedPred.as<float>()));
+    uint64_t *out = c.red.as<uint64_t>() + 44;
+    HIP_CHECK(ctx, hdm::sched_min_prediction(c, nb, out));
+    uint64_t k = 0;
+    HIP_CHECK(ctx, d2h(&k, out, sizeof(k), c.stream));
+    const double m = hdm::ordered_value(k);
+    double h = (double)c.sched.safety * std::exp2(m < 30.0 ? m : 30.0);
+    if (!(h >= (double)lo)) h = (double)lo;   // also NaN
+    if (h > (double)hi) h = (double)hi;
+    *H = (int64_t)h;
+    return HDM_OK;
+}
+
+// HDM_REBUILD_LEARNED: the exact test runs on the device in every step's last
+// workgroup (StepGuard); the host enqueues scheduler-sized batches and reads
+// the guard once per batch.  A step launched after the test tripped does
+// nothing, so the state advances exactly as under HDM_REBUILD_EXACT.
+int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats *stats) {
+    Context &c = ctx->c;
+    uint64_t *skin = c.red.as<uint64_t>() + 32;
+    const double thr = rp->interactDist - rp->force.attractDistance;
+    hdm::StepGuard guard{thr};
+    auto t0 = std::chrono::steady_clock::now();
+    int64_t rebuilds = 0;
+    if (rebuildLog) std::memset(rebuildLog, 0, (size_t)nSteps);
+    // initial decision as HDM_REBUILD_EXACT takes it
+    bool regen = !c.neighBuilt || !c.snapshotValid;
+    if (!regen) {
+        double d = 0;
+        if (c.skinFused && c.skinFusedDisp) {
+            uint64_t k = 0;
+            HIP_CHECK(ctx, d2h(&k, skin + 4, sizeof(k), c.stream));
+            d = std::sqrt(hdm::ordered_value(k));
+        } else {
+            int r = hdm_skin_exact(ctx, &d);
+            if (r != HDM_OK) return r;
+        }
+        regen = 2.0 * d >= thr;
+    }
+    bool breakPending = false;
+    int64_t step = 0;
+    while (step < nSteps) {
+        if (regen) {
+            int r = rebuild(ctx, rp);
+            if (r != HDM_OK) return r;
+            rebuilds++;
+            if (rebuildLog) rebuildLog[step] = 1;
+        }
+        int64_t H = 0;
+        int r = sched_horizon(ctx, rp, &H);
+        if (r != HDM_OK) return r;
+        if (H > nSteps - step) H = nSteps - step;
+        // guard flag and executed-step count start at 0 (a fill kernel, capture-safe)
+        HIP_CHECK(ctx, hdm::launch_guard_reset(c));
+        for (int64_t i = 0; i < H; i++) {
+            Timer t(c, HDM_T_STEP);
+            HIP_CHECK(ctx, hdm::launch_step(c, rp->force, rp->integ, skin, true, breakPending ? &rp->brk : nullptr,
+                                            step + i == nSteps - 1, &guard));
+            breakPending = (double)(c.ts + i) * rp->dt > rp->breakTime;
+        }
+        uint32_t words[hdm::GUARD_WORDS] = {0, 0, 0, 0};
+        HIP_CHECK(ctx, d2h(words, c.guardState.ptr, sizeof(words), c.stream));
+        const int64_t k = words[hdm::GUARD_STEPS];
+        regen = words[hdm::GUARD_FLAG] != 0;
+        if ((H - k) & 1) {  // launch_step swapped once per launch; undo the skipped ones
+            hdm::DevBuf *cur[12] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz,
+                                    &c.diff[0], &c.diff[1], &c.diff[2], &c.diff[3], &c.diff[4], &c.diff[5]};
+            for (int q = 0; q < 12; q++) hdm::swap_buf(*cur[q], c.alt[q]);
+            hdm::swap_buf(c.rec, c.recAlt);
+            hdm::swap_buf(c.drec, c.drecAlt);
+        }
+        if (k > 0) {
+            breakPending = (double)(c.ts + k - 1) * rp->dt > rp->breakTime;
+            c.altHoldsPrev = true;
+        }
+        c.ts += k;
+        c.stepsSinceRebuild += k;
+        c.schedChecks++;
+        c.schedLaunched += H;
+        c.schedExecuted += k;
+        step += k;
+        c.skinFused = true;
+        c.skinFusedDisp = true;
+        if (k == 0 && !regen) return fail(ctx, HDM_E_STATE, "guarded batch executed no step");
+    }
+    if (breakPending) {
+        int r = hdm_break(ctx, &rp->brk, nullptr);
+        if (r != HDM_OK) return r;
+    }
+    HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
+    auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        stats->timestep = c.ts;
+        stats->steps = nSteps;
+        stats->rebuilds = rebuilds;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+        int64_t b = 0;
+        int r = hdm_count_broken(ctx, &b);
+        if (r != HDM_OK) return r;
+        stats->brokenTotal = b;
+    }
+    return HDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats *stats) {
     if (!valid(ctx) || !rp || nSteps < 0) return HDM_E_INVALID;
     Context &c = ctx->c;
     if (rp->integ.variant != HDM_INTEG_ICEBLOCK && rp->integ.variant != HDM_INTEG_GLACIER)
         return fail(ctx, HDM_E_INVALID, "unknown integrator variant");
+    if (rp->policy < HDM_REBUILD_HEURISTIC || rp->policy > HDM_REBUILD_LEARNED)
+        return fail(ctx, HDM_E_INVALID, "unknown rebuild policy");
     if (!c.linksBuilt) {
         int r = hdm_links_build(ctx);
         if (r != HDM_OK) return r;
     }
+    if (rp->policy == HDM_REBUILD_LEARNED) return run_learned(ctx, rp, nSteps, rebuildLog, stats);
     uint64_t *skin = c.red.as<uint64_t>() + 32;
     const bool wantSkin = rp->policy != HDM_REBUILD_INTERVAL;
     const bool wantDisp = rp->policy == HDM_REBUILD_EXACT;
@@ -739,10 +945,7 @@ int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *reb
         }
         if (rebuildLog) rebuildLog[step] = regen ? 1 : 0;
         if (regen) {
-            c.accumulatedDist = 0;
-            int r = hdm_grid_build(ctx, rp->searchDistance);
-            if (r == HDM_OK) r = hdm_neighbours_build(ctx, rp->interactDist);
-            if (r == HDM_OK) r = hdm_skin_snapshot(ctx);
+            int r = rebuild(ctx, rp);
             if (r != HDM_OK) return r;
             rebuilds++;
         }
@@ -753,8 +956,10 @@ int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *reb
         }
         c.skinFused = wantSkin;
         c.skinFusedDisp = wantDisp;
+        c.altHoldsPrev = true;
         breakPending = (double)c.ts * rp->dt > rp->breakTime;
         c.ts++;
+        c.stepsSinceRebuild++;
     }
     if (breakPending) {
         int r = hdm_break(ctx, &rp->brk, nullptr);
@@ -772,6 +977,166 @@ int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *reb
         if (r != HDM_OK) return r;
         stats->brokenTotal = b;
     }
+    return HDM_OK;
+}
+
+
+// ---- learned rebuild scheduler ---------------------------------------------------
+int hdm_sched_configure(hdm_ctx *ctx, const hdm_sched_params *sp) {
+    if (!valid(ctx) || !sp) return HDM_E_INVALID;
+    if (sp->blockCells < 1 || sp->minBatch < 1 || sp->maxBatch < sp->minBatch || !(sp->safety > 0.0f))
+        return fail(ctx, HDM_E_INVALID, "hdm_sched_params: need blockCells >= 1, 1 <= minBatch <= maxBatch, safety > 0");
+    Context &c = ctx->c;
+    if (sp->blockCells != c.sched.blockCells) c.schedAssigned = false;
+    c.sched = *sp;
+    return HDM_OK;
+}
+
+int hdm_sched_set_model(hdm_ctx *ctx, const hdm_mlp_weights *w) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!w) {
+        c.haveModel = false;
+        return HDM_OK;
+    }
+    int r = mlp_use_weights(ctx, w);
+    if (r != HDM_OK) return r;
+    c.schedModel = *w;
+    c.mlpHoldsModel = true;
+    c.haveModel = true;
+    return HDM_OK;
+}
+
+int hdm_sched_features(hdm_ctx *ctx, const hdm_run_params *rp, int64_t stepsSinceRebuild, int32_t capacity,
+                       int32_t *nBlocks, float *features, float *blockDisp, uint8_t *validOut) {
+    if (!valid(ctx) || !rp || stepsSinceRebuild < 0) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!c.linksBuilt) return fail(ctx, HDM_E_STATE, "link lists not built");
+    if (c.grid.built) {
+        const hdm::SchedGeometry g = sched_geometry(c);
+        const int64_t need = (int64_t)g.BX * g.BY * g.BZ;
+        if ((features || blockDisp || validOut) && need > capacity)
+            return fail(ctx, HDM_E_CAPACITY, "hdm_sched_features: " + std::to_string(need) + " blocks exceed capacity");
+    }
+    int32_t nb = 0;
+    int r = sched_eval_features(ctx, rp, stepsSinceRebuild, &nb);
+    if (r != HDM_OK) return r;
+    if (nBlocks) *nBlocks = nb;
+    if (features) HIP_CHECK(ctx, d2h(features, c.schedFeat.ptr, sizeof(float) * HDM_SCHED_FEATURES * (size_t)nb, c.stream));
+    if (blockDisp) HIP_CHECK(ctx, d2h(blockDisp, c.schedDisp.ptr, sizeof(float) * (size_t)nb, c.stream));
+    if (validOut) HIP_CHECK(ctx, d2h(validOut, c.schedValid.ptr, (size_t)nb, c.stream));
+    HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
+    return HDM_OK;
+}
+
+int hdm_sched_predict(hdm_ctx *ctx, int32_t capacity, float *pred, double *minPred) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!c.haveModel) return fail(ctx, HDM_E_STATE, "no scheduler model set");
+    if (c.schedNb == 0) return fail(ctx, HDM_E_STATE, "no features evaluated");
+    if (pred && c.schedNb > capacity) return fail(ctx, HDM_E_CAPACITY, "hdm_sched_predict: capacity below nBlocks");
+    int r = sched_use_model(ctx);
+    if (r != HDM_OK) return r;
+    HIP_CHECK(ctx, hdm::mlp_forward(c, c.schedFeat.as<float>(), c.schedNb, c.schedPred.as<float>()));
+    if (minPred) {
+        uint64_t *out = c.red.as<uint64_t>() + 44;
+        HIP_CHECK(ctx, hdm::sched_min_prediction(c, c.schedNb, out));
+        uint64_t k = 0;
+        HIP_CHECK(ctx, d2h(&k, out, sizeof(k), c.stream));
+        *minPred = hdm::ordered_value(k);
+    }
+    if (pred) HIP_CHECK(ctx, d2h(pred, c.schedPred.ptr, sizeof(float) * (size_t)c.schedNb, c.stream));
+    HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
+    return HDM_OK;
+}
+
+int hdm_mlp_forward(hdm_ctx *ctx, const hdm_mlp_weights *w, const float *X, int64_t n, float *y) {
+    if (!valid(ctx) || !w || n < 0 || (n > 0 && (!X || !y))) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    (void)hipSetDevice(c.device);
+    int r = mlp_use_weights(ctx, w);
+    if (r != HDM_OK) return r;
+    HIP_CHECK(ctx, h2d_buf(c.mlpX, X, n * HDM_SCHED_FEATURES, c.stream));
+    HIP_CHECK(ctx, c.mlpY.ensure(sizeof(float) * (size_t)(n > 0 ? n : 1)));
+    HIP_CHECK(ctx, hdm::mlp_forward(c, c.mlpX.as<float>(), n, c.mlpY.as<float>()));
+    if (n > 0) HIP_CHECK(ctx, d2h(y, c.mlpY.ptr, sizeof(float) * (size_t)n, c.stream));
+    return HDM_OK;
+}
+
+int hdm_mlp_gradient(hdm_ctx *ctx, const hdm_mlp_weights *w, const float *X, const float *target,
+                     const uint8_t *censored, int64_t n, float *grad, double *loss) {
+    if (!valid(ctx) || !w || n < 1 || !X || !target || !grad) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    (void)hipSetDevice(c.device);
+    int r = mlp_use_weights(ctx, w);
+    if (r != HDM_OK) return r;
+    const int64_t P = hdm::mlp_param_count(), tiles = (n + 31) / 32;
+    HIP_CHECK(ctx, h2d_buf(c.mlpX, X, n * HDM_SCHED_FEATURES, c.stream));
+    HIP_CHECK(ctx, h2d_buf(c.mlpT, target, n, c.stream));
+    HIP_CHECK(ctx, h2d_buf(c.mlpC, censored, censored ? n : 0, c.stream));
+    HIP_CHECK(ctx, c.mlpGrad.ensure(sizeof(float) * (size_t)P));
+    HIP_CHECK(ctx, c.mlpLoss.ensure(sizeof(float) * (size_t)tiles));
+    HIP_CHECK(ctx, hdm::mlp_gradient(c, c.mlpX.as<float>(), c.mlpT.as<float>(), censored ? c.mlpC.as<uint8_t>() : nullptr,
+                                     n, c.mlpGrad.as<float>(), c.mlpLoss.as<float>()));
+    HIP_CHECK(ctx, d2h(grad, c.mlpGrad.ptr, sizeof(float) * (size_t)P, c.stream));
+    if (loss) {
+        std::vector<float> lp((size_t)tiles);
+        HIP_CHECK(ctx, d2h(lp.data(), c.mlpLoss.ptr, sizeof(float) * (size_t)tiles, c.stream));
+        double s = 0;
+        for (float v : lp) s += v;
+        *loss = s;
+    }
+    return HDM_OK;
+}
+
+int hdm_mlp_train(hdm_ctx *ctx, hdm_mlp_weights *w, const float *X, const float *target, const uint8_t *censored,
+                  int64_t n, const hdm_mlp_train_params *tp, double *loss) {
+    if (!valid(ctx) || !w || n < 1 || !X || !target || !tp) return HDM_E_INVALID;
+    if (tp->epochs < 0 || tp->batch < 1 || !(tp->lr > 0.0f) || !(tp->eps > 0.0f))
+        return fail(ctx, HDM_E_INVALID, "hdm_mlp_train_params: need epochs >= 0, batch >= 1, lr > 0, eps > 0");
+    Context &c = ctx->c;
+    (void)hipSetDevice(c.device);
+    int r = mlp_use_weights(ctx, w);
+    if (r != HDM_OK) return r;
+    const int64_t P = hdm::mlp_param_count();
+    HIP_CHECK(ctx, h2d_buf(c.mlpX, X, n * HDM_SCHED_FEATURES, c.stream));
+    HIP_CHECK(ctx, h2d_buf(c.mlpT, target, n, c.stream));
+    HIP_CHECK(ctx, h2d_buf(c.mlpC, censored, censored ? n : 0, c.stream));
+    HIP_CHECK(ctx, c.mlpGrad.ensure(sizeof(float) * (size_t)P));
+    HIP_CHECK(ctx, c.mlpM.ensure(sizeof(float) * (size_t)P));
+    HIP_CHECK(ctx, c.mlpV.ensure(sizeof(float) * (size_t)P));
+    HIP_CHECK(ctx, c.mlpLoss.ensure(sizeof(float) * (size_t)((n + 31) / 32)));
+    HIP_CHECK(ctx, hdm::launch_fill_u32(c.stream, c.mlpM.as<uint32_t>(), 0u, P));
+    HIP_CHECK(ctx, hdm::launch_fill_u32(c.stream, c.mlpV.as<uint32_t>(), 0u, P));
+    int64_t t = 0;
+    for (int32_t ep = 0; ep < tp->epochs; ep++)
+        for (int64_t off = 0; off < n; off += tp->batch) {
+            const int64_t rows = (n - off < tp->batch) ? n - off : tp->batch;
+            HIP_CHECK(ctx, hdm::mlp_gradient(c, c.mlpX.as<float>() + off * HDM_SCHED_FEATURES, c.mlpT.as<float>() + off,
+                                             censored ? c.mlpC.as<uint8_t>() + off : nullptr, rows,
+                                             c.mlpGrad.as<float>(), c.mlpLoss.as<float>()));
+            HIP_CHECK(ctx, hdm::mlp_adam(c, c.mlpGrad.as<float>(), c.mlpM.as<float>(), c.mlpV.as<float>(),
+                                         1.0f / (float)rows, tp->lr, tp->beta1, tp->beta2, tp->eps, ++t));
+        }
+    HIP_CHECK(ctx, d2h(w, c.mlpW.ptr, sizeof(hdm_mlp_weights), c.stream));
+    if (loss) {
+        HIP_CHECK(ctx, hdm::mlp_gradient(c, c.mlpX.as<float>(), c.mlpT.as<float>(),
+                                         censored ? c.mlpC.as<uint8_t>() : nullptr, n, c.mlpGrad.as<float>(),
+                                         c.mlpLoss.as<float>()));
+        std::vector<float> lp((size_t)((n + 31) / 32));
+        HIP_CHECK(ctx, d2h(lp.data(), c.mlpLoss.ptr, sizeof(float) * lp.size(), c.stream));
+        double s = 0;
+        for (float v : lp) s += v;
+        *loss = s / (double)n;
+    }
+    return HDM_OK;
+}
+
+int hdm_sched_stats(hdm_ctx *ctx, int64_t *hostChecks, int64_t *stepsLaunched, int64_t *stepsExecuted) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    if (hostChecks) *hostChecks = ctx->c.schedChecks;
+    if (stepsLaunched) *stepsLaunched = ctx->c.schedLaunched;
+    if (stepsExecuted) *stepsExecuted = ctx->c.schedExecuted;
     return HDM_OK;
 }
 

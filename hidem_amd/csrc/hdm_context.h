@@ -94,6 +94,7 @@ struct Context {
     //         |xTheta| + |yTheta| + |zTheta| (as a float), 0
     DevBuf rec, recAlt, drec, drecAlt;
     bool recCurrent = false;     // rec/drec hold the current SoA state
+    bool altHoldsPrev = false;   // alt[6..8] hold the Diff before the last fused step
     double recTame = 0;          // Diff bound the radius flags were made with
     // batched form: block id per particle, waterline per block (nBlocks = 0: unused)
     int32_t nBlocks = 0;
@@ -119,11 +120,27 @@ struct Context {
     // reductions / scalars
     DevBuf red;            // u64 scratch for bbox / max reductions and counters
     DevBuf skinPartials;   // f64 [workgroups][4]: rebuild-test maxima per integrating workgroup
+    DevBuf guardState;     // u32 [GUARD_WORDS]: ticket, guard flag, executed-step counter
     DevBuf x0, y0, z0;     // positions at the last neighbour-list build
     DevBuf probe;          // bandwidth-probe buffer (hdm_bw_probe)
     bool snapshotValid = false;
     bool skinFused = false;      // red[32..34] hold max+ Diff from the last integrate of hdm_run
     bool skinFusedDisp = false;  // red[36] holds max |x - x0|^2 from that integrate
+    // learned rebuild scheduler
+    hdm_sched_params sched = {4, 8, 256, 0.5f};
+    bool haveModel = false;
+    bool guardArmed = false;     // guard flag may be set (HDM_REBUILD_LEARNED left it)
+    int32_t schedBX = 0, schedBY = 0, schedBZ = 0;
+    double schedOx = 0, schedOy = 0, schedOz = 0, schedEdge = 0;
+    bool schedAssigned = false;  // block ids match the last grid build
+    int64_t stepsSinceRebuild = 0, schedLastEval = 0;
+    int64_t schedChecks = 0, schedLaunched = 0, schedExecuted = 0;
+    DevBuf schedBlockOf, schedAcc, schedFeat, schedDisp, schedValid, schedPred, schedBrokenPrev;
+    int32_t schedNb = 0;         // blocks of the last feature evaluation
+    hdm_mlp_weights schedModel;  // the scheduler's network (host copy)
+    bool mlpHoldsModel = false;  // mlpW holds schedModel (standalone MLP calls replace it)
+    DevBuf mlpW, mlpW1b, mlpW2b, mlpGpart;
+    DevBuf mlpX, mlpT, mlpC, mlpY, mlpGrad, mlpM, mlpV, mlpLoss;
     // run state
     int64_t ts = 0;
     double accumulatedDist = 0;
@@ -146,6 +163,17 @@ constexpr int32_t SLOT_LINK_MASK = 0x7fffffff;
 // Launch helpers shared by the kernel files.
 int xcd_grid(int64_t nBlocks);
 
+// Block grid of the learned scheduler.
+struct SchedGeometry {
+    double ox, oy, oz, edge;
+    int32_t BX, BY, BZ;
+};
+struct SchedFeatureInputs {
+    double budget;          // (interactDist - attractDistance) / 2
+    double waterLevel;
+    int64_t stepsSince, stepsSincePrev;
+};
+
 // Kernel launchers (defined in kernels_*.hip).  All enqueue on ctx.stream.
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, int64_t n);
 hipError_t launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t v, int64_t n);
@@ -163,20 +191,48 @@ hipError_t launch_ct_export(Context &c);   // ct (tiled) -> ctStage (checkpoint 
 hipError_t launch_side_thresholds(Context &c);  // initialLen / maxLoad -> slotBrk (after link_fill)
 hipError_t launch_pack_records(Context &c, double tame);  // SoA state -> rec / drec
 hipError_t launch_force(Context &c, const hdm_force_params &fp);
+// skin != nullptr: the rebuild-test maxima of the new state land in
+// skin[0..2] (max+ Diff per axis) and skin[4] (max |x - x0|^2, when
+// withDisplacement), as ordered keys, when the launch completes.
 hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *skin = nullptr,
                             bool withDisplacement = false);
+// Guard words in Context::guardState.
+enum { GUARD_TICKET = 0, GUARD_FLAG = 1, GUARD_STEPS = 2, GUARD_WORDS = 4 };
+
+// Exact rebuild test evaluated on the device by launch_step: the step's last
+// workgroup sets the guard flag when 2 max |x - x0| >= threshold; steps
+// launched after that do nothing until the host clears the flag.
+struct StepGuard {
+    double threshold;   // interactDist - attractDistance
+};
+
 // Force + integration of one timestep in one launch (state ping-pong; the
 // integrated state is current afterwards).  skin as for launch_integrate.
 // brk != nullptr: first apply the link break that the previous timestep's
 // launch_break would have applied to the current state.  storeForce: write
-// the force arrays (else they keep their previous contents).
+// the force arrays (else they keep their previous contents).  guard: see
+// StepGuard (requires skin and withDisplacement).
 hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_params &ip, uint64_t *skin,
-                       bool withDisplacement, const hdm_break_params *brk, bool storeForce);
+                       bool withDisplacement, const hdm_break_params *brk, bool storeForce,
+                       const StepGuard *guard = nullptr);
+// Clears the guard flag and the executed-step counter (GUARD_FLAG, GUARD_STEPS).
+hipError_t launch_guard_reset(Context &c);
 hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter);
 hipError_t launch_skin_heuristic(Context &c, uint64_t *out3);
 hipError_t launch_skin_exact(Context &c, uint64_t *out1);
 hipError_t launch_snapshot(Context &c);
 hipError_t launch_count_broken(Context &c, uint64_t *counter);
+// learned scheduler (kernels_sched.hip)
+int64_t mlp_param_count();
+hipError_t sched_assign_blocks(Context &c, const SchedGeometry &g);
+hipError_t sched_features(Context &c, const SchedGeometry &g, const SchedFeatureInputs &in);
+hipError_t sched_min_prediction(Context &c, int64_t nBlocks, uint64_t *out);
+hipError_t mlp_upload(Context &c, const hdm_mlp_weights &w);
+hipError_t mlp_forward(Context &c, const float *X, int64_t n, float *y);
+hipError_t mlp_gradient(Context &c, const float *X, const float *target, const uint8_t *censored, int64_t n,
+                        float *grad, float *lossPart);
+hipError_t mlp_adam(Context &c, const float *grad, float *m, float *v, float scale, float lr, float beta1,
+                    float beta2, float eps, int64_t t);
 hipError_t launch_probe(Context &c, int kind, void *buf, int64_t bytes);
 hipError_t launch_probe_zero(Context &c, void *buf, int64_t bytes);
 

@@ -351,8 +351,18 @@ struct IntegArgs {
     hdm_integ_params ip;
     // rebuild-test reductions of the new state (nullptr: none): one row of 4
     // doubles per workgroup, max(0, max Diff) per axis (HiDEMLib.h:424-461)
-    // and max |x - x0|^2, folded into the skin slots by k_skin_finalize.
+    // and max |x - x0|^2; the last workgroup to finish folds the rows into
+    // skin[0..2] and skin[4] (ordered keys) and re-arms ticket.
     double *partials;
+    uint32_t *ticket;
+    uint64_t *skin;
+    // exact-test guard (guard == nullptr: off): the folding workgroup sets
+    // *guard when 2 |x - x0|max >= guardThr (the neighbour list no longer
+    // covers the next step) and counts executed steps in *stepsDone; a launch
+    // that finds *guard set does nothing.
+    uint32_t *guard;
+    uint32_t *stepsDone;
+    double guardThr;
     const double *x0, *y0, *z0;   // nullptr: no displacement reduction
     const int32_t *blockId;       // batched form: waterline of the particle's block
     const double *blockWater;     // (nullptr: ip.waterLevel for every particle)
@@ -487,10 +497,13 @@ __device__ __forceinline__ void integrate_particle(const IntegArgs &a, int64_t i
 
 // Workgroup maxima of the rebuild-test quantities of the particles this
 // workgroup integrated, written as one row of a.partials (every quantity is
-// >= 0, so the maxima start from 0 and inactive lanes contribute 0).
-__device__ __forceinline__ void skin_partials(const IntegArgs &a, bool active, int64_t i, const double cord[3],
-                                              const double diff[3]) {
+// >= 0, so the maxima start from 0 and inactive lanes contribute 0).  The
+// last workgroup of the launch (atomic ticket after a release fence) reduces
+// all rows in a fixed order, stores the skin slots and evaluates the guard.
+__device__ __forceinline__ void skin_reduce(const IntegArgs &a, bool active, int64_t i, const double cord[3],
+                                            const double diff[3]) {
     __shared__ double rows[TPB / 64][4];
+    __shared__ int last;
     double m[4] = {0.0, 0.0, 0.0, 0.0};
     if (active) {
         for (int c = 0; c < 3; c++) m[c] = diff[c] > 0.0 ? diff[c] : 0.0;
@@ -510,6 +523,40 @@ __device__ __forceinline__ void skin_partials(const IntegArgs &a, bool active, i
         for (int k = 1; k < TPB / 64; k++) v = rows[k][threadIdx.x] > v ? rows[k][threadIdx.x] : v;
         a.partials[(int64_t)blockIdx.x * 4 + threadIdx.x] = v;
     }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(a.ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    // every other workgroup's row is visible (their fence precedes their ticket)
+    double f[4] = {0.0, 0.0, 0.0, 0.0};
+    const unsigned long long *rowsG = reinterpret_cast<const unsigned long long *>(a.partials);
+    for (int64_t r = threadIdx.x; r < (int64_t)gridDim.x; r += TPB)
+        for (int c = 0; c < 4; c++) {
+            double v = __longlong_as_double(
+                (long long)__hip_atomic_load(rowsG + r * 4 + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            f[c] = v > f[c] ? v : f[c];
+        }
+#pragma unroll
+    for (int c = 0; c < 4; c++) f[c] = wave_max(f[c]);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 4; c++) rows[w][c] = f[c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < 4; c++) {
+            double v = rows[0][c];
+            for (int k = 1; k < TPB / 64; k++) v = rows[k][c] > v ? rows[k][c] : v;
+            f[c] = v;
+        }
+        for (int c = 0; c < 3; c++) a.skin[c] = ordered_key(f[c]);
+        a.skin[4] = ordered_key(f[3]);
+        *a.ticket = 0;
+        if (a.guard) {
+            if (2.0 * sqrt(f[3]) >= a.guardThr) *a.guard = 1u;
+            *a.stepsDone += 1u;
+        }
+    }
 }
 
 template <class FO>
@@ -523,7 +570,7 @@ __global__ void __launch_bounds__(TPB) k_integrate(IntegArgs a, const FO *__rest
         const double F[6] = {(double)f0[i], (double)f1[i], (double)f2[i], (double)f3[i], (double)f4[i], (double)f5[i]};
         integrate_particle(a, i, F, cord, diff);
     }
-    if (a.partials) skin_partials(a, active, i, cord, diff);
+    if (a.partials) skin_reduce(a, active, i, cord, diff);
 }
 
 // ---- fused step ------------------------------------------------------------------
@@ -605,6 +652,7 @@ __device__ __forceinline__ bool certainly_intact(float2 brk, float maxLoad, floa
 // The FP32 form fits 4 waves per SIMD (<= 128 VGPRs) without spilling.
 template <class R, class CT>
 __global__ void __launch_bounds__(TPB, sizeof(R) == 4 ? 4 : 1) k_step(ForceArgs<R, CT, R> f, IntegArgs g, StepRecs<R> r) {
+    if (g.guard && *g.guard) return;  // an earlier step found the neighbour list stale
     const int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
     const bool active = p < f.n;
     double cordN[3] = {0, 0, 0}, diffN[3] = {0, 0, 0};
@@ -711,7 +759,7 @@ __global__ void __launch_bounds__(TPB, sizeof(R) == 4 ? 4 : 1) k_step(ForceArgs<
             diffN[c] = diff[c];
         }
     }
-    if (g.partials) skin_partials(g, active, p, cordN, diffN);
+    if (g.partials) skin_reduce(g, active, p, cordN, diffN);
 }
 
 // Gather records of the current SoA state (hdm_run entry).
@@ -732,24 +780,6 @@ __global__ void __launch_bounds__(TPB) k_pack_records(const double *__restrict__
     store_records(r, p, cord, theta, diff, radius[p]);
 }
 
-// Folds the per-workgroup rows into skin[0..2] (max+ Diff) and skin[4]
-// (max |x - x0|^2) as ordered keys.
-__global__ void __launch_bounds__(TPB) k_skin_finalize(const double *__restrict__ partials, int64_t rows,
-                                                        uint64_t *skin) {
-    double m[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t r = (int64_t)blockIdx.x * TPB + threadIdx.x; r < rows; r += (int64_t)gridDim.x * TPB)
-        for (int c = 0; c < 4; c++) {
-            double v = partials[r * 4 + c];
-            m[c] = v > m[c] ? v : m[c];
-        }
-#pragma unroll
-    for (int c = 0; c < 4; c++) m[c] = wave_max(m[c]);
-    if ((threadIdx.x & 63) == 0) {
-        for (int c = 0; c < 3; c++)
-            if (m[c] > 0.0) atomicMax((unsigned long long *)&skin[c], (unsigned long long)ordered_key(m[c]));
-        if (m[3] > 0.0) atomicMax((unsigned long long *)&skin[4], (unsigned long long)ordered_key(m[3]));
-    }
-}
 
 // ---- link break --------------------------------------------------------------
 struct BreakArgs {
@@ -839,7 +869,7 @@ inline int reduce_grid(const Context &c, int64_t n) {
 
 // State pointers for the integrator: in place (alt == nullptr) or from the
 // current buffers into alt[0..11] (x, y, z, xTheta.., Diff0..5).
-IntegArgs integ_args(Context &c, const hdm_integ_params &ip, bool skin, bool withDisplacement, DevBuf *alt) {
+IntegArgs integ_args(Context &c, const hdm_integ_params &ip, uint64_t *skin, bool withDisplacement, DevBuf *alt) {
     IntegArgs a;
     a.x = c.x.as<double>(); a.y = c.y.as<double>(); a.z = c.z.as<double>();
     a.tx = c.tx.as<double>(); a.ty = c.ty.as<double>(); a.tz = c.tz.as<double>();
@@ -855,6 +885,11 @@ IntegArgs integ_args(Context &c, const hdm_integ_params &ip, bool skin, bool wit
     a.n = c.n;
     a.ip = ip;
     a.partials = skin ? c.skinPartials.as<double>() : nullptr;
+    a.skin = skin;
+    a.ticket = c.guardState.as<uint32_t>() + GUARD_TICKET;
+    a.guard = nullptr;
+    a.stepsDone = nullptr;
+    a.guardThr = 0;
     a.x0 = withDisplacement ? c.x0.as<double>() : nullptr;
     a.y0 = withDisplacement ? c.y0.as<double>() : nullptr;
     a.z0 = withDisplacement ? c.z0.as<double>() : nullptr;
@@ -896,20 +931,23 @@ hipError_t force_impl(Context &c, const hdm_force_params &fp) {
     return hipGetLastError();
 }
 
-// Skin slots start at ordered_key(0.0); k_skin_finalize folds the rows of the
-// workgroups that ran (one per block of the integrating launch).
-hipError_t skin_finalize(Context &c, uint64_t *skin) {
-    hipError_t e = launch_fill_u64(c.stream, skin, ordered_key(0.0), 5);
+// Rows for the rebuild-test partials, and the guard words (zeroed once; the
+// folding workgroup re-arms the ticket).
+hipError_t ensure_partials(Context &c) {
+    hipError_t e = c.skinPartials.ensure(sizeof(double) * 4 * (size_t)blocks_for(c.n));
+    if (e != hipSuccess || c.guardState.ptr) return e;
+    e = c.guardState.ensure(sizeof(uint32_t) * GUARD_WORDS);
     if (e != hipSuccess) return e;
-    const int64_t rows = blocks_for(c.n);
-    hipLaunchKernelGGL(k_skin_finalize, dim3(reduce_grid(c, rows)), dim3(TPB), 0, c.stream,
-                       c.skinPartials.as<double>(), rows, skin);
-    return hipGetLastError();
+    return launch_fill_u32(c.stream, c.guardState.as<uint32_t>(), 0u, GUARD_WORDS);
 }
 
-hipError_t ensure_partials(Context &c) { return c.skinPartials.ensure(sizeof(double) * 4 * (size_t)blocks_for(c.n)); }
-
 }  // namespace
+
+hipError_t launch_guard_reset(Context &c) {
+    hipError_t e = ensure_partials(c);
+    if (e != hipSuccess) return e;
+    return launch_fill_u32(c.stream, c.guardState.as<uint32_t>() + GUARD_FLAG, 0u, 2);
+}
 
 hipError_t launch_force(Context &c, const hdm_force_params &fp) {
     if (c.n == 0) return hipSuccess;
@@ -923,7 +961,7 @@ hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *sk
         hipError_t e = ensure_partials(c);
         if (e != hipSuccess) return e;
     }
-    IntegArgs a = integ_args(c, ip, skin != nullptr, withDisplacement, nullptr);
+    IntegArgs a = integ_args(c, ip, skin, withDisplacement, nullptr);
     if (c.prec == HDM_PREC_FP64)
         hipLaunchKernelGGL(k_integrate<double>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, a,
                            c.force[0].as<double>(), c.force[1].as<double>(), c.force[2].as<double>(),
@@ -932,9 +970,7 @@ hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *sk
         hipLaunchKernelGGL(k_integrate<float>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, a,
                            c.force[0].as<float>(), c.force[1].as<float>(), c.force[2].as<float>(),
                            c.force[3].as<float>(), c.force[4].as<float>(), c.force[5].as<float>());
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !skin) return e;
-    return skin_finalize(c, skin);
+    return hipGetLastError();
 }
 
 namespace {
@@ -1016,7 +1052,7 @@ hipError_t launch_pack_records(Context &c, double tame) {
 }
 
 hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_params &ip, uint64_t *skin,
-                       bool withDisplacement, const hdm_break_params *brk, bool storeForce) {
+                       bool withDisplacement, const hdm_break_params *brk, bool storeForce, const StepGuard *guard) {
     if (c.n == 0) return hipSuccess;
     const size_t b = sizeof(double) * (size_t)c.n;
     for (int k = 0; k < 12; k++) {
@@ -1032,7 +1068,12 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
         hipError_t e = launch_pack_records(c, tame);
         if (e != hipSuccess) return e;
     }
-    IntegArgs g = integ_args(c, ip, skin != nullptr, withDisplacement, c.alt);
+    IntegArgs g = integ_args(c, ip, skin, withDisplacement, c.alt);
+    if (guard) {
+        g.guard = c.guardState.as<uint32_t>() + GUARD_FLAG;
+        g.stepsDone = c.guardState.as<uint32_t>() + GUARD_STEPS;
+        g.guardThr = guard->threshold;
+    }
     auto launch = [&](auto f, auto r) {
         r.storeForce = storeForce ? 1 : 0;
         if (brk && c.L > 0) {
@@ -1056,8 +1097,7 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
     for (int k = 0; k < 12; k++) swap_buf(*cur[k], c.alt[k]);
     swap_buf(c.rec, c.recAlt);
     swap_buf(c.drec, c.drecAlt);
-    if (!skin) return hipSuccess;
-    return skin_finalize(c, skin);
+    return hipSuccess;
 }
 
 hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter) {

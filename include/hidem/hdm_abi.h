@@ -125,7 +125,12 @@ typedef struct hdm_grid_info {
 enum {
     HDM_REBUILD_HEURISTIC = 0,  /* reference accumulated-Diff estimate (HiDEMLib.h:548-559) */
     HDM_REBUILD_EXACT = 1,      /* max |x - x_at_last_build| vs (interactDist - attractDistance)/2 */
-    HDM_REBUILD_INTERVAL = 2    /* every regenInterval steps ([grid] gridRegenInterval) */
+    HDM_REBUILD_INTERVAL = 2,   /* every regenInterval steps ([grid] gridRegenInterval) */
+    HDM_REBUILD_LEARNED = 3     /* exact test evaluated on the device every step (a step that finds the
+                                   list stale does nothing); the host enqueues batches of steps sized by
+                                   the learned scheduler (hdm_sched_*) without a per-step check.  Same
+                                   rebuild steps and results as HDM_REBUILD_EXACT.  No reference
+                                   counterpart: replaces the gridService check (HiDEMLib.h:544-559). */
 };
 
 typedef struct hdm_run_params {
@@ -214,6 +219,64 @@ int  hdm_skin_exact(hdm_ctx *ctx, double *maxDisp);
 int  hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats *stats);
 int  hdm_set_timestep(hdm_ctx *ctx, int64_t ts);
 int  hdm_count_broken(hdm_ctx *ctx, int64_t *broken);
+
+/* ---- learned rebuild scheduler (no reference counterpart; it schedules the
+ *      gridService rebuild check, HiDEMLib.h:544-559) -------------------------
+ * Blocks of blockCells^3 neighbour-grid cells get 16 features each; a BF16
+ * MLP 16-64-64-1 (MFMA, FP32 accumulation) predicts log2 of the steps until
+ * the block would trip the exact test.  HDM_REBUILD_LEARNED enqueues
+ * clamp(safety * 2^min_prediction, minBatch, maxBatch) steps per host check
+ * (maxBatch when no model is set). */
+#define HDM_SCHED_FEATURES 16
+#define HDM_SCHED_HIDDEN 64
+typedef struct hdm_mlp_weights {        /* FP32 master weights */
+    float W1[HDM_SCHED_FEATURES * HDM_SCHED_HIDDEN];   /* [in][out] */
+    float b1[HDM_SCHED_HIDDEN];
+    float W2[HDM_SCHED_HIDDEN * HDM_SCHED_HIDDEN];     /* [in][out] */
+    float b2[HDM_SCHED_HIDDEN];
+    float w3[HDM_SCHED_HIDDEN];
+    float b3;
+    float featMean[HDM_SCHED_FEATURES];                /* input standardisation */
+    float featStd[HDM_SCHED_FEATURES];
+} hdm_mlp_weights;
+
+typedef struct hdm_sched_params {
+    int32_t blockCells;         /* block edge in grid cells (4) */
+    int32_t minBatch, maxBatch; /* steps per host check */
+    float safety;               /* fraction of the predicted horizon enqueued */
+} hdm_sched_params;
+
+typedef struct hdm_mlp_train_params {
+    int32_t epochs, batch;      /* minibatches are consecutive slices of the (pre-shuffled) rows */
+    float lr, beta1, beta2, eps;
+} hdm_mlp_train_params;
+
+int  hdm_sched_configure(hdm_ctx *ctx, const hdm_sched_params *sp);
+int  hdm_sched_set_model(hdm_ctx *ctx, const hdm_mlp_weights *w);   /* NULL: no model */
+/* Features of the current state (blocks laid over the last grid build).
+ * Host arrays may be NULL, else hold `capacity` blocks (HDM_E_CAPACITY when
+ * nBlocks exceeds it): features [nBlocks][16], blockDisp [nBlocks] (max
+ * |x - x0| of the block, metres), valid [nBlocks]. */
+int  hdm_sched_features(hdm_ctx *ctx, const hdm_run_params *rp, int64_t stepsSinceRebuild, int32_t capacity,
+                        int32_t *nBlocks, float *features, float *blockDisp, uint8_t *valid);
+/* Predictions (log2 steps) for the features of the last feature evaluation;
+ * pred (may be NULL) holds `capacity` floats; minPred: minimum over valid
+ * blocks. */
+int  hdm_sched_predict(hdm_ctx *ctx, int32_t capacity, float *pred, double *minPred);
+/* The network on host arrays (copied to the device): X [n][16] raw features. */
+int  hdm_mlp_forward(hdm_ctx *ctx, const hdm_mlp_weights *w, const float *X, int64_t n, float *y);
+/* Sum over rows of the gradient of the loss (squared error in log2 steps;
+ * censored rows only below their target) with respect to W1, b1, W2, b2, w3,
+ * b3 (the first 5313 floats of hdm_mlp_weights' layout), and the loss sum. */
+int  hdm_mlp_gradient(hdm_ctx *ctx, const hdm_mlp_weights *w, const float *X, const float *target,
+                      const uint8_t *censored, int64_t n, float *grad, double *loss);
+/* Minibatch Adam on the device; w is updated in place.  loss: mean loss of
+ * the last epoch. */
+int  hdm_mlp_train(hdm_ctx *ctx, hdm_mlp_weights *w, const float *X, const float *target, const uint8_t *censored,
+                   int64_t n, const hdm_mlp_train_params *tp, double *loss);
+/* Host checks and skipped launches of HDM_REBUILD_LEARNED runs since the
+ * context was created. */
+int  hdm_sched_stats(hdm_ctx *ctx, int64_t *hostChecks, int64_t *stepsLaunched, int64_t *stepsExecuted);
 
 /* ---- HBM bandwidth probes (the idea of Benchmark/BenchMain.cpp:14-83):
  *      kind 0 = streaming copy (read `bytes`, write `bytes`),
