@@ -351,18 +351,11 @@ struct IntegArgs {
     hdm_integ_params ip;
     // rebuild-test reductions of the new state (nullptr: none): one row of 4
     // doubles per workgroup, max(0, max Diff) per axis (HiDEMLib.h:424-461)
-    // and max |x - x0|^2; the last workgroup to finish folds the rows into
-    // skin[0..2] and skin[4] (ordered keys) and re-arms ticket.
+    // and max |x - x0|^2, folded by k_skin_finalize
     double *partials;
-    uint32_t *ticket;
-    uint64_t *skin;
-    // exact-test guard (guard == nullptr: off): the folding workgroup sets
-    // *guard when 2 |x - x0|max >= guardThr (the neighbour list no longer
-    // covers the next step) and counts executed steps in *stepsDone; a launch
-    // that finds *guard set does nothing.
-    uint32_t *guard;
-    uint32_t *stepsDone;
-    double guardThr;
+    // exact-test guard (nullptr: off): a launch that finds *guard set does
+    // nothing (k_skin_finalize sets it when the list no longer covers a step)
+    const uint32_t *guard;
     const double *x0, *y0, *z0;   // nullptr: no displacement reduction
     const int32_t *blockId;       // batched form: waterline of the particle's block
     const double *blockWater;     // (nullptr: ip.waterLevel for every particle)
@@ -497,13 +490,11 @@ __device__ __forceinline__ void integrate_particle(const IntegArgs &a, int64_t i
 
 // Workgroup maxima of the rebuild-test quantities of the particles this
 // workgroup integrated, written as one row of a.partials (every quantity is
-// >= 0, so the maxima start from 0 and inactive lanes contribute 0).  The
-// last workgroup of the launch (atomic ticket after a release fence) reduces
-// all rows in a fixed order, stores the skin slots and evaluates the guard.
-__device__ __forceinline__ void skin_reduce(const IntegArgs &a, bool active, int64_t i, const double cord[3],
-                                            const double diff[3]) {
+// >= 0, so the maxima start from 0 and inactive lanes contribute 0); the rows
+// are folded by k_skin_finalize after the launch.
+__device__ __forceinline__ void skin_partials(const IntegArgs &a, bool active, int64_t i, const double cord[3],
+                                              const double diff[3]) {
     __shared__ double rows[TPB / 64][4];
-    __shared__ int last;
     double m[4] = {0.0, 0.0, 0.0, 0.0};
     if (active) {
         for (int c = 0; c < 3; c++) m[c] = diff[c] > 0.0 ? diff[c] : 0.0;
@@ -523,39 +514,64 @@ __device__ __forceinline__ void skin_reduce(const IntegArgs &a, bool active, int
         for (int k = 1; k < TPB / 64; k++) v = rows[k][threadIdx.x] > v ? rows[k][threadIdx.x] : v;
         a.partials[(int64_t)blockIdx.x * 4 + threadIdx.x] = v;
     }
+}
+
+struct FinalizeArgs {
+    const double *partials;
+    int64_t rows;
+    double *stage;        // [gridDim.x][4]
+    uint32_t *ticket;
+    uint64_t *skin;
+    uint32_t *guard;      // nullptr: no guard
+    uint32_t *stepsDone;
+    double guardThr;
+};
+
+// Folds the rows of the integrating launch: each workgroup reduces a slice
+// into stage, and the last one (a ticket after a release fence: one fence
+// per workgroup of this small launch) stores skin[0..2], skin[4] as ordered
+// keys, evaluates the exact-test guard and counts the executed step.  With the
+// guard set on entry (the step was skipped) it does nothing.
+__global__ void __launch_bounds__(TPB) k_skin_finalize(FinalizeArgs a) {
+    if (a.guard && *a.guard) return;
+    __shared__ double part[TPB / 64][4];
+    __shared__ int last;
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t r = (int64_t)blockIdx.x * TPB + threadIdx.x; r < a.rows; r += (int64_t)gridDim.x * TPB)
+        for (int c = 0; c < 4; c++) {
+            const double v = a.partials[r * 4 + c];
+            m[c] = v > m[c] ? v : m[c];
+        }
+#pragma unroll
+    for (int c = 0; c < 4; c++) m[c] = wave_max(m[c]);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 4; c++) part[w][c] = m[c];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double v = part[0][threadIdx.x];
+        for (int k = 1; k < TPB / 64; k++) v = part[k][threadIdx.x] > v ? part[k][threadIdx.x] : v;
+        a.stage[blockIdx.x * 4 + threadIdx.x] = v;
+    }
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(a.ticket, 1u) == gridDim.x - 1;
     __syncthreads();
-    if (!last) return;
-    // every other workgroup's row is visible (their fence precedes their ticket)
+    if (!last || threadIdx.x != 0) return;
     double f[4] = {0.0, 0.0, 0.0, 0.0};
-    const unsigned long long *rowsG = reinterpret_cast<const unsigned long long *>(a.partials);
-    for (int64_t r = threadIdx.x; r < (int64_t)gridDim.x; r += TPB)
+    const unsigned long long *st = reinterpret_cast<const unsigned long long *>(a.stage);
+    for (uint32_t b = 0; b < gridDim.x; b++)
         for (int c = 0; c < 4; c++) {
-            double v = __longlong_as_double(
-                (long long)__hip_atomic_load(rowsG + r * 4 + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            const double v = __longlong_as_double(
+                (long long)__hip_atomic_load(st + b * 4 + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
             f[c] = v > f[c] ? v : f[c];
         }
-#pragma unroll
-    for (int c = 0; c < 4; c++) f[c] = wave_max(f[c]);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0)
-        for (int c = 0; c < 4; c++) rows[w][c] = f[c];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int c = 0; c < 4; c++) {
-            double v = rows[0][c];
-            for (int k = 1; k < TPB / 64; k++) v = rows[k][c] > v ? rows[k][c] : v;
-            f[c] = v;
-        }
-        for (int c = 0; c < 3; c++) a.skin[c] = ordered_key(f[c]);
-        a.skin[4] = ordered_key(f[3]);
-        *a.ticket = 0;
-        if (a.guard) {
-            if (2.0 * sqrt(f[3]) >= a.guardThr) *a.guard = 1u;
-            *a.stepsDone += 1u;
-        }
+    for (int c = 0; c < 3; c++) a.skin[c] = ordered_key(f[c]);
+    a.skin[4] = ordered_key(f[3]);
+    *a.ticket = 0;
+    if (a.guard) {
+        if (2.0 * sqrt(f[3]) >= a.guardThr) *a.guard = 1u;
+        *a.stepsDone += 1u;
     }
 }
 
@@ -570,7 +586,7 @@ __global__ void __launch_bounds__(TPB) k_integrate(IntegArgs a, const FO *__rest
         const double F[6] = {(double)f0[i], (double)f1[i], (double)f2[i], (double)f3[i], (double)f4[i], (double)f5[i]};
         integrate_particle(a, i, F, cord, diff);
     }
-    if (a.partials) skin_reduce(a, active, i, cord, diff);
+    if (a.partials) skin_partials(a, active, i, cord, diff);
 }
 
 // ---- fused step ------------------------------------------------------------------
@@ -759,7 +775,7 @@ __global__ void __launch_bounds__(TPB, sizeof(R) == 4 ? 4 : 1) k_step(ForceArgs<
             diffN[c] = diff[c];
         }
     }
-    if (g.partials) skin_reduce(g, active, p, cordN, diffN);
+    if (g.partials) skin_partials(g, active, p, cordN, diffN);
 }
 
 // Gather records of the current SoA state (hdm_run entry).
@@ -885,11 +901,7 @@ IntegArgs integ_args(Context &c, const hdm_integ_params &ip, uint64_t *skin, boo
     a.n = c.n;
     a.ip = ip;
     a.partials = skin ? c.skinPartials.as<double>() : nullptr;
-    a.skin = skin;
-    a.ticket = c.guardState.as<uint32_t>() + GUARD_TICKET;
     a.guard = nullptr;
-    a.stepsDone = nullptr;
-    a.guardThr = 0;
     a.x0 = withDisplacement ? c.x0.as<double>() : nullptr;
     a.y0 = withDisplacement ? c.y0.as<double>() : nullptr;
     a.z0 = withDisplacement ? c.z0.as<double>() : nullptr;
@@ -931,14 +943,31 @@ hipError_t force_impl(Context &c, const hdm_force_params &fp) {
     return hipGetLastError();
 }
 
-// Rows for the rebuild-test partials, and the guard words (zeroed once; the
-// folding workgroup re-arms the ticket).
+constexpr int FINALIZE_BLOCKS = 32;
+
+// Rows for the rebuild-test partials (plus the finalize stage), and the guard
+// words (zeroed once; the folding workgroup re-arms the ticket).
 hipError_t ensure_partials(Context &c) {
-    hipError_t e = c.skinPartials.ensure(sizeof(double) * 4 * (size_t)blocks_for(c.n));
+    hipError_t e = c.skinPartials.ensure(sizeof(double) * 4 * ((size_t)blocks_for(c.n) + FINALIZE_BLOCKS));
     if (e != hipSuccess || c.guardState.ptr) return e;
     e = c.guardState.ensure(sizeof(uint32_t) * GUARD_WORDS);
     if (e != hipSuccess) return e;
     return launch_fill_u32(c.stream, c.guardState.as<uint32_t>(), 0u, GUARD_WORDS);
+}
+
+// k_skin_finalize over the rows of a launch of blocks_for(n) workgroups.
+hipError_t skin_finalize(Context &c, uint64_t *skin, const StepGuard *guard) {
+    FinalizeArgs f;
+    f.rows = blocks_for(c.n);
+    f.partials = c.skinPartials.as<double>();
+    f.stage = c.skinPartials.as<double>() + 4 * f.rows;
+    f.ticket = c.guardState.as<uint32_t>() + GUARD_TICKET;
+    f.skin = skin;
+    f.guard = guard ? c.guardState.as<uint32_t>() + GUARD_FLAG : nullptr;
+    f.stepsDone = guard ? c.guardState.as<uint32_t>() + GUARD_STEPS : nullptr;
+    f.guardThr = guard ? guard->threshold : 0.0;
+    hipLaunchKernelGGL(k_skin_finalize, dim3(FINALIZE_BLOCKS), dim3(TPB), 0, c.stream, f);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -970,7 +999,9 @@ hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *sk
         hipLaunchKernelGGL(k_integrate<float>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, a,
                            c.force[0].as<float>(), c.force[1].as<float>(), c.force[2].as<float>(),
                            c.force[3].as<float>(), c.force[4].as<float>(), c.force[5].as<float>());
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !skin) return e;
+    return skin_finalize(c, skin, nullptr);
 }
 
 namespace {
@@ -1070,9 +1101,8 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
     }
     IntegArgs g = integ_args(c, ip, skin, withDisplacement, c.alt);
     if (guard) {
+        if (!skin || !withDisplacement) return hipErrorInvalidValue;
         g.guard = c.guardState.as<uint32_t>() + GUARD_FLAG;
-        g.stepsDone = c.guardState.as<uint32_t>() + GUARD_STEPS;
-        g.guardThr = guard->threshold;
     }
     auto launch = [&](auto f, auto r) {
         r.storeForce = storeForce ? 1 : 0;
@@ -1097,7 +1127,8 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
     for (int k = 0; k < 12; k++) swap_buf(*cur[k], c.alt[k]);
     swap_buf(c.rec, c.recAlt);
     swap_buf(c.drec, c.drecAlt);
-    return hipSuccess;
+    if (!skin) return hipSuccess;
+    return skin_finalize(c, skin, guard);
 }
 
 hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter) {

@@ -127,10 +127,11 @@ def test_block_features_match_host():
     assert np.all(np.isfinite(f))
 
 
-def _weak_block(interact=0.002):
+def _weak_block(skin=0.004):
     s, p, l = params.build_inputs(params.load_config("iceblock_16k"))
     l.maxLoad[:] *= 0.25
-    s.run.interactDist = interact   # a thin skin: the exact test trips every few dozen steps
+    # a thin skin beyond the attraction range: the exact test trips every few dozen steps
+    s.run.interactDist = s.run.force.attractDistance + skin
     return s, p, l
 
 
@@ -157,7 +158,7 @@ def test_learned_policy_is_bit_identical_to_exact(prec):
     s, p, l = _weak_block()
     say("learned vs exact", prec)
     rl_e, pe, le, _ = _run(prec, s, p, l, A.HDM_REBUILD_EXACT, [400])
-    assert rl_e.sum() >= 4, rl_e.sum()
+    assert 4 <= rl_e.sum() <= 100, rl_e.sum()
     cases = [
         dict(chunks=[400], sched=dict(min_batch=64, max_batch=64)),                       # fixed batches
         dict(chunks=[150, 1, 249], sched=dict(min_batch=1, max_batch=37, safety=0.5),     # learned, odd sizes
