@@ -5,9 +5,13 @@ A "step" is one full timestep of the hot path -- rebuild check (and rebuild
 when due), contact + beam force gather, time integration, link break -- over
 all 8,000,000 particles / ~48M links resident in HBM.  Prints one JSON line.
 
-value      whole-job particle-steps/s of the headline form (--precision):
+value      whole-job particle-steps/s of the headline form, mixed+learned
+           (FP32 force arithmetic, FP64 integration and strain, learned
+           rebuild scheduler with the exact skin test on the device):
            N_particles * K / wall time of the K timed steps, bracketed by
-           device synchronisation.
+           device synchronisation.  The plain FP64 method (reference rebuild
+           heuristic) and the mixed form with a host-checked exact test run
+           beside it under "forms".
 roofline   for the dominant kernel (k_step: link break of the previous step,
            force gather and integration in one launch): algorithmic bytes per
            launch (compulsory traffic of the arrays it must touch, see
@@ -64,27 +68,51 @@ def break_bytes(n_links):
     return n_links * (4 + 4 + 1 + 8 + 8)
 
 
-def run_gpu(setup, p, l, prec, steps, warmup):
+# Forms: (name, precision, rebuild policy, uses the trained scheduler network)
+FORMS = {
+    "fp64": (A.HDM_PREC_FP64, A.HDM_REBUILD_HEURISTIC, False),      # plain FP64 method, reference gridService
+    "mixed": (A.HDM_PREC_MIXED, A.HDM_REBUILD_EXACT, False),        # precision form, exact test checked each step
+    "mixed+learned": (A.HDM_PREC_MIXED, A.HDM_REBUILD_LEARNED, True),  # precision form + learned scheduler
+}
+
+
+def sched_model(config):
+    path = os.path.join(ROOT, "hidem_amd", "models", f"sched_{config}.bin")
+    if not os.path.exists(path):
+        return None, None
+    from hidem_amd import sched_train
+    return sched_train.load_model(path), os.path.relpath(path, ROOT)
+
+
+def run_gpu(setup, p, l, prec, policy, model, steps, warmup):
+    rp = A.HdmRunParams.from_buffer_copy(setup.run)
+    rp.policy = policy
     with A.Context(prec) as ctx:
         ctx.upload(p, l)
         ctx.links_build()
-        st = ctx.run(setup.run, warmup) if warmup > 0 else None
-        n_neigh, _ = 0, None
+        ctx.sched_configure(block_cells=4, min_batch=8, max_batch=256, safety=0.5)
+        if model is not None:
+            ctx.sched_set_model(model)
+        if warmup > 0:
+            ctx.run(rp, warmup)
         ctx.sync()
+        s0 = ctx.sched_stats()
         ctx.set_timing(True)
         ctx.reset_kernel_times()
         t0 = time.perf_counter()
-        st = ctx.run(setup.run, steps)
+        st = ctx.run(rp, steps)
         ctx.sync()
         t1 = time.perf_counter()
         kt = ctx.kernel_times()
         ctx.set_timing(False)
+        s1 = ctx.sched_stats()
         counts, _, _ = ctx.neighbours_download()
         n_neigh = int(counts.sum())
         broken = ctx.count_broken()
         dev_bytes = ctx.device_bytes()
-    wall = t1 - t0
-    return dict(wall=wall, stats=st, kernels=kt, n_neigh=n_neigh, broken=broken, dev_bytes=dev_bytes)
+    sched = {k: s1[k] - s0[k] for k in s1}
+    return dict(wall=t1 - t0, stats=st, kernels=kt, n_neigh=n_neigh, broken=broken, dev_bytes=dev_bytes,
+                sched=sched)
 
 
 def cpu_baseline(setup, p, l, steps, fast=True):
@@ -144,30 +172,32 @@ def main():
     setup, p, l = params.build_inputs(cfg, args.cells)
     log(f"generated {p.n} particles, {l.n} links in {time.perf_counter() - t0:.1f}s")
 
-    forms = {"fp64": [A.HDM_PREC_FP64], "mixed": [A.HDM_PREC_MIXED],
-             "both": [A.HDM_PREC_FP64, A.HDM_PREC_MIXED]}[args.precision]
+    names = {"fp64": ["fp64"], "mixed": ["mixed", "mixed+learned"],
+             "both": ["fp64", "mixed", "mixed+learned"]}[args.precision]
     if args.probe_traffic:
-        probe_traffic(setup, p, l, forms)
+        probe_traffic(setup, p, l, sorted({FORMS[n][0] for n in names}))
         return
+    model, model_path = sched_model(args.config)
     results = {}
-    for prec in forms:
-        name = "fp64" if prec == A.HDM_PREC_FP64 else "mixed"
+    for name in names:
+        prec, policy, learned = FORMS[name]
         log(f"running {name}: warmup {args.warmup}, timed {args.steps}")
-        r = run_gpu(setup, p, l, prec, args.steps, args.warmup)
+        r = run_gpu(setup, p, l, prec, policy, model if learned else None, args.steps, args.warmup)
         r["value"] = p.n * args.steps / r["wall"]
         sms, scnt = r["kernels"]["step"]
-        r["step_avg_ms"] = sms / max(scnt, 1)
+        executed = r["sched"]["steps_executed"] if policy == A.HDM_REBUILD_LEARNED else scnt
+        r["step_avg_ms"] = sms / max(executed, 1)     # skipped (guarded) launches cost ~µs and are not steps
         r["step_bytes"] = step_bytes(p.n, r["n_neigh"], l.n, prec)
         results[name] = r
         log(f"  {name}: {r['value']:.4e} p-steps/s, {1e3 * r['wall'] / args.steps:.3f} ms/step, "
             f"step kernel {r['step_avg_ms']:.3f} ms ({r['step_bytes'] / r['step_avg_ms'] / 1e6:.0f} GB/s), "
-            f"rebuilds {r['stats'].rebuilds}, broken {r['broken']}")
+            f"rebuilds {r['stats'].rebuilds}, broken {r['broken']}, sched {r['sched']}")
 
-    head = "mixed" if "mixed" in results else "fp64"
+    head = next(n for n in ("mixed+learned", "mixed", "fp64") if n in results)
     h = results[head]
     achieved = h["step_bytes"] / (h["step_avg_ms"] * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"step_{head}_{args.config}_traffic.json")
+    tf = os.path.join(ROOT, "profiles", f"step_{'fp64' if head == 'fp64' else 'mixed'}_{args.config}_traffic.json")
     if os.path.exists(tf) and args.cells is None:
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -198,10 +228,13 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{args.config}: FCC glacier slab, {p.n} particles, {l.n} links, "
-                        f"{'plain FP64' if head == 'fp64' else 'mixed FP32 force / FP64 integration'}, "
-                        f"exact skin rebuild test",
-            "particles": p.n, "links": l.n, "precision": head,
+                        + {"fp64": "plain FP64 method, reference rebuild heuristic",
+                           "mixed": "FP32 force / FP64 integration, exact skin test checked every step",
+                           "mixed+learned": "FP32 force / FP64 integration, learned rebuild scheduler "
+                                            "with the exact skin test on the device"}[head],
+            "particles": p.n, "links": l.n, "form": head,
             "rebuilds_in_timed_steps": int(h["stats"].rebuilds),
+            "scheduler_model": model_path if head == "mixed+learned" else None,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -212,11 +245,12 @@ def main():
                          "step_avg_ms": r["step_avg_ms"],
                          "step_GBps": r["step_bytes"] / (r["step_avg_ms"] * 1e-3) / 1e9,
                          "rebuilds": int(r["stats"].rebuilds), "broken_links": r["broken"],
-                         "device_bytes": r["dev_bytes"], "kernels": kern[name]}
+                         "device_bytes": r["dev_bytes"], "kernels": kern[name], "scheduler": r["sched"]}
                   for name, r in results.items()},
     }
-    if "fp64" in results and "mixed" in results:
-        out["mixed_over_fp64"] = results["mixed"]["value"] / results["fp64"]["value"]
+    for name in ("mixed", "mixed+learned"):
+        if "fp64" in results and name in results:
+            out[f"{name}_over_fp64"] = results[name]["value"] / results["fp64"]["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -123,6 +123,15 @@ class HdmMlpWeights(C.Structure):
         return cls.from_buffer_copy(flat.tobytes())
 
 
+class HdmExecParams(C.Structure):
+    _fields_ = [("autotune", C.c_int32), ("graphs", C.c_int32), ("variant", C.c_int32)]
+
+
+class HdmExecInfo(C.Structure):
+    _fields_ = [("variant", C.c_int32), ("variants", C.c_int32), ("tuned", C.c_int32), ("name", C.c_char * 32),
+                ("tuneMs", C.c_double * 8), ("graphCaptures", C.c_int64), ("graphReplays", C.c_int64)]
+
+
 class HdmSchedParams(C.Structure):
     _fields_ = [("blockCells", C.c_int32), ("minBatch", C.c_int32), ("maxBatch", C.c_int32), ("safety", C.c_float)]
 
@@ -200,6 +209,8 @@ def _declare(L):
         "hdm_mlp_train": ([P, C.POINTER(HdmMlpWeights), _f32p, _f32p, _u8p, C.c_int64,
                            C.POINTER(HdmMlpTrainParams), _dp], C.c_int),
         "hdm_sched_stats": ([P, _i64p, _i64p, _i64p], C.c_int),
+        "hdm_set_exec": ([P, C.POINTER(HdmExecParams)], C.c_int),
+        "hdm_exec_info_get": ([P, C.POINTER(HdmExecInfo)], C.c_int),
         "hdm_gen_fcc_count": ([C.POINTER(HdmFccSpec), _i64p], C.c_int),
         "hdm_gen_fcc_fill": ([C.POINTER(HdmFccSpec), C.POINTER(HdmParticlesHost)], C.c_int),
         "hdm_gen_links_count": ([C.POINTER(HdmParticlesHost), C.c_double, _i64p], C.c_int),
@@ -531,6 +542,17 @@ class Context:
                                     None if cen is None else _ptr(cen, C.c_uint8), X.shape[0], C.byref(tp),
                                     C.byref(loss)), "hdm_mlp_train")
         return loss.value
+
+    def set_exec(self, autotune=False, graphs=False, variant=-1):
+        ep = HdmExecParams(int(autotune), int(graphs), variant)
+        self._c(lib().hdm_set_exec(self._h, C.byref(ep)), "hdm_set_exec")
+
+    def exec_info(self):
+        i = HdmExecInfo()
+        self._c(lib().hdm_exec_info_get(self._h, C.byref(i)), "hdm_exec_info_get")
+        return {"variant": i.variant, "name": i.name.decode(), "tuned": bool(i.tuned),
+                "tune_ms": [i.tuneMs[v] for v in range(i.variants)], "graph_captures": i.graphCaptures,
+                "graph_replays": i.graphReplays}
 
     def sched_stats(self):
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()

@@ -43,13 +43,13 @@ struct Timer {
     int cls;
     hipEvent_t start = nullptr, stop = nullptr;
     Timer(Context &cc, int k) : c(cc), cls(k) {
-        if (!c.timing) return;
+        if (!c.timing || c.capturing) return;
         start = take();
         stop = take();
         if (start && stop) (void)hipEventRecord(start, c.stream);
     }
     ~Timer() {
-        if (!c.timing || !start || !stop) return;
+        if (!c.timing || c.capturing || !start || !stop) return;
         (void)hipEventRecord(stop, c.stream);
         c.pending.push_back(hdm::TimedLaunch{cls, start, stop});
     }
@@ -173,6 +173,7 @@ int hdm_ctx_destroy(hdm_ctx *ctx) {
         (void)hipEventDestroy(t.stop);
     }
     for (auto e : c.eventPool) (void)hipEventDestroy(e);
+    if (c.graphExec) (void)hipGraphExecDestroy(c.graphExec);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     delete ctx;
     return HDM_OK;
@@ -796,6 +797,84 @@ int sched_horizon(hdm_ctx *ctx, const hdm_run_params *rp, int64_t *H) {
     return HDM_OK;
 }
 
+// Everything a captured step pair bakes into its kernel arguments.
+std::vector<unsigned char> graph_key(const Context &c, const hdm_run_params *rp, const uint64_t *skin) {
+    std::vector<unsigned char> k;
+    auto put = [&](const void *p, size_t n) {
+        const unsigned char *b = static_cast<const unsigned char *>(p);
+        k.insert(k.end(), b, b + n);
+    };
+    const hdm::DevBuf *bufs[] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz, &c.radius, &c.type, &c.rec, &c.recAlt,
+                                 &c.drec, &c.drecAlt, &c.nbrCount, &c.nbr, &c.linkCount, &c.slotLink, &c.slotOther,
+                                 &c.state, &c.initialLen, &c.maxLoad, &c.slotBrk, &c.skinPartials, &c.guardState,
+                                 &c.x0, &c.y0, &c.z0, &c.blockId, &c.blockWater};
+    for (auto *b : bufs) put(&b->ptr, sizeof(void *));
+    for (int q = 0; q < 6; q++) {
+        put(&c.diff[q].ptr, sizeof(void *));
+        put(&c.force[q].ptr, sizeof(void *));
+        put(&c.ct[q].ptr, sizeof(void *));
+    }
+    for (int q = 0; q < 12; q++) put(&c.alt[q].ptr, sizeof(void *));
+    put(&c.n, sizeof(c.n));
+    put(&c.L, sizeof(c.L));
+    put(&c.maxNeigh, sizeof(c.maxNeigh));
+    put(&c.maxLinks, sizeof(c.maxLinks));
+    put(&c.nBlocks, sizeof(c.nBlocks));
+    put(&c.stepVariant, sizeof(c.stepVariant));
+    put(&c.recTame, sizeof(c.recTame));
+    put(&c.hasMaxLoad, sizeof(c.hasMaxLoad));
+    put(&skin, sizeof(skin));
+    put(&rp->force, sizeof(rp->force));
+    put(&rp->integ, sizeof(rp->integ));
+    put(&rp->brk, sizeof(rp->brk));
+    put(&rp->interactDist, sizeof(rp->interactDist));
+    return k;
+}
+
+// Two fused steps (each with the previous step's break, no force store)
+// replayed from a captured hipGraph; captured again whenever anything baked
+// into the arguments changed (a rebuild that moved the neighbour list, new
+// parameters, a new launch shape).  The pair swaps the state buffers twice,
+// so the host's view of the current state is unchanged by a replay.
+int graph_pair(hdm_ctx *ctx, const hdm_run_params *rp, uint64_t *skin, const hdm::StepGuard *guard) {
+    Context &c = ctx->c;
+    std::vector<unsigned char> key = graph_key(c, rp, skin);
+    if (!c.graphExec || key != c.graphKey) {
+        if (c.graphExec) (void)hipGraphExecDestroy(c.graphExec);
+        c.graphExec = nullptr;
+        hipGraph_t graph = nullptr;
+        HIP_CHECK(ctx, hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        c.capturing = true;
+        hipError_t e1 = hdm::launch_step(c, rp->force, rp->integ, skin, true, &rp->brk, false, guard);
+        hipError_t e2 = e1 == hipSuccess ? hdm::launch_step(c, rp->force, rp->integ, skin, true, &rp->brk, false, guard)
+                                         : e1;
+        if (e1 == hipSuccess && e2 != hipSuccess) {  // keep the host's buffer view consistent with the device
+            hdm::DevBuf *cur[12] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz,
+                                    &c.diff[0], &c.diff[1], &c.diff[2], &c.diff[3], &c.diff[4], &c.diff[5]};
+            for (int q = 0; q < 12; q++) hdm::swap_buf(*cur[q], c.alt[q]);
+            hdm::swap_buf(c.rec, c.recAlt);
+            hdm::swap_buf(c.drec, c.drecAlt);
+        }
+        hipError_t e3 = hipStreamEndCapture(c.stream, &graph);
+        c.capturing = false;
+        if (e2 != hipSuccess || e3 != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return hip_fail(ctx, e2 != hipSuccess ? e2 : e3, "step pair capture");
+        }
+        hipGraphExec_t exec = nullptr;
+        hipError_t e4 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e4 != hipSuccess) return hip_fail(ctx, e4, "hipGraphInstantiate");
+        c.graphExec = exec;
+        c.graphKey = std::move(key);
+        c.graphCaptures++;
+    }
+    Timer t(c, HDM_T_STEP);
+    HIP_CHECK(ctx, hipGraphLaunch(c.graphExec, c.stream));
+    c.graphReplays++;
+    return HDM_OK;
+}
+
 // HDM_REBUILD_LEARNED: the exact test runs on the device in every step's last
 // workgroup (StepGuard); the host enqueues scheduler-sized batches and reads
 // the guard once per batch.  A step launched after the test tripped does
@@ -834,18 +913,73 @@ int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t 
         int64_t H = 0;
         int r = sched_horizon(ctx, rp, &H);
         if (r != HDM_OK) return r;
+        if (c.autotune && !c.tuned) H = 3 * hdm::STEP_VARIANTS;   // a tuning batch: every shape three times
         if (H > nSteps - step) H = nSteps - step;
         // guard flag and executed-step count start at 0 (a fill kernel, capture-safe)
         HIP_CHECK(ctx, hdm::launch_guard_reset(c));
-        for (int64_t i = 0; i < H; i++) {
-            Timer t(c, HDM_T_STEP);
-            HIP_CHECK(ctx, hdm::launch_step(c, rp->force, rp->integ, skin, true, breakPending ? &rp->brk : nullptr,
-                                            step + i == nSteps - 1, &guard));
+        // autotuning launches every shape in turn (same results) and times it
+        const bool tuning = c.autotune && !c.tuned;
+        std::vector<hipEvent_t> tev;
+        std::vector<int> tvar;
+        int64_t i = 0;
+        while (i < H) {
+            const int64_t s = step + i;
+            const bool pairOk = !tuning && c.useGraphs && i + 1 < H && breakPending && s + 1 < nSteps - 1 &&
+                                (double)(c.ts + i) * rp->dt > rp->breakTime && c.recCurrent;
+            if (pairOk) {
+                int r2 = graph_pair(ctx, rp, skin, &guard);
+                if (r2 != HDM_OK) return r2;
+                i += 2;
+                breakPending = (double)(c.ts + i - 1) * rp->dt > rp->breakTime;
+                continue;
+            }
+            if (tuning) {
+                c.stepVariant = (int32_t)(tvar.size() % hdm::STEP_VARIANTS);
+                hipEvent_t a = nullptr, b = nullptr;
+                HIP_CHECK(ctx, hipEventCreate(&a));
+                HIP_CHECK(ctx, hipEventCreate(&b));
+                HIP_CHECK(ctx, hipEventRecord(a, c.stream));
+                HIP_CHECK(ctx, hdm::launch_step(c, rp->force, rp->integ, skin, true, breakPending ? &rp->brk : nullptr,
+                                                s == nSteps - 1, &guard));
+                HIP_CHECK(ctx, hipEventRecord(b, c.stream));
+                tev.push_back(a);
+                tev.push_back(b);
+                tvar.push_back(c.stepVariant);
+            } else {
+                Timer t(c, HDM_T_STEP);
+                HIP_CHECK(ctx, hdm::launch_step(c, rp->force, rp->integ, skin, true, breakPending ? &rp->brk : nullptr,
+                                                s == nSteps - 1, &guard));
+            }
             breakPending = (double)(c.ts + i) * rp->dt > rp->breakTime;
+            i++;
         }
         uint32_t words[hdm::GUARD_WORDS] = {0, 0, 0, 0};
         HIP_CHECK(ctx, d2h(words, c.guardState.ptr, sizeof(words), c.stream));
         const int64_t k = words[hdm::GUARD_STEPS];
+        if (tuning) {
+            // only launches that executed (the first k) time a real step
+            for (size_t q = 0; q < tvar.size(); q++) {
+                float ms = 0;
+                if ((int64_t)q < k && hipEventElapsedTime(&ms, tev[2 * q], tev[2 * q + 1]) == hipSuccess && q > 0) {
+                    const int v = tvar[q];
+                    if (c.tuneCount[v] == 0 || ms < c.tuneMs[v]) c.tuneMs[v] = ms;   // best of the samples
+                    c.tuneCount[v]++;
+                }
+                (void)hipEventDestroy(tev[2 * q]);
+                (void)hipEventDestroy(tev[2 * q + 1]);
+            }
+            bool done = true;
+            for (int v = 0; v < hdm::STEP_VARIANTS; v++) done &= c.tuneCount[v] >= 2;
+            if (done) {
+                int best = 0;
+                for (int v = 1; v < hdm::STEP_VARIANTS; v++)
+                    if (c.tuneMs[v] < c.tuneMs[best]) best = v;
+                c.stepVariant = best;
+                c.tuned = true;
+            } else {
+                c.stepVariant = 0;
+            }
+        }
         regen = words[hdm::GUARD_FLAG] != 0;
         if ((H - k) & 1) {  // launch_step swapped once per launch; undo the skipped ones
             hdm::DevBuf *cur[12] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz,
@@ -1137,6 +1271,42 @@ int hdm_sched_stats(hdm_ctx *ctx, int64_t *hostChecks, int64_t *stepsLaunched, i
     if (hostChecks) *hostChecks = ctx->c.schedChecks;
     if (stepsLaunched) *stepsLaunched = ctx->c.schedLaunched;
     if (stepsExecuted) *stepsExecuted = ctx->c.schedExecuted;
+    return HDM_OK;
+}
+
+
+// ---- execution options -----------------------------------------------------------
+int hdm_set_exec(hdm_ctx *ctx, const hdm_exec_params *ep) {
+    if (!valid(ctx) || !ep) return HDM_E_INVALID;
+    if (ep->variant < -1 || ep->variant >= hdm::STEP_VARIANTS)
+        return fail(ctx, HDM_E_INVALID, "hdm_exec_params.variant out of range");
+    Context &c = ctx->c;
+    c.useGraphs = ep->graphs != 0;
+    if (ep->autotune) {
+        c.autotune = true;
+        c.tuned = false;
+        for (int v = 0; v < 8; v++) {
+            c.tuneMs[v] = 0;
+            c.tuneCount[v] = 0;
+        }
+    } else {
+        c.autotune = false;
+    }
+    if (ep->variant >= 0) c.stepVariant = ep->variant;
+    return HDM_OK;
+}
+
+int hdm_exec_info_get(hdm_ctx *ctx, hdm_exec_info *info) {
+    if (!valid(ctx) || !info) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    std::memset(info, 0, sizeof(*info));
+    info->variant = c.stepVariant;
+    info->variants = hdm::STEP_VARIANTS;
+    info->tuned = c.tuned ? 1 : 0;
+    std::snprintf(info->name, sizeof(info->name), "%s", hdm::step_variant_name(c.stepVariant));
+    for (int v = 0; v < hdm::STEP_VARIANTS && v < 8; v++) info->tuneMs[v] = c.tuneMs[v];
+    info->graphCaptures = c.graphCaptures;
+    info->graphReplays = c.graphReplays;
     return HDM_OK;
 }
 

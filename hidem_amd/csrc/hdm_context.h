@@ -126,6 +126,14 @@ struct Context {
     bool snapshotValid = false;
     bool skinFused = false;      // red[32..34] hold max+ Diff from the last integrate of hdm_run
     bool skinFusedDisp = false;  // red[36] holds max |x - x0|^2 from that integrate
+    // execution: launch shape of the fused step (autotuned) and graph replay
+    int32_t stepVariant = 0;     // index into the step shapes (kernels_step.hip)
+    bool autotune = false, tuned = false, useGraphs = false, capturing = false;
+    double tuneMs[8] = {0};
+    int64_t tuneCount[8] = {0};
+    hipGraphExec_t graphExec = nullptr;
+    std::vector<unsigned char> graphKey;
+    int64_t graphCaptures = 0, graphReplays = 0;
     // learned rebuild scheduler
     hdm_sched_params sched = {4, 8, 256, 0.5f};
     bool haveModel = false;
@@ -196,6 +204,10 @@ hipError_t launch_force(Context &c, const hdm_force_params &fp);
 // withDisplacement), as ordered keys, when the launch completes.
 hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *skin = nullptr,
                             bool withDisplacement = false);
+// Launch shapes of the fused step (autotuner candidates).
+constexpr int STEP_VARIANTS = 4;
+const char *step_variant_name(int v);
+
 // Guard words in Context::guardState.
 enum { GUARD_TICKET = 0, GUARD_FLAG = 1, GUARD_STEPS = 2, GUARD_WORDS = 4 };
 

@@ -492,9 +492,10 @@ __device__ __forceinline__ void integrate_particle(const IntegArgs &a, int64_t i
 // workgroup integrated, written as one row of a.partials (every quantity is
 // >= 0, so the maxima start from 0 and inactive lanes contribute 0); the rows
 // are folded by k_skin_finalize after the launch.
+template <int BLOCK = TPB>
 __device__ __forceinline__ void skin_partials(const IntegArgs &a, bool active, int64_t i, const double cord[3],
                                               const double diff[3]) {
-    __shared__ double rows[TPB / 64][4];
+    __shared__ double rows[BLOCK / 64][4];
     double m[4] = {0.0, 0.0, 0.0, 0.0};
     if (active) {
         for (int c = 0; c < 3; c++) m[c] = diff[c] > 0.0 ? diff[c] : 0.0;
@@ -511,7 +512,7 @@ __device__ __forceinline__ void skin_partials(const IntegArgs &a, bool active, i
     __syncthreads();
     if (threadIdx.x < 4) {
         double v = rows[0][threadIdx.x];
-        for (int k = 1; k < TPB / 64; k++) v = rows[k][threadIdx.x] > v ? rows[k][threadIdx.x] : v;
+        for (int k = 1; k < BLOCK / 64; k++) v = rows[k][threadIdx.x] > v ? rows[k][threadIdx.x] : v;
         a.partials[(int64_t)blockIdx.x * 4 + threadIdx.x] = v;
     }
 }
@@ -666,10 +667,14 @@ __device__ __forceinline__ bool certainly_intact(float2 brk, float maxLoad, floa
 // break test of an intact side runs exactly (FP64, as k_break) unless the
 // float prefilter proves the link stays intact.
 // The FP32 form fits 4 waves per SIMD (<= 128 VGPRs) without spilling.
-template <class R, class CT>
-__global__ void __launch_bounds__(TPB, sizeof(R) == 4 ? 4 : 1) k_step(ForceArgs<R, CT, R> f, IntegArgs g, StepRecs<R> r) {
+// BLOCK (workgroup size) and XCD (the per-XCD contiguous renumbering of
+// workgroups) are the autotuned launch shape; every shape computes the same
+// per-particle arithmetic, so results do not depend on it.
+template <class R, class CT, int BLOCK, bool XCD>
+__global__ void __launch_bounds__(BLOCK, sizeof(R) == 4 ? 4 : 1) k_step(ForceArgs<R, CT, R> f, IntegArgs g, StepRecs<R> r) {
     if (g.guard && *g.guard) return;  // an earlier step found the neighbour list stale
-    const int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    const int64_t wg = XCD ? xcd_block_index(gridDim.x) : (int64_t)blockIdx.x;
+    const int64_t p = wg * BLOCK + threadIdx.x;
     const bool active = p < f.n;
     double cordN[3] = {0, 0, 0}, diffN[3] = {0, 0, 0};
     if (active) {
@@ -775,7 +780,7 @@ __global__ void __launch_bounds__(TPB, sizeof(R) == 4 ? 4 : 1) k_step(ForceArgs<
             diffN[c] = diff[c];
         }
     }
-    if (g.partials) skin_partials(g, active, p, cordN, diffN);
+    if (g.partials) skin_partials<BLOCK>(g, active, p, cordN, diffN);
 }
 
 // Gather records of the current SoA state (hdm_run entry).
@@ -945,20 +950,32 @@ hipError_t force_impl(Context &c, const hdm_force_params &fp) {
 
 constexpr int FINALIZE_BLOCKS = 32;
 
+// Launch shapes of the fused step the autotuner chooses from.
+struct StepShape {
+    int block;
+    bool xcd;
+    const char *name;
+};
+constexpr StepShape kShapes[STEP_VARIANTS] = {
+    {256, true, "wg256-xcd"}, {128, true, "wg128-xcd"}, {512, true, "wg512-xcd"}, {256, false, "wg256-linear"}};
+constexpr int kMinBlock = 128;
+
+int64_t rows_for(int64_t n, int block) { return (n + block - 1) / block; }
+
 // Rows for the rebuild-test partials (plus the finalize stage), and the guard
 // words (zeroed once; the folding workgroup re-arms the ticket).
 hipError_t ensure_partials(Context &c) {
-    hipError_t e = c.skinPartials.ensure(sizeof(double) * 4 * ((size_t)blocks_for(c.n) + FINALIZE_BLOCKS));
+    hipError_t e = c.skinPartials.ensure(sizeof(double) * 4 * ((size_t)rows_for(c.n, kMinBlock) + FINALIZE_BLOCKS));
     if (e != hipSuccess || c.guardState.ptr) return e;
     e = c.guardState.ensure(sizeof(uint32_t) * GUARD_WORDS);
     if (e != hipSuccess) return e;
     return launch_fill_u32(c.stream, c.guardState.as<uint32_t>(), 0u, GUARD_WORDS);
 }
 
-// k_skin_finalize over the rows of a launch of blocks_for(n) workgroups.
-hipError_t skin_finalize(Context &c, uint64_t *skin, const StepGuard *guard) {
+// k_skin_finalize over the rows of a launch of rows_for(n, block) workgroups.
+hipError_t skin_finalize(Context &c, uint64_t *skin, const StepGuard *guard, int block = TPB) {
     FinalizeArgs f;
-    f.rows = blocks_for(c.n);
+    f.rows = rows_for(c.n, block);
     f.partials = c.skinPartials.as<double>();
     f.stage = c.skinPartials.as<double>() + 4 * f.rows;
     f.ticket = c.guardState.as<uint32_t>() + GUARD_TICKET;
@@ -1104,6 +1121,8 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
         if (!skin || !withDisplacement) return hipErrorInvalidValue;
         g.guard = c.guardState.as<uint32_t>() + GUARD_FLAG;
     }
+    const int v = (c.stepVariant >= 0 && c.stepVariant < STEP_VARIANTS) ? c.stepVariant : 0;
+    const StepShape shape = kShapes[v];
     auto launch = [&](auto f, auto r) {
         r.storeForce = storeForce ? 1 : 0;
         if (brk && c.L > 0) {
@@ -1112,10 +1131,15 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
             f.scalarMaxLoad = brk->maxLoad;
         }
         using A = decltype(f);
-        if constexpr (std::is_same<A, ForceArgs<double, double, double>>::value)
-            hipLaunchKernelGGL((k_step<double, double>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, f, g, r);
-        else
-            hipLaunchKernelGGL((k_step<float, float>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, f, g, r);
+        using R = typename std::conditional<std::is_same<A, ForceArgs<double, double, double>>::value, double,
+                                            float>::type;
+        const dim3 grid((unsigned)rows_for(c.n, shape.block));
+        switch (v) {
+        case 0: hipLaunchKernelGGL((k_step<R, R, 256, true>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        case 1: hipLaunchKernelGGL((k_step<R, R, 128, true>), grid, dim3(128), 0, c.stream, f, g, r); break;
+        case 2: hipLaunchKernelGGL((k_step<R, R, 512, true>), grid, dim3(512), 0, c.stream, f, g, r); break;
+        default: hipLaunchKernelGGL((k_step<R, R, 256, false>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        }
     };
     if (c.prec == HDM_PREC_FP64) launch(force_args<double, double, double>(c, fp), step_recs<double>(c, tame, brk));
     else launch(force_args<float, float, float>(c, fp), step_recs<float>(c, tame, brk));
@@ -1128,8 +1152,10 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
     swap_buf(c.rec, c.recAlt);
     swap_buf(c.drec, c.drecAlt);
     if (!skin) return hipSuccess;
-    return skin_finalize(c, skin, guard);
+    return skin_finalize(c, skin, guard, shape.block);
 }
+
+const char *step_variant_name(int v) { return (v >= 0 && v < STEP_VARIANTS) ? kShapes[v].name : "?"; }
 
 hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter) {
     if (c.L == 0) return hipSuccess;

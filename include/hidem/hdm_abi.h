@@ -278,6 +278,29 @@ int  hdm_mlp_train(hdm_ctx *ctx, hdm_mlp_weights *w, const float *X, const float
  * context was created. */
 int  hdm_sched_stats(hdm_ctx *ctx, int64_t *hostChecks, int64_t *stepsLaunched, int64_t *stepsExecuted);
 
+/* ---- execution options of HDM_REBUILD_LEARNED runs (no reference
+ *      counterpart; the reference tunes launch constants statically in
+ *      Tune.h) ---------------------------------------------------------------
+ * autotune: the next run's first steps launch every shape of the fused step
+ * (workgroup size, XCD-contiguous workgroup numbering) in turn -- all shapes
+ * compute identical results -- and keep the fastest.  graphs: plain middle
+ * steps run as replays of a captured two-step hipGraph. */
+typedef struct hdm_exec_params {
+    int32_t autotune;
+    int32_t graphs;
+    int32_t variant;            /* shape to use (0..variants-1); -1 keeps the current one */
+} hdm_exec_params;
+
+typedef struct hdm_exec_info {
+    int32_t variant, variants, tuned;
+    char name[32];
+    double tuneMs[8];           /* best step time per shape seen by the autotuner (ms) */
+    int64_t graphCaptures, graphReplays;
+} hdm_exec_info;
+
+int  hdm_set_exec(hdm_ctx *ctx, const hdm_exec_params *ep);
+int  hdm_exec_info_get(hdm_ctx *ctx, hdm_exec_info *info);
+
 /* ---- HBM bandwidth probes (the idea of Benchmark/BenchMain.cpp:14-83):
  *      kind 0 = streaming copy (read `bytes`, write `bytes`),
  *      kind 1 = six-array read-modify-write, the access pattern of the CT

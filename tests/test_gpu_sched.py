@@ -135,7 +135,7 @@ def _weak_block(skin=0.004):
     return s, p, l
 
 
-def _run(prec, s, p, l, policy, chunks, sched=None, model=None):
+def _run(prec, s, p, l, policy, chunks, sched=None, model=None, exec_=None):
     rp = A.HdmRunParams.from_buffer_copy(s.run)
     rp.policy = policy
     logs = []
@@ -146,10 +146,13 @@ def _run(prec, s, p, l, policy, chunks, sched=None, model=None):
             c.sched_configure(**sched)
         if model is not None:
             c.sched_set_model(model)
+        if exec_:
+            c.set_exec(**exec_)
         for k in chunks:
             st, log = c.run(rp, k, log_rebuilds=True)
             logs.append(log)
         stats = c.sched_stats()
+        stats.update(c.exec_info())
         return np.concatenate(logs), c.download_particles(), c.download_links(), stats
 
 
@@ -163,6 +166,8 @@ def test_learned_policy_is_bit_identical_to_exact(prec):
         dict(chunks=[400], sched=dict(min_batch=64, max_batch=64)),                       # fixed batches
         dict(chunks=[150, 1, 249], sched=dict(min_batch=1, max_batch=37, safety=0.5),     # learned, odd sizes
              model=weights_struct(M.init_weights(31, mean=np.zeros(16), std=np.ones(16)))),
+        dict(chunks=[200, 200], sched=dict(min_batch=16, max_batch=48),                   # autotuned, graphs
+             exec_=dict(autotune=True, graphs=True)),
     ]
     for case in cases:
         rl, pl, ll, stats = _run(prec, s, p, l, A.HDM_REBUILD_LEARNED, **case)
@@ -175,4 +180,6 @@ def test_learned_policy_is_bit_identical_to_exact(prec):
         assert stats["steps_executed"] == 400
         assert stats["steps_launched"] >= 400
         assert stats["host_checks"] < 400
+        if case.get("exec_"):
+            assert stats["tuned"] and stats["graph_replays"] > 50 and stats["graph_captures"] >= 1
         say("  ", case["sched"], stats)
