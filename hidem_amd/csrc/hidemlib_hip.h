@@ -75,6 +75,13 @@ public:
     int64_t numLinks() const { return nLinks; }
     int64_t rebuilds() const { return nRebuilds; }
     bool lastStepRebuilt() const { return lastRebuilt; }
+    // after steps run through hdm_run on this context (fused mode)
+    void syncFromRun(const hdm_run_stats &st) {
+        currentTimestep = (int)st.timestep;
+        nRebuilds += st.rebuilds;
+        built = true;
+    }
+    bool fusedSteps = true;           // hdm.fusedSteps: hdm_sim_steps runs hdm_run
 
     double searchDistance = 0, interactDist = 0, attractDistance = 0;
     int32_t policy = HDM_REBUILD_HEURISTIC;

@@ -168,7 +168,8 @@ SetupInfo configure(ArgParser &parser) {
     if (pol == "heuristic") policy = HDM_REBUILD_HEURISTIC;
     else if (pol == "exact") policy = HDM_REBUILD_EXACT;
     else if (pol == "interval") policy = HDM_REBUILD_INTERVAL;
-    else throw TomlError("hdm.rebuildPolicy must be heuristic, exact or interval");
+    else if (pol == "learned") policy = HDM_REBUILD_LEARNED;
+    else throw TomlError("hdm.rebuildPolicy must be heuristic, exact, interval or learned");
     if (regen != -1) policy = HDM_REBUILD_INTERVAL;
 
     // HiDEMLib.h:759-761
@@ -189,6 +190,7 @@ void HiDEMLibHip::setup(ArgParser &parser, double scale) {
     SetupInfo s = configure(parser);
     if (scale != s.gp.scale) throw TomlError("setup: scale differs from general.scale");
     maxMinutes = arg_or<unsigned int>(parser, "iteration.maxMinutes", (unsigned int)-1);
+    fusedSteps = arg_or<bool>(parser, "hdm.fusedSteps", true);
     setup(s.info);
 }
 
@@ -291,7 +293,7 @@ void HiDEMLibHip::gridService() {
             accumulatedDist += d;
             if (accumulatedDist >= interactDist / 2.0) needsGridRegen = true;
         }
-    } else if (built) {
+    } else if (built) {  // exact (and learned, whose device-side test is the exact one)
         double d = 0;
         hidem::hdm_check(accel.ctx, hdm_skin_exact(accel.ctx, &d), "hdm_skin_exact");
         if (2.0 * d >= interactDist - attractDistance) needsGridRegen = true;
@@ -417,6 +419,24 @@ int hdm_sim_steps(hdm_sim *s, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats
     auto bF = s->gp.linkBreak();
     auto t0 = std::chrono::steady_clock::now();
     int64_t r0 = h.rebuilds(), done = 0;
+    if (h.fusedSteps) {
+        // the same loop as one hdm_run call: one fused launch per step, same results
+        int64_t n = nSteps;
+        if (n > (int64_t)h.endTimestep - h.getCurrentTimestep()) n = (int64_t)h.endTimestep - h.getCurrentTimestep();
+        if (n < 0) n = 0;
+        hdm_run_params rp;
+        int rc = hdm_sim_run_params(s, &rp);
+        if (rc != HDM_OK) return rc;
+        hdm_run_stats st{};
+        rc = hdm_run(h.accel.ctx, &rp, n, rebuildLog, &st);
+        if (rc != HDM_OK) {
+            g_sim_error = hdm_last_error(h.accel.ctx);
+            return rc;
+        }
+        h.syncFromRun(st);
+        if (stats) *stats = st;
+        return HDM_OK;
+    }
     for (; done < nSteps && h.run(); done++) {
         h.gridService();
         if (rebuildLog) rebuildLog[done] = h.lastStepRebuilt() ? 1 : 0;
