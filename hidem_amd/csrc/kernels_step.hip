@@ -670,8 +670,13 @@ __device__ __forceinline__ bool certainly_intact(float2 brk, float maxLoad, floa
 // BLOCK (workgroup size) and XCD (the per-XCD contiguous renumbering of
 // workgroups) are the autotuned launch shape; every shape computes the same
 // per-particle arithmetic, so results do not depend on it.
-template <class R, class CT, int BLOCK, bool XCD>
-__global__ void __launch_bounds__(BLOCK, sizeof(R) == 4 ? 4 : 1) k_step(ForceArgs<R, CT, R> f, IntegArgs g, StepRecs<R> r) {
+//
+// PIPE software-pipelines the partner loads: the next neighbour's / link
+// side's partner index is loaded two iterations ahead and its records one
+// iteration ahead, so their latency overlaps the current pair's arithmetic
+// (more registers, fewer waves).
+template <class R, class CT, int BLOCK, bool XCD, bool PIPE>
+__global__ void __launch_bounds__(BLOCK, (sizeof(R) == 4 && !PIPE) ? 4 : 1) k_step(ForceArgs<R, CT, R> f, IntegArgs g, StepRecs<R> r) {
     if (g.guard && *g.guard) return;  // an earlier step found the neighbour list stale
     const int64_t wg = XCD ? xcd_block_index(gridDim.x) : (int64_t)blockIdx.x;
     const int64_t p = wg * BLOCK + threadIdx.x;
@@ -690,9 +695,30 @@ __global__ void __launch_bounds__(BLOCK, sizeof(R) == 4 ? 4 : 1) k_step(ForceArg
         {
             const int32_t cnt = f.nbrCount[p];
             const int32_t *nl = f.nbr + (p >> 6) * (int64_t)64 * f.nbrWidth + lane;
+            int32_t qNext = 0, qAfter = 0;
+            double4 QNext = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (PIPE) {
+                if (cnt > 0) {
+                    qNext = nl[0];
+                    QNext = r.rec[qNext];
+                }
+                if (cnt > 1) qAfter = nl[64];
+            }
             for (int32_t k = 0; k < cnt; k++) {
-                const int32_t q = nl[(int64_t)k * 64];
-                const double4 Q = r.rec[q];
+                int32_t q;
+                double4 Q;
+                if (PIPE) {
+                    q = qNext;
+                    Q = QNext;
+                    if (k + 1 < cnt) {
+                        qNext = qAfter;
+                        QNext = r.rec[qNext];
+                        if (k + 2 < cnt) qAfter = nl[(int64_t)(k + 2) * 64];
+                    }
+                } else {
+                    q = nl[(int64_t)k * 64];
+                    Q = r.rec[q];
+                }
                 const R xDist = (R)(xCord - Q.x);
                 const R yDist = (R)(yCord - Q.y);
                 const R zDist = (R)(zCord - Q.z);
@@ -719,14 +745,42 @@ __global__ void __launch_bounds__(BLOCK, sizeof(R) == 4 ? 4 : 1) k_step(ForceArg
         {
             const int32_t cnt = (int32_t)f.linkCount[p];
             const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + lane;
+            int32_t qNext = 0, qAfter = 0;
+            double4 QNext = make_double4(0.0, 0.0, 0.0, 0.0);
+            double odNext[6] = {0, 0, 0, 0, 0, 0};
+            float rotNext = 0.0f;
+            if (PIPE) {
+                if (cnt > 0) {
+                    qNext = f.slotOther[base];
+                    QNext = r.rec[qNext];
+                    load_drec(r.drec, qNext, odNext, rotNext);
+                }
+                if (cnt > 1) qAfter = f.slotOther[base + 64];
+            }
             for (int32_t k = 0; k < cnt; k++) {
                 const int64_t sidx = base + (int64_t)k * 64;
-                const int32_t q = f.slotOther[sidx];
-                const double4 Q = r.rec[q];
-                const double dx = xCord - Q.x, dy = yCord - Q.y, dz = zCord - Q.z;
+                int32_t q;
+                double4 Q;
                 double od[6];
                 float rotOther;
-                load_drec(r.drec, q, od, rotOther);
+                if (PIPE) {
+                    q = qNext;
+                    Q = QNext;
+#pragma unroll
+                    for (int c = 0; c < 6; c++) od[c] = odNext[c];
+                    rotOther = rotNext;
+                    if (k + 1 < cnt) {
+                        qNext = qAfter;
+                        QNext = r.rec[qNext];
+                        load_drec(r.drec, qNext, odNext, rotNext);
+                        if (k + 2 < cnt) qAfter = f.slotOther[sidx + 128];
+                    }
+                } else {
+                    q = f.slotOther[sidx];
+                    Q = r.rec[q];
+                    load_drec(r.drec, q, od, rotOther);
+                }
+                const double dx = xCord - Q.x, dy = yCord - Q.y, dz = zCord - Q.z;
                 R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], R(0)};  // no radius needed
                 R d[6];
                 diff_delta<R>(diff, od, d);
@@ -957,7 +1011,8 @@ struct StepShape {
     const char *name;
 };
 constexpr StepShape kShapes[STEP_VARIANTS] = {
-    {256, true, "wg256-xcd"}, {128, true, "wg128-xcd"}, {512, true, "wg512-xcd"}, {256, false, "wg256-linear"}};
+    {256, true, "wg256-xcd"},        {128, true, "wg128-xcd"},      {512, true, "wg512-xcd"},
+    {256, false, "wg256-linear"},    {256, true, "wg256-xcd-pipe"}, {128, true, "wg128-xcd-pipe"}};
 constexpr int kMinBlock = 128;
 
 int64_t rows_for(int64_t n, int block) { return (n + block - 1) / block; }
@@ -1135,10 +1190,12 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
                                             float>::type;
         const dim3 grid((unsigned)rows_for(c.n, shape.block));
         switch (v) {
-        case 0: hipLaunchKernelGGL((k_step<R, R, 256, true>), grid, dim3(256), 0, c.stream, f, g, r); break;
-        case 1: hipLaunchKernelGGL((k_step<R, R, 128, true>), grid, dim3(128), 0, c.stream, f, g, r); break;
-        case 2: hipLaunchKernelGGL((k_step<R, R, 512, true>), grid, dim3(512), 0, c.stream, f, g, r); break;
-        default: hipLaunchKernelGGL((k_step<R, R, 256, false>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        case 0: hipLaunchKernelGGL((k_step<R, R, 256, true, false>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        case 1: hipLaunchKernelGGL((k_step<R, R, 128, true, false>), grid, dim3(128), 0, c.stream, f, g, r); break;
+        case 2: hipLaunchKernelGGL((k_step<R, R, 512, true, false>), grid, dim3(512), 0, c.stream, f, g, r); break;
+        case 3: hipLaunchKernelGGL((k_step<R, R, 256, false, false>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        case 4: hipLaunchKernelGGL((k_step<R, R, 256, true, true>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        default: hipLaunchKernelGGL((k_step<R, R, 128, true, true>), grid, dim3(128), 0, c.stream, f, g, r); break;
         }
     };
     if (c.prec == HDM_PREC_FP64) launch(force_args<double, double, double>(c, fp), step_recs<double>(c, tame, brk));
