@@ -93,8 +93,13 @@ def run_gpu(setup, p, l, prec, policy, model, steps, warmup):
         ctx.sched_configure(block_cells=4, min_batch=8, max_batch=256, safety=0.5)
         if model is not None:
             ctx.sched_set_model(model)
+        # every form autotunes the step's launch shape during warmup (all shapes give
+        # identical results); the timed steps then use the form's own rebuild policy
+        ctx.set_exec(autotune=True, graphs=policy == A.HDM_REBUILD_LEARNED)
         if warmup > 0:
-            ctx.run(rp, warmup)
+            rw = A.HdmRunParams.from_buffer_copy(rp)
+            rw.policy = A.HDM_REBUILD_LEARNED
+            ctx.run(rw, warmup)
         ctx.sync()
         s0 = ctx.sched_stats()
         ctx.set_timing(True)
@@ -106,13 +111,14 @@ def run_gpu(setup, p, l, prec, policy, model, steps, warmup):
         kt = ctx.kernel_times()
         ctx.set_timing(False)
         s1 = ctx.sched_stats()
+        exec_info = ctx.exec_info()
         counts, _, _ = ctx.neighbours_download()
         n_neigh = int(counts.sum())
         broken = ctx.count_broken()
         dev_bytes = ctx.device_bytes()
     sched = {k: s1[k] - s0[k] for k in s1}
     return dict(wall=t1 - t0, stats=st, kernels=kt, n_neigh=n_neigh, broken=broken, dev_bytes=dev_bytes,
-                sched=sched)
+                sched=sched, exec=exec_info)
 
 
 def cpu_baseline(setup, p, l, steps, fast=True):
@@ -152,7 +158,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--config", default="glacier_8m")
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps in the CPU baseline sample (0: skip)")
@@ -245,7 +251,8 @@ def main():
                          "step_avg_ms": r["step_avg_ms"],
                          "step_GBps": r["step_bytes"] / (r["step_avg_ms"] * 1e-3) / 1e9,
                          "rebuilds": int(r["stats"].rebuilds), "broken_links": r["broken"],
-                         "device_bytes": r["dev_bytes"], "kernels": kern[name], "scheduler": r["sched"]}
+                         "device_bytes": r["dev_bytes"], "kernels": kern[name], "scheduler": r["sched"],
+                         "launch_shape": r["exec"]}
                   for name, r in results.items()},
     }
     for name in ("mixed", "mixed+learned"):
