@@ -205,7 +205,7 @@ hipError_t launch_force(Context &c, const hdm_force_params &fp);
 hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *skin = nullptr,
                             bool withDisplacement = false);
 // Launch shapes of the fused step (autotuner candidates).
-constexpr int STEP_VARIANTS = 6;
+constexpr int STEP_VARIANTS = 7;
 const char *step_variant_name(int v);
 
 // Guard words in Context::guardState.

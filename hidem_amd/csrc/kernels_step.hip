@@ -653,6 +653,90 @@ __device__ __forceinline__ bool certainly_intact(float2 brk, float maxLoad, floa
     return L0 >= 1e-30f && lhs + margin < rhs;
 }
 
+// Link sides of particle p with every memory access of side k + 1 issued
+// before the CT stores of side k - 1.  Loads and stores retire in issue order
+// through one counter (vmcnt) on CDNA, so a wait for a load issued after a
+// store also waits for that store to reach memory; issuing the next side's
+// loads first keeps the loop from waiting on its own write-backs.  Same
+// arithmetic and order as the plain loop.
+template <class R, class CT>
+__device__ __forceinline__ void link_loop_deep(const ForceArgs<R, CT, R> &f, const StepRecs<R> &r, int64_t p,
+                                               double xCord, double yCord, double zCord, const double diff[6],
+                                               const R self[7], const double theta[3], float rotSelf, R F[6]) {
+    const int32_t cnt = (int32_t)f.linkCount[p];
+    if (cnt == 0) return;
+    const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + (p & 63);
+    struct Side {
+        int32_t q, word;
+        double4 Q;
+        double od[6];
+        float rot;
+        float2 brk;
+        R ct[6];
+    };
+    auto load_side = [&](int64_t sidx, int32_t q, Side &s) {
+        s.q = q;
+        s.word = f.slotLink[sidx];
+        s.Q = r.rec[q];
+        load_drec(r.drec, q, s.od, s.rot);
+        s.brk = r.slotBrk[sidx];
+        s.ct[0] = (R)f.c0[sidx];
+        s.ct[1] = (R)f.c1[sidx];
+        s.ct[2] = (R)f.c2[sidx];
+        s.ct[3] = (R)f.c3[sidx];
+        s.ct[4] = (R)f.c4[sidx];
+        s.ct[5] = (R)f.c5[sidx];
+    };
+    Side cur, nxt;
+    int32_t qAfter = cnt > 1 ? f.slotOther[base + 64] : 0;
+    load_side(base, f.slotOther[base], cur);
+    R done[6];
+    int64_t doneIdx = -1;
+    for (int32_t k = 0; k < cnt; k++) {
+        const int64_t sidx = base + (int64_t)k * 64;
+        if (k + 1 < cnt) {
+            load_side(sidx + 64, qAfter, nxt);
+            if (k + 2 < cnt) qAfter = f.slotOther[sidx + 128];
+        }
+        if (doneIdx >= 0) {
+            f.c0[doneIdx] = (CT)done[0];
+            f.c1[doneIdx] = (CT)done[1];
+            f.c2[doneIdx] = (CT)done[2];
+            f.c3[doneIdx] = (CT)done[3];
+            f.c4[doneIdx] = (CT)done[4];
+            f.c5[doneIdx] = (CT)done[5];
+        }
+        const double dx = xCord - cur.Q.x, dy = yCord - cur.Q.y, dz = zCord - cur.Q.z;
+        R other[7] = {(R)cur.od[0], (R)cur.od[1], (R)cur.od[2], (R)cur.od[3], (R)cur.od[4], (R)cur.od[5], R(0)};
+        R d[6];
+        diff_delta<R>(diff, cur.od, d);
+        R mask = 1;
+        if (cur.word < 0) {
+            mask = 0;  // broken link (state == 0)
+        } else if (f.breakCheck) {
+            const float maxLoadF = r.perLinkMaxLoad ? cur.brk.y : r.scalarMaxLoad;
+            const bool skip = r.prefilter &&
+                              certainly_intact(cur.brk, maxLoadF, r.bendBreak, dx, dy, dz, rotSelf + cur.rot);
+            if (!skip && link_breaks(f, cur.word, cur.q, dx, dy, dz, theta)) {
+                mask = 0;
+                f.slotLinkW[sidx] = cur.word | SLOT_BROKEN;
+                f.state[cur.word] = 0;
+            }
+        }
+        beam_law<R>(mask, (R)dx, (R)dy, (R)dz, self, other, d, f.fp, cur.ct, F);
+#pragma unroll
+        for (int c = 0; c < 6; c++) done[c] = cur.ct[c];
+        doneIdx = sidx;
+        if (k + 1 < cnt) cur = nxt;
+    }
+    f.c0[doneIdx] = (CT)done[0];
+    f.c1[doneIdx] = (CT)done[1];
+    f.c2[doneIdx] = (CT)done[2];
+    f.c3[doneIdx] = (CT)done[3];
+    f.c4[doneIdx] = (CT)done[4];
+    f.c5[doneIdx] = (CT)done[5];
+}
+
 // One timestep of hdm_run in one launch: the link break of the previous
 // timestep (when f.breakCheck), force on particle p from the current state
 // (CombinedAccel::iterateCombined, as k_force), then its integration into the
@@ -671,12 +755,14 @@ __device__ __forceinline__ bool certainly_intact(float2 brk, float maxLoad, floa
 // workgroups) are the autotuned launch shape; every shape computes the same
 // per-particle arithmetic, so results do not depend on it.
 //
-// PIPE software-pipelines the partner loads: the next neighbour's / link
+// PIPE = 1 software-pipelines the partner loads: the next neighbour's / link
 // side's partner index is loaded two iterations ahead and its records one
-// iteration ahead, so their latency overlaps the current pair's arithmetic
-// (more registers, fewer waves).
-template <class R, class CT, int BLOCK, bool XCD, bool PIPE>
-__global__ void __launch_bounds__(BLOCK, (sizeof(R) == 4 && !PIPE) ? 4 : 1) k_step(ForceArgs<R, CT, R> f, IntegArgs g, StepRecs<R> r) {
+// iteration ahead, so their latency overlaps the current pair's arithmetic.
+// PIPE = 2 also loads the next side's CT ahead and stores each side's CT one
+// iteration late (link_loop_deep).  More registers, fewer waves; the
+// autotuner weighs that.
+template <class R, class CT, int BLOCK, bool XCD, int PIPE>
+__global__ void __launch_bounds__(BLOCK, (sizeof(R) == 4 && PIPE == 0) ? 4 : 1) k_step(ForceArgs<R, CT, R> f, IntegArgs g, StepRecs<R> r) {
     if (g.guard && *g.guard) return;  // an earlier step found the neighbour list stale
     const int64_t wg = XCD ? xcd_block_index(gridDim.x) : (int64_t)blockIdx.x;
     const int64_t p = wg * BLOCK + threadIdx.x;
@@ -742,7 +828,9 @@ __global__ void __launch_bounds__(BLOCK, (sizeof(R) == 4 && !PIPE) ? 4 : 1) k_st
                 contact_law<R>(R(1), xDist, yDist, zDist, self, other, d, f.fp, F);
             }
         }
-        {
+        if (PIPE == 2) {
+            link_loop_deep(f, r, p, xCord, yCord, zCord, diff, self, theta, rotSelf, F);
+        } else {
             const int32_t cnt = (int32_t)f.linkCount[p];
             const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + lane;
             int32_t qNext = 0, qAfter = 0;
@@ -1011,8 +1099,9 @@ struct StepShape {
     const char *name;
 };
 constexpr StepShape kShapes[STEP_VARIANTS] = {
-    {256, true, "wg256-xcd"},        {128, true, "wg128-xcd"},      {512, true, "wg512-xcd"},
-    {256, false, "wg256-linear"},    {256, true, "wg256-xcd-pipe"}, {128, true, "wg128-xcd-pipe"}};
+    {256, true, "wg256-xcd"},       {128, true, "wg128-xcd"},      {512, true, "wg512-xcd"},
+    {256, false, "wg256-linear"},   {256, true, "wg256-xcd-pipe"}, {256, true, "wg256-xcd-deep"},
+    {128, true, "wg128-xcd-deep"}};
 constexpr int kMinBlock = 128;
 
 int64_t rows_for(int64_t n, int block) { return (n + block - 1) / block; }
@@ -1190,12 +1279,13 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
                                             float>::type;
         const dim3 grid((unsigned)rows_for(c.n, shape.block));
         switch (v) {
-        case 0: hipLaunchKernelGGL((k_step<R, R, 256, true, false>), grid, dim3(256), 0, c.stream, f, g, r); break;
-        case 1: hipLaunchKernelGGL((k_step<R, R, 128, true, false>), grid, dim3(128), 0, c.stream, f, g, r); break;
-        case 2: hipLaunchKernelGGL((k_step<R, R, 512, true, false>), grid, dim3(512), 0, c.stream, f, g, r); break;
-        case 3: hipLaunchKernelGGL((k_step<R, R, 256, false, false>), grid, dim3(256), 0, c.stream, f, g, r); break;
-        case 4: hipLaunchKernelGGL((k_step<R, R, 256, true, true>), grid, dim3(256), 0, c.stream, f, g, r); break;
-        default: hipLaunchKernelGGL((k_step<R, R, 128, true, true>), grid, dim3(128), 0, c.stream, f, g, r); break;
+        case 0: hipLaunchKernelGGL((k_step<R, R, 256, true, 0>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        case 1: hipLaunchKernelGGL((k_step<R, R, 128, true, 0>), grid, dim3(128), 0, c.stream, f, g, r); break;
+        case 2: hipLaunchKernelGGL((k_step<R, R, 512, true, 0>), grid, dim3(512), 0, c.stream, f, g, r); break;
+        case 3: hipLaunchKernelGGL((k_step<R, R, 256, false, 0>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        case 4: hipLaunchKernelGGL((k_step<R, R, 256, true, 1>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        case 5: hipLaunchKernelGGL((k_step<R, R, 256, true, 2>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        default: hipLaunchKernelGGL((k_step<R, R, 128, true, 2>), grid, dim3(128), 0, c.stream, f, g, r); break;
         }
     };
     if (c.prec == HDM_PREC_FP64) launch(force_args<double, double, double>(c, fp), step_recs<double>(c, tame, brk));
