@@ -102,6 +102,8 @@ def labels(intervals, budget, steps):
         for (t, f, v) in iv["decisions"]:
             rel = t - start
             after = tripped[rel + 1:]                   # states after steps t+1 ...
+            if after.shape[0] == 0:                     # the decision closes its interval: nothing to learn
+                continue
             any_trip = after.any(axis=0)
             first = np.where(any_trip, after.argmax(axis=0) + 1, end - t)
             cens = ~any_trip

@@ -70,7 +70,7 @@ def test_batch_graph_autotuned_matches_eager(batch):
             out[name] = (np.concatenate(logs), c.download_particles(), c.download_links(), c.exec_info())
     (le, pe, ke, _), (ll, pl, kl, info) = out["exact"], out["learned"]
     say("  ", info)
-    assert le.sum() >= 3
+    assert le.sum() >= 2
     assert np.array_equal(le, ll)
     for f in A.PARTICLE_FIELDS:
         assert bits_equal(pl[f], pe[f]), f

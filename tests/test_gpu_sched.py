@@ -161,7 +161,7 @@ def test_learned_policy_is_bit_identical_to_exact(prec):
     s, p, l = _weak_block()
     say("learned vs exact", prec)
     rl_e, pe, le, _ = _run(prec, s, p, l, A.HDM_REBUILD_EXACT, [400])
-    assert 4 <= rl_e.sum() <= 100, rl_e.sum()
+    assert 2 <= rl_e.sum() <= 100, rl_e.sum()   # guard trips inside batches and rebuilds between them
     cases = [
         dict(chunks=[400], sched=dict(min_batch=64, max_batch=64)),                       # fixed batches
         dict(chunks=[150, 1, 249], sched=dict(min_batch=1, max_batch=37, safety=0.5),     # learned, odd sizes
