@@ -156,6 +156,14 @@ def _run(prec, s, p, l, policy, chunks, sched=None, model=None, exec_=None):
         return np.concatenate(logs), c.download_particles(), c.download_links(), stats
 
 
+def _biased(w, b3):
+    """An untrained network whose predictions sit near b3 (log2 steps), so the
+    horizons vary around 0.5 * 2^b3 instead of collapsing to minBatch."""
+    w = w.copy()
+    w[M.N_PARAMS - 1] = b3
+    return w
+
+
 @pytest.mark.parametrize("prec", [A.HDM_PREC_FP64, A.HDM_PREC_MIXED])
 def test_learned_policy_is_bit_identical_to_exact(prec):
     s, p, l = _weak_block()
@@ -165,12 +173,13 @@ def test_learned_policy_is_bit_identical_to_exact(prec):
     cases = [
         dict(chunks=[400], sched=dict(min_batch=64, max_batch=64)),                       # fixed batches
         dict(chunks=[150, 1, 249], sched=dict(min_batch=1, max_batch=37, safety=0.5),     # learned, odd sizes
-             model=weights_struct(M.init_weights(31, mean=np.zeros(16), std=np.ones(16)))),
+             model=weights_struct(_biased(M.init_weights(31, mean=np.zeros(16), std=np.ones(16)), 5.0))),
         dict(chunks=[200, 200], sched=dict(min_batch=16, max_batch=48),                   # autotuned, graphs
              exec_=dict(autotune=True, graphs=True)),
     ]
     for case in cases:
         rl, pl, ll, stats = _run(prec, s, p, l, A.HDM_REBUILD_LEARNED, **case)
+        say("  ", case["sched"], stats)
         assert np.array_equal(rl, rl_e)
         for f in A.PARTICLE_FIELDS:
             assert bits_equal(pl[f], pe[f]), f
@@ -182,4 +191,3 @@ def test_learned_policy_is_bit_identical_to_exact(prec):
         assert stats["host_checks"] < 400
         if case.get("exec_"):
             assert stats["tuned"] and stats["graph_replays"] > 50 and stats["graph_captures"] >= 1
-        say("  ", case["sched"], stats)
