@@ -210,6 +210,9 @@ def _declare(L):
                            C.POINTER(HdmMlpTrainParams), _dp], C.c_int),
         "hdm_sched_stats": ([P, _i64p, _i64p, _i64p], C.c_int),
         "hdm_set_exec": ([P, C.POINTER(HdmExecParams)], C.c_int),
+        "hdm_sched_label_begin": ([P, C.c_int32, C.POINTER(HdmRunParams)], C.c_int),
+        "hdm_sched_label_track": ([P], C.c_int),
+        "hdm_sched_label_end": ([P, C.c_int32, C.c_int32, _i32p, _i32p], C.c_int),
         "hdm_exec_info_get": ([P, C.POINTER(HdmExecInfo)], C.c_int),
         "hdm_gen_fcc_count": ([C.POINTER(HdmFccSpec), _i64p], C.c_int),
         "hdm_gen_fcc_fill": ([C.POINTER(HdmFccSpec), C.POINTER(HdmParticlesHost)], C.c_int),
@@ -542,6 +545,19 @@ class Context:
                                     None if cen is None else _ptr(cen, C.c_uint8), X.shape[0], C.byref(tp),
                                     C.byref(loss)), "hdm_mlp_train")
         return loss.value
+
+    def sched_label_begin(self, slot, rp):
+        self._c(lib().hdm_sched_label_begin(self._h, slot, C.byref(rp)), "hdm_sched_label_begin")
+
+    def sched_label_track(self):
+        self._c(lib().hdm_sched_label_track(self._h), "hdm_sched_label_track")
+
+    def sched_label_end(self, slot, capacity):
+        out = np.zeros(capacity, np.int32)
+        nb = C.c_int32()
+        self._c(lib().hdm_sched_label_end(self._h, slot, capacity, _ptr(out, C.c_int32), C.byref(nb)),
+                "hdm_sched_label_end")
+        return out[:nb.value]
 
     def set_exec(self, autotune=False, graphs=False, variant=-1):
         ep = HdmExecParams(int(autotune), int(graphs), variant)

@@ -1310,4 +1310,32 @@ int hdm_exec_info_get(hdm_ctx *ctx, hdm_exec_info *info) {
     return HDM_OK;
 }
 
+
+int hdm_sched_label_begin(hdm_ctx *ctx, int32_t slot, const hdm_run_params *rp) {
+    if (!valid(ctx) || !rp || slot < 0 || slot >= hdm::LABEL_SLOTS) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (c.schedNb == 0 || !c.schedAssigned) return fail(ctx, HDM_E_STATE, "hdm_sched_label_begin: evaluate features first");
+    HIP_CHECK(ctx, hdm::sched_label_begin(c, slot, 0.5 * (rp->interactDist - rp->force.attractDistance)));
+    return HDM_OK;
+}
+
+int hdm_sched_label_track(hdm_ctx *ctx) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    HIP_CHECK(ctx, hdm::sched_label_track(ctx->c));
+    return HDM_OK;
+}
+
+int hdm_sched_label_end(hdm_ctx *ctx, int32_t slot, int32_t capacity, int32_t *firstCross, int32_t *nBlocks) {
+    if (!valid(ctx) || slot < 0 || slot >= hdm::LABEL_SLOTS) return HDM_E_INVALID;
+    hdm::LabelSlot &s = ctx->c.labels[slot];
+    if (!s.active) return fail(ctx, HDM_E_STATE, "hdm_sched_label_end: slot not active");
+    if (nBlocks) *nBlocks = s.nb;
+    if (firstCross) {
+        if (s.nb > capacity) return fail(ctx, HDM_E_CAPACITY, "hdm_sched_label_end: capacity below nBlocks");
+        HIP_CHECK(ctx, d2h(firstCross, s.first.ptr, sizeof(int32_t) * (size_t)s.nb, ctx->c.stream));
+    }
+    s.active = false;
+    return HDM_OK;
+}
+
 }  // extern "C"

@@ -73,6 +73,15 @@ struct GridState {
     bool built = false;
 };
 
+// One training-label reference of the learned scheduler (hdm_sched_label_*).
+struct LabelSlot {
+    DevBuf ref, blockOf, remaining, first;
+    int32_t nb = 0;
+    int64_t start = 0;
+    bool active = false;
+};
+constexpr int LABEL_SLOTS = 32;
+
 struct Context {
     int32_t prec = HDM_PREC_FP64;
     int32_t device = 0;
@@ -149,6 +158,7 @@ struct Context {
     bool mlpHoldsModel = false;  // mlpW holds schedModel (standalone MLP calls replace it)
     DevBuf mlpW, mlpW1b, mlpW2b, mlpGpart;
     DevBuf mlpX, mlpT, mlpC, mlpY, mlpGrad, mlpM, mlpV, mlpLoss;
+    LabelSlot labels[LABEL_SLOTS];
     // run state
     int64_t ts = 0;
     double accumulatedDist = 0;
@@ -239,6 +249,8 @@ int64_t mlp_param_count();
 hipError_t sched_assign_blocks(Context &c, const SchedGeometry &g);
 hipError_t sched_features(Context &c, const SchedGeometry &g, const SchedFeatureInputs &in);
 hipError_t sched_min_prediction(Context &c, int64_t nBlocks, uint64_t *out);
+hipError_t sched_label_begin(Context &c, int slot, double budget);
+hipError_t sched_label_track(Context &c);
 hipError_t mlp_upload(Context &c, const hdm_mlp_weights &w);
 hipError_t mlp_forward(Context &c, const float *X, int64_t n, float *y);
 hipError_t mlp_gradient(Context &c, const float *X, const float *target, const uint8_t *censored, int64_t n,

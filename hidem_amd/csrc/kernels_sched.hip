@@ -470,6 +470,44 @@ __global__ void __launch_bounds__(TPB) k_mlp_cast(const float *__restrict__ w, _
     if (k < NH * NH) w2b[k] = (__bf16)w[OFF_W2 + k];
 }
 
+// Training labels: per block, the first step k (since the reference was
+// taken) at which some particle of the block has moved `remaining[b]` from its
+// reference position; firstCross[b] = -1 until then.  Steps arrive in order,
+// so the first CAS from -1 records the first crossing.
+__global__ void __launch_bounds__(TPB) k_label_track(const double *__restrict__ x, const double *__restrict__ y,
+                                                      const double *__restrict__ z, const double *__restrict__ ref,
+                                                      const int32_t *__restrict__ blockOf,
+                                                      const float *__restrict__ remaining, int32_t *firstCross,
+                                                      int64_t n, int32_t k) {
+    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (p >= n) return;
+    const int32_t b = blockOf[p];
+    if (firstCross[b] >= 0) return;
+    const double ex = x[p] - ref[3 * p], ey = y[p] - ref[3 * p + 1], ez = z[p] - ref[3 * p + 2];
+    if (sqrt(ex * ex + ey * ey + ez * ez) >= (double)remaining[b]) atomicCAS(&firstCross[b], -1, k);
+}
+
+__global__ void __launch_bounds__(TPB) k_label_begin(const double *__restrict__ x, const double *__restrict__ y,
+                                                      const double *__restrict__ z, const int32_t *__restrict__ blockOf,
+                                                      int64_t n, double *__restrict__ ref,
+                                                      int32_t *__restrict__ refBlockOf) {
+    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (p >= n) return;
+    ref[3 * p] = x[p];
+    ref[3 * p + 1] = y[p];
+    ref[3 * p + 2] = z[p];
+    refBlockOf[p] = blockOf[p];
+}
+
+__global__ void __launch_bounds__(TPB) k_label_remaining(const float *__restrict__ blockDisp, int64_t nb, float budget,
+                                                          float *__restrict__ remaining, int32_t *__restrict__ first) {
+    const int64_t b = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (b >= nb) return;
+    const float r = budget - blockDisp[b];
+    remaining[b] = r > 0.0f ? r : 0.0f;
+    first[b] = -1;
+}
+
 // Minimum prediction over valid blocks (log2 steps), as an ordered key.
 __global__ void __launch_bounds__(TPB) k_min_valid(const float *__restrict__ pred, const uint8_t *__restrict__ valid,
                                                    int64_t nb, uint64_t *out) {
@@ -582,6 +620,37 @@ hipError_t mlp_adam(Context &c, const float *grad, float *m, float *v, float sca
     const float c1 = 1.0f - powf(beta1, (float)t), c2 = 1.0f - powf(beta2, (float)t);
     hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(N_PARAMS)), dim3(TPB), 0, c.stream, c.mlpW.as<float>(), grad,
                        m, v, scale, lr, beta1, beta2, eps, c1, c2, c.mlpW1b.as<__bf16>(), c.mlpW2b.as<__bf16>());
+    return hipGetLastError();
+}
+
+hipError_t sched_label_begin(Context &c, int slot, double budget) {
+    const int64_t n = c.n, nb = c.schedNb;
+    LabelSlot &s = c.labels[slot];
+    hipError_t e = s.ref.ensure(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+    if (e == hipSuccess) e = s.blockOf.ensure(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    if (e == hipSuccess) e = s.remaining.ensure(sizeof(float) * (size_t)(nb > 0 ? nb : 1));
+    if (e == hipSuccess) e = s.first.ensure(sizeof(int32_t) * (size_t)(nb > 0 ? nb : 1));
+    if (e != hipSuccess) return e;
+    if (n > 0)
+        hipLaunchKernelGGL(k_label_begin, dim3(blocks_for(n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
+                           c.y.as<double>(), c.z.as<double>(), c.schedBlockOf.as<int32_t>(), n, s.ref.as<double>(),
+                           s.blockOf.as<int32_t>());
+    if (nb > 0)
+        hipLaunchKernelGGL(k_label_remaining, dim3(blocks_for(nb)), dim3(TPB), 0, c.stream, c.schedDisp.as<float>(), nb,
+                           (float)budget, s.remaining.as<float>(), s.first.as<int32_t>());
+    s.nb = (int32_t)nb;
+    s.start = c.ts;
+    s.active = true;
+    return hipGetLastError();
+}
+
+hipError_t sched_label_track(Context &c) {
+    for (LabelSlot &s : c.labels) {
+        if (!s.active || c.n == 0) continue;
+        hipLaunchKernelGGL(k_label_track, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
+                           c.y.as<double>(), c.z.as<double>(), s.ref.as<double>(), s.blockOf.as<int32_t>(),
+                           s.remaining.as<float>(), s.first.as<int32_t>(), c.n, (int32_t)(c.ts - s.start));
+    }
     return hipGetLastError();
 }
 

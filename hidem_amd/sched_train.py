@@ -2,15 +2,16 @@
 """Trains and evaluates the learned rebuild scheduler on the GPU.
 
 1. Trajectory: the mixed-precision form of the config (default: the 8M
-   glacier slab) runs steps [0, train_steps) under the exact rebuild test.
-   After every step the per-block maximum displacement since the last
-   rebuild is recorded; every `every` steps the 16 block features too.
+   glacier slab) runs steps [0, train_steps) under the exact rebuild test;
+   every `every` steps the 16 block features are recorded.
 2. Labels: for a decision after step t and a block b, the number of steps
-   until b alone trips the exact test (2 max |x - x0| >= interactDist -
-   attractDistance), i.e. the first later step s with blockDisp >= budget,
-   label log2(s - t).  When the rebuild (triggered by another block) or the
-   end of the trajectory comes first, the row is censored at that distance:
-   the true value is only known to be larger.
+   until some particle of b has moved b's remaining skin budget (the budget
+   (interactDist - attractDistance)/2 minus b's displacement since the
+   rebuild at t) from where it was at t -- by the triangle inequality a lower
+   bound of the steps until b alone trips the exact test, tracked on the
+   device for every block (hdm_sched_label_*).  Rows whose block has not moved
+   that far after 32 decision intervals are censored at that age (the true
+   value is only known to be larger).
 3. Training: hdm_mlp_train (BF16 MFMA forward/backward + Adam on the device)
    on the standardised, shuffled rows.
 4. Evaluation: from the state after train_steps, eval_steps more steps under
@@ -60,58 +61,58 @@ def load_model(path) -> A.HdmMlpWeights:
     return A.HdmMlpWeights.from_numpy(flat)
 
 
-def generate(setup, p, l, steps, every):
-    """Trajectory under the exact test: decision rows and per-step block displacement."""
+def generate(setup, p, l, steps, every, slots=32):
+    """Trajectory under the exact test with a label reference at every decision.
+
+    Every `every` steps the block features are evaluated and a label reference
+    is taken (hdm_sched_label_begin); after each step the device records, per
+    block, the first step at which a particle of the block has moved the
+    block's remaining skin budget from its reference position.  A reference
+    closes after slots * every steps (censoring its open blocks at that age)
+    or at the end of the trajectory."""
     rp = A.HdmRunParams.from_buffer_copy(setup.run)
     rp.policy = A.HDM_REBUILD_EXACT
-    budget = 0.5 * (rp.interactDist - rp.force.attractDistance)
-    intervals = []          # per rebuild interval: dict(start, disp [steps][nb], decisions [(t, feats, valid)])
-    cur = None
+    X, T, Cn = [], [], []
+    open_refs = {}          # slot -> (features, valid, start step)
+    rebuilds = 0
     since = 0
     t0 = time.perf_counter()
+
+    def close(c, slot, step):
+        f, v, start = open_refs.pop(slot)
+        first = c.sched_label_end(slot, len(v))
+        age = step - start
+        cens = first < 0
+        k = np.where(cens, age, first)
+        keep = (v != 0) & (k > 0)
+        X.append(f[keep])
+        T.append(np.log2(k[keep]).astype(np.float32))
+        Cn.append(cens[keep].astype(np.uint8))
+
     with A.Context(A.HDM_PREC_MIXED) as c:
         c.upload(p, l)
         c.links_build()
         for step in range(steps):
             st, rl = c.run(rp, 1, log_rebuilds=True)
             if rl[0]:
-                cur = {"start": step, "disp": [], "decisions": []}
-                intervals.append(cur)
+                rebuilds += 1
                 since = 0
             since += 1
-            f, d, v = c.sched_features(rp, since)
-            cur["disp"].append(d)
+            c.sched_label_track()
             if step % every == every - 1:
-                cur["decisions"].append((step, f, v))
+                slot = (step // every) % slots
+                if slot in open_refs:
+                    close(c, slot, step + 1)
+                f, d, v = c.sched_features(rp, since)
+                c.sched_label_begin(slot, rp)
+                open_refs[slot] = (f, v, step + 1)
             if step % 250 == 249:
-                log(f"# trajectory step {step + 1}/{steps}, {len(intervals)} rebuilds, "
-                    f"{time.perf_counter() - t0:.0f} s")
+                log(f"# trajectory step {step + 1}/{steps}, {rebuilds} rebuilds, {time.perf_counter() - t0:.0f} s")
+        for slot in list(open_refs):
+            close(c, slot, steps)
         state = (c.download_particles(), c.download_links())
         ts = st.timestep
-    return intervals, budget, state, ts
-
-
-def labels(intervals, budget, steps):
-    X, T, Cn = [], [], []
-    for i, iv in enumerate(intervals):
-        disp = np.stack(iv["disp"])                     # [len][nb]
-        start, length = iv["start"], disp.shape[0]
-        end = start + length                            # first step of the next interval (or trajectory end)
-        tripped = disp >= budget
-        # first step index (relative) at which each block trips, scanning forward from each decision
-        for (t, f, v) in iv["decisions"]:
-            rel = t - start
-            after = tripped[rel + 1:]                   # states after steps t+1 ...
-            if after.shape[0] == 0:                     # the decision closes its interval: nothing to learn
-                continue
-            any_trip = after.any(axis=0)
-            first = np.where(any_trip, after.argmax(axis=0) + 1, end - t)
-            cens = ~any_trip
-            keep = v != 0
-            X.append(f[keep])
-            T.append(np.log2(np.maximum(first[keep], 1)).astype(np.float32))
-            Cn.append(cens[keep].astype(np.uint8))
-    return np.concatenate(X), np.concatenate(T), np.concatenate(Cn)
+    return np.concatenate(X), np.concatenate(T), np.concatenate(Cn), rebuilds, state, ts
 
 
 def evaluate(setup, state, ts, steps, model, chunk=1000):
@@ -164,7 +165,7 @@ def main(argv=None):
     ap.add_argument("--cells", type=int, nargs=3, default=None)
     ap.add_argument("--train-steps", type=int, default=5000)
     ap.add_argument("--eval-steps", type=int, default=10000)
-    ap.add_argument("--every", type=int, default=25, help="steps between decision rows")
+    ap.add_argument("--every", type=int, default=50, help="steps between decision rows (label references)")
     ap.add_argument("--max-rows", type=int, default=4_000_000)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--out", default=None, help="model file (default hidem_amd/models/sched_<config>.bin)")
@@ -174,10 +175,8 @@ def main(argv=None):
     setup, p, l = params.build_inputs(cfg, args.cells)
     log(f"# {args.config}: {p.n} particles, {l.n} links")
     t0 = time.perf_counter()
-    intervals, budget, state, ts = generate(setup, p, l, args.train_steps, args.every)
-    X, T, Cn = labels(intervals, budget, args.train_steps)
-    log(f"# {len(X)} rows, {Cn.mean():.2f} censored, {len(intervals)} rebuild intervals, "
-        f"{time.perf_counter() - t0:.0f} s")
+    X, T, Cn, rebuilds, state, ts = generate(setup, p, l, args.train_steps, args.every)
+    log(f"# {len(X)} rows, {Cn.mean():.3f} censored, {rebuilds} rebuilds, {time.perf_counter() - t0:.0f} s")
     rng = np.random.default_rng(0)
     order = rng.permutation(len(X))[:args.max_rows]
     X, T, Cn = X[order], T[order], Cn[order]
@@ -202,7 +201,7 @@ def main(argv=None):
     ev = evaluate(setup, state, ts, args.eval_steps, w)
     report = {"config": args.config, "particles": p.n, "links": l.n, "train_steps": args.train_steps,
               "eval_steps": args.eval_steps, "rows": int(len(X)), "censored_fraction": float(Cn.mean()),
-              "rebuild_intervals_in_training": len(intervals), "loss_initial": loss0 / len(X),
+              "rebuilds_in_training": rebuilds, "loss_initial": loss0 / len(X),
               "loss_trained": loss, "evaluation": ev, "seconds": time.perf_counter() - t0}
     save_model(out, w, report)
     print(json.dumps(report), flush=True)

@@ -274,6 +274,17 @@ int  hdm_mlp_gradient(hdm_ctx *ctx, const hdm_mlp_weights *w, const float *X, co
  * the last epoch. */
 int  hdm_mlp_train(hdm_ctx *ctx, hdm_mlp_weights *w, const float *X, const float *target, const uint8_t *censored,
                    int64_t n, const hdm_mlp_train_params *tp, double *loss);
+/* Training labels (up to 32 concurrent references).  _begin takes the blocks
+ * and block displacements of the last hdm_sched_features call and the
+ * current positions as a reference; after every later step, _track records
+ * per block the first step count at which one of its particles has moved the
+ * block's remaining skin budget ((interactDist - attractDistance)/2 minus the
+ * block displacement at _begin) from its reference position -- a lower bound
+ * of the steps until the block trips the exact test.  _end downloads the
+ * counts (-1: not reached) and frees the slot. */
+int  hdm_sched_label_begin(hdm_ctx *ctx, int32_t slot, const hdm_run_params *rp);
+int  hdm_sched_label_track(hdm_ctx *ctx);
+int  hdm_sched_label_end(hdm_ctx *ctx, int32_t slot, int32_t capacity, int32_t *firstCross, int32_t *nBlocks);
 /* Host checks and skipped launches of HDM_REBUILD_LEARNED runs since the
  * context was created. */
 int  hdm_sched_stats(hdm_ctx *ctx, int64_t *hostChecks, int64_t *stepsLaunched, int64_t *stepsExecuted);

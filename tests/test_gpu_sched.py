@@ -191,3 +191,41 @@ def test_learned_policy_is_bit_identical_to_exact(prec):
         assert stats["host_checks"] < 400
         if case.get("exec_"):
             assert stats["tuned"] and stats["graph_replays"] > 50 and stats["graph_captures"] >= 1
+
+
+def test_label_tracking_matches_host():
+    say("label tracking")
+    s, p, l = _weak_block(skin=0.02)
+    rp = A.HdmRunParams.from_buffer_copy(s.run)
+    rp.policy = A.HDM_REBUILD_EXACT
+    with A.Context(A.HDM_PREC_FP64) as c:
+        c.upload(p, l)
+        c.links_build()
+        st, log = c.run(rp, 5, log_rebuilds=True)
+        g = c.grid_info()
+        f, d, v = c.sched_features(rp, 5)
+        q0 = c.download_particles()
+        c.sched_label_begin(3, rp)
+        xs = []
+        for k in range(60):
+            c.run(rp, 1)
+            c.sched_label_track()
+            q = c.download_particles()
+            xs.append(np.stack([q["xCord"], q["yCord"], q["zCord"]], 1))
+        first = c.sched_label_end(3, len(v))
+    x0 = np.stack([q0["xCord"], q0["yCord"], q0["zCord"]], 1)
+    edge = 4 * g.spacing
+    BX, BY, BZ = -(-g.X // 4), -(-g.Y // 4), -(-g.Z // 4)
+    ijk = np.clip(np.floor((x0 - np.array([g.xMin, g.yMin, g.zMin])) / edge).astype(np.int64), 0,
+                  np.array([BX, BY, BZ]) - 1)
+    blk = ijk[:, 0] + BX * (ijk[:, 1] + BY * ijk[:, 2])
+    budget = np.float32(0.5 * (rp.interactDist - rp.force.attractDistance))
+    remaining = np.maximum(budget - d, np.float32(0))
+    want = np.full(len(v), -1)
+    for k, x in enumerate(xs, start=1):
+        moved = np.linalg.norm(x - x0, axis=1)
+        hit = np.zeros(len(v), bool)
+        np.logical_or.at(hit, blk, moved >= remaining[blk].astype(np.float64))
+        want = np.where((want < 0) & hit, k, want)
+    assert (want > 0).sum() > 0 and (want < 0).sum() > 0   # some blocks cross within 60 steps, some do not
+    assert np.array_equal(first, want)
