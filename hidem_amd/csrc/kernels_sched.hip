@@ -76,18 +76,31 @@ struct AccArgs {
     float *acc;                        // [nBlocks][NACC]
 };
 
-// Per-particle quantities folded into their block's accumulators; a wave
-// whose active lanes share one block (the usual case: particles are stored in
-// spatial order) reduces first and issues one atomic per accumulator.
+// Per-particle quantities folded into their block's accumulators.  A
+// workgroup touches few blocks (particles are stored in spatial order), so it
+// first combines its particles in LDS -- a small open-addressed table of block
+// ids with LDS atomics -- and then issues one global atomic per accumulator
+// and distinct block.  A particle whose block finds the table full goes to
+// global memory directly.
+constexpr int ACC_TABLE = 32;
+
+__device__ __forceinline__ void acc_add(float *acc, int k, float v) {
+    if (kIsMax[k]) atomicMax(reinterpret_cast<unsigned int *>(acc + k), __float_as_uint(v));
+    else atomicAdd(acc + k, v);
+}
+
 __global__ void __launch_bounds__(TPB) k_sched_accumulate(AccArgs a) {
+    __shared__ int32_t keys[ACC_TABLE];
+    __shared__ float tab[ACC_TABLE][NACC];
+    for (int i = threadIdx.x; i < ACC_TABLE * NACC; i += TPB) {
+        if (i < ACC_TABLE) keys[i] = -1;
+        tab[i / NACC][i % NACC] = 0.0f;
+    }
+    __syncthreads();
     const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    const bool active = p < a.n;
-    float v[NACC];
-#pragma unroll
-    for (int k = 0; k < NACC; k++) v[k] = 0.0f;
-    int32_t b = -1;
-    if (active) {
-        b = a.blockOf[p];
+    if (p < a.n) {
+        float v[NACC];
+        const int32_t b = a.blockOf[p];
         const double dx = a.d0[p], dy = a.d1[p], dz = a.d2[p];
         const double D = sqrt(dx * dx + dy * dy + dz * dz);
         double Ac = 0.0;
@@ -117,36 +130,28 @@ __global__ void __launch_bounds__(TPB) k_sched_accumulate(AccArgs a) {
         v[A_SUMX] = (float)a.x[p];
         v[A_BROKEN] = (float)broken;
         v[A_SIDES] = (float)cnt;
+        // find or claim the block's LDS slot
+        int slot = -1;
+        int h = (int)((uint32_t)b * 2654435761u % ACC_TABLE);
+        for (int probe = 0; probe < ACC_TABLE; probe++, h = (h + 1) % ACC_TABLE) {
+            const int32_t prev = atomicCAS(&keys[h], -1, b);
+            if (prev == -1 || prev == b) {
+                slot = h;
+                break;
+            }
+        }
+        if (slot >= 0) {
+            for (int k = 0; k < NACC - 1; k++) acc_add(tab[slot], k, v[k]);
+        } else {
+            float *acc = a.acc + (int64_t)b * NACC;
+            for (int k = 0; k < NACC - 1; k++) acc_add(acc, k, v[k]);
+        }
     }
-    const unsigned long long act = __ballot(active);
-    if (act == 0) return;
-    const int first = __ffsll((long long)act) - 1;
-    const int32_t b0 = __shfl(b, first, 64);
-    const bool uniform = __all(!active || b == b0);
-    if (uniform) {
-#pragma unroll
-        for (int k = 0; k < NACC; k++) {
-            float w = v[k];
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                float u = __shfl_xor(w, o, 64);
-                w = kIsMax[k] ? fmaxf(w, u) : w + u;
-            }
-            v[k] = w;
-        }
-        if ((threadIdx.x & 63) == first) {
-            float *acc = a.acc + (int64_t)b0 * NACC;
-            for (int k = 0; k < NACC - 1; k++) {
-                if (kIsMax[k]) atomicMax(reinterpret_cast<unsigned int *>(acc + k), __float_as_uint(v[k]));
-                else atomicAdd(acc + k, v[k]);
-            }
-        }
-    } else if (active) {
-        float *acc = a.acc + (int64_t)b * NACC;
-        for (int k = 0; k < NACC - 1; k++) {
-            if (kIsMax[k]) atomicMax(reinterpret_cast<unsigned int *>(acc + k), __float_as_uint(v[k]));
-            else atomicAdd(acc + k, v[k]);
-        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ACC_TABLE * (NACC - 1); i += TPB) {
+        const int s = i / (NACC - 1), k = i % (NACC - 1);
+        const int32_t b = keys[s];
+        if (b >= 0) acc_add(a.acc + (int64_t)b * NACC, k, tab[s][k]);
     }
 }
 
