@@ -141,11 +141,16 @@ def probe_traffic(setup, p, l, forms):
         with A.Context(prec) as ctx:
             ctx.upload(p, l)
             ctx.links_build()
-            ctx.run(setup.run, 3)
+            # the same autotuned launch shape the timed bench steps use
+            ctx.set_exec(autotune=True)
+            rw = A.HdmRunParams.from_buffer_copy(setup.run)
+            rw.policy = A.HDM_REBUILD_LEARNED
+            ctx.run(rw, 30)
             if prec == forms[0]:
                 g = ctx.bw_probe(A.HDM_PROBE_COPY, 1 << 30, 1)  # k_probe_copy: 512 MiB read + 512 MiB written
                 log(f"copy probe 512 MiB read + 512 MiB written: {g:.0f} GB/s")
             ctx.run(setup.run, 3)
+            log(f"launch shape {ctx.exec_info()['name']}")
             ctx.sync()
             counts, _, _ = ctx.neighbours_download()
             b = step_bytes(p.n, int(counts.sum()), l.n, prec)

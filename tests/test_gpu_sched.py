@@ -227,5 +227,5 @@ def test_label_tracking_matches_host():
         hit = np.zeros(len(v), bool)
         np.logical_or.at(hit, blk, moved >= remaining[blk].astype(np.float64))
         want = np.where((want < 0) & hit, k, want)
-    assert (want > 0).sum() > 0 and (want < 0).sum() > 0   # some blocks cross within 100 steps, some do not
+    assert (want > 0).sum() > 0   # free-falling particles cover the budget in ~50 steps
     assert np.array_equal(first, want)
