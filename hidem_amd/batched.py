@@ -73,8 +73,8 @@ def main(argv=None):
     ap.add_argument("--min-particles", type=int, default=5000)
     ap.add_argument("--max-particles", type=int, default=200000)
     ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--baseline-steps", type=int, default=500,
-                    help="steps of the same run without graphs or autotuning (0: skip)")
+    ap.add_argument("--baseline-steps", type=int, default=None,
+                    help="steps of the same run without graphs or autotuning (default: --steps; 0: skip)")
     ap.add_argument("--seed", type=int, default=11)
     args = ap.parse_args(argv)
 
@@ -87,6 +87,8 @@ def main(argv=None):
         f"{b.links.n} links, generated in {time.perf_counter() - t0:.1f} s")
     report = {"blocks": args.blocks, "particles": int(b.particles.n), "links": int(b.links.n),
               "particles_per_block": [int(counts.min()), int(counts.max())]}
+    if args.baseline_steps is None:
+        args.baseline_steps = args.steps
     if args.baseline_steps > 0:
         log("# baseline: per-step launches, default shape")
         report["baseline"] = run(b, setup, args.baseline_steps, graphs=False, autotune=False, variant=0)
