@@ -271,6 +271,8 @@ int hdm_particles_upload(hdm_ctx *ctx, const hdm_particles_host *p) {
     c.snapshotValid = false;
     c.skinFused = false;
     c.recCurrent = false;
+    c.soaStale = false;
+    c.altHoldsPrev = false;
     return HDM_OK;
 }
 
@@ -279,6 +281,7 @@ int hdm_particles_download(hdm_ctx *ctx, hdm_particles_host *p) {
     Context &c = ctx->c;
     if (p->n != c.n) return fail(ctx, HDM_E_INVALID, "hdm_particles_download: n mismatch");
     hipStream_t s = c.stream;
+    HIP_CHECK(ctx, hdm::sync_state(c));
     HIP_CHECK(ctx, download_as<double>(p->xCord, c.x, c.n, s));
     HIP_CHECK(ctx, download_as<double>(p->yCord, c.y, c.n, s));
     HIP_CHECK(ctx, download_as<double>(p->zCord, c.z, c.n, s));
@@ -564,6 +567,7 @@ int hdm_integrate(hdm_ctx *ctx, const hdm_integ_params *ip) {
     HIP_CHECK(ctx, hdm::launch_integrate(ctx->c, *ip));
     ctx->c.skinFused = false;
     ctx->c.recCurrent = false;
+    ctx->c.altHoldsPrev = false;
     return HDM_OK;
 }
 
@@ -849,11 +853,7 @@ int graph_pair(hdm_ctx *ctx, const hdm_run_params *rp, uint64_t *skin, const hdm
         hipError_t e2 = e1 == hipSuccess ? hdm::launch_step(c, rp->force, rp->integ, skin, true, &rp->brk, false, guard)
                                          : e1;
         if (e1 == hipSuccess && e2 != hipSuccess) {  // keep the host's buffer view consistent with the device
-            hdm::DevBuf *cur[12] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz,
-                                    &c.diff[0], &c.diff[1], &c.diff[2], &c.diff[3], &c.diff[4], &c.diff[5]};
-            for (int q = 0; q < 12; q++) hdm::swap_buf(*cur[q], c.alt[q]);
-            hdm::swap_buf(c.rec, c.recAlt);
-            hdm::swap_buf(c.drec, c.drecAlt);
+            hdm::swap_step_state(c);
         }
         hipError_t e3 = hipStreamEndCapture(c.stream, &graph);
         c.capturing = false;
@@ -982,11 +982,7 @@ int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t 
         }
         regen = words[hdm::GUARD_FLAG] != 0;
         if ((H - k) & 1) {  // launch_step swapped once per launch; undo the skipped ones
-            hdm::DevBuf *cur[12] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz,
-                                    &c.diff[0], &c.diff[1], &c.diff[2], &c.diff[3], &c.diff[4], &c.diff[5]};
-            for (int q = 0; q < 12; q++) hdm::swap_buf(*cur[q], c.alt[q]);
-            hdm::swap_buf(c.rec, c.recAlt);
-            hdm::swap_buf(c.drec, c.drecAlt);
+            hdm::swap_step_state(c);
         }
         if (k > 0) {
             breakPending = (double)(c.ts + k - 1) * rp->dt > rp->breakTime;

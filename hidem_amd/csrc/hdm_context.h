@@ -92,8 +92,9 @@ struct Context {
     // particles
     int64_t n = 0;
     DevBuf x, y, z, tx, ty, tz, diff[6], radius, force[6], type;
-    // second state buffer of the fused step (x, y, z, x..zTheta, Diff0..5):
-    // each hdm_run step integrates into it and then swaps it with the above
+    // second buffers of the fused step: alt[3..5] hold the integrated
+    // rotations (each hdm_run step writes them and then swaps them with
+    // tx, ty, tz); the other entries are unused
     DevBuf alt[12];
     // gather records of the fused step, ping-pong like the state:
     //   rec   double4 {x, y, z, radius}; the radius reads NaN when a Diff
@@ -101,9 +102,13 @@ struct Context {
     //   drec  8 x f64: Diff0..5 (FP64 in both forms: pair differences are
     //         formed before any rounding to FP32), rotation bound
     //         |xTheta| + |yTheta| + |zTheta| (as a float), 0
+    // The fused step reads and writes only the records for coordinates and
+    // Diff: after it the SoA x, y, z and diff[] are stale (soaStale) until
+    // sync_state unpacks them.
     DevBuf rec, recAlt, drec, drecAlt;
-    bool recCurrent = false;     // rec/drec hold the current SoA state
-    bool altHoldsPrev = false;   // alt[6..8] hold the Diff before the last fused step
+    bool recCurrent = false;     // rec/drec hold the current state
+    bool soaStale = false;       // x, y, z, diff[] lag the records
+    bool altHoldsPrev = false;   // drecAlt holds the Diff before the last fused step
     double recTame = 0;          // Diff bound the radius flags were made with
     // batched form: block id per particle, waterline per block (nBlocks = 0: unused)
     int32_t nBlocks = 0;
@@ -208,6 +213,11 @@ hipError_t launch_ct_import(Context &c);   // ctStage (checkpoint order) -> ct (
 hipError_t launch_ct_export(Context &c);   // ct (tiled) -> ctStage (checkpoint order)
 hipError_t launch_side_thresholds(Context &c);  // initialLen / maxLoad -> slotBrk (after link_fill)
 hipError_t launch_pack_records(Context &c, double tame);  // SoA state -> rec / drec
+// Brings x, y, z and diff[] up to date with the records after fused steps
+// (no-op otherwise).  Every launcher that reads them calls it first.
+hipError_t sync_state(Context &c);
+// The buffer swap of one fused step (undone by a second call).
+void swap_step_state(Context &c);
 hipError_t launch_force(Context &c, const hdm_force_params &fp);
 // skin != nullptr: the rebuild-test maxima of the new state land in
 // skin[0..2] (max+ Diff per axis) and skin[4] (max |x - x0|^2, when

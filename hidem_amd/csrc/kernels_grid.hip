@@ -282,6 +282,7 @@ hipError_t launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t v, int64_t n) {
 }
 
 hipError_t launch_bbox(Context &c) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     uint64_t *r = c.red.as<uint64_t>();
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(TPB), 0, c.stream, r, ordered_key(__builtin_inf()), (int64_t)3);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(TPB), 0, c.stream, r + 3, ordered_key(-__builtin_inf()), (int64_t)3);
@@ -295,6 +296,7 @@ hipError_t launch_bbox(Context &c) {
 }
 
 hipError_t launch_colour_count(Context &c) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     GridArgs g{c.grid.xMin, c.grid.yMin, c.grid.zMin, c.grid.spacing, c.grid.X, c.grid.Y, c.grid.totalCells};
     hipError_t e = launch_fill_u32(c.stream, c.cellCursor.as<uint32_t>(), 0u, c.grid.totalCells);
     if (e != hipSuccess) return e;
@@ -343,6 +345,7 @@ static NeighArgs neigh_args(Context &c, double interactDist) {
 }
 
 hipError_t launch_neigh_count(Context &c, double interactDist) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     NeighArgs a = neigh_args(c, interactDist);
     hipLaunchKernelGGL(k_neigh<false>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
                        c.y.as<double>(), c.z.as<double>(), c.radius.as<double>(), c.n, c.colour.as<int32_t>(),
@@ -352,6 +355,7 @@ hipError_t launch_neigh_count(Context &c, double interactDist) {
 }
 
 hipError_t launch_neigh_fill(Context &c, double interactDist) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     NeighArgs a = neigh_args(c, interactDist);
     hipLaunchKernelGGL(k_neigh<true>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
                        c.y.as<double>(), c.z.as<double>(), c.radius.as<double>(), c.n, c.colour.as<int32_t>(),

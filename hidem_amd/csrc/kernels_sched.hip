@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(TPB) k_sched_blocks(const double *__restrict__
 struct AccArgs {
     const double *x, *y, *z, *x0, *y0, *z0;
     const double *d0, *d1, *d2, *d3, *d4, *d5;
-    const double *p0, *p1, *p2;       // Diff of the previous step (nullptr: unknown)
+    const double *prev;                // records before the last fused step (8 per particle; nullptr: unknown)
     const int32_t *blockOf;
     const uint32_t *linkCount;
     const int32_t *slotLink;
@@ -104,8 +104,9 @@ __global__ void __launch_bounds__(TPB) k_sched_accumulate(AccArgs a) {
         const double dx = a.d0[p], dy = a.d1[p], dz = a.d2[p];
         const double D = sqrt(dx * dx + dy * dy + dz * dz);
         double Ac = 0.0;
-        if (a.p0) {
-            const double ax = dx - a.p0[p], ay = dy - a.p1[p], az = dz - a.p2[p];
+        if (a.prev) {
+            const double *q = a.prev + p * 8;
+            const double ax = dx - q[0], ay = dy - q[1], az = dz - q[2];
             Ac = sqrt(ax * ax + ay * ay + az * az);
         }
         const double ex = a.x[p] - a.x0[p], ey = a.y[p] - a.y0[p], ez = a.z[p] - a.z0[p];
@@ -530,6 +531,7 @@ inline int blocks_for(int64_t n) { return (int)((n + TPB - 1) / TPB); }
 int64_t mlp_param_count() { return N_PARAMS; }
 
 hipError_t sched_assign_blocks(Context &c, const SchedGeometry &g) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     hipError_t e = c.schedBlockOf.ensure(sizeof(int32_t) * (size_t)(c.n > 0 ? c.n : 1));
     if (e != hipSuccess || c.n == 0) return e;
     hipLaunchKernelGGL(k_sched_blocks, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
@@ -539,6 +541,7 @@ hipError_t sched_assign_blocks(Context &c, const SchedGeometry &g) {
 }
 
 hipError_t sched_features(Context &c, const SchedGeometry &g, const SchedFeatureInputs &in) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     const int64_t nb = (int64_t)g.BX * g.BY * g.BZ;
     hipError_t e = c.schedAcc.ensure(sizeof(float) * NACC * (size_t)nb);
     if (e == hipSuccess) e = c.schedFeat.ensure(sizeof(float) * NF * (size_t)nb);
@@ -559,9 +562,7 @@ hipError_t sched_features(Context &c, const SchedGeometry &g, const SchedFeature
     a.x0 = c.x0.as<double>(); a.y0 = c.y0.as<double>(); a.z0 = c.z0.as<double>();
     a.d0 = c.diff[0].as<double>(); a.d1 = c.diff[1].as<double>(); a.d2 = c.diff[2].as<double>();
     a.d3 = c.diff[3].as<double>(); a.d4 = c.diff[4].as<double>(); a.d5 = c.diff[5].as<double>();
-    a.p0 = c.altHoldsPrev ? c.alt[6].as<double>() : nullptr;
-    a.p1 = c.altHoldsPrev ? c.alt[7].as<double>() : nullptr;
-    a.p2 = c.altHoldsPrev ? c.alt[8].as<double>() : nullptr;
+    a.prev = c.altHoldsPrev ? c.drecAlt.as<double>() : nullptr;
     a.blockOf = c.schedBlockOf.as<int32_t>();
     a.linkCount = c.linkCount.as<uint32_t>();
     a.slotLink = c.slotLink.as<int32_t>();
@@ -629,6 +630,7 @@ hipError_t mlp_adam(Context &c, const float *grad, float *m, float *v, float sca
 }
 
 hipError_t sched_label_begin(Context &c, int slot, double budget) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     const int64_t n = c.n, nb = c.schedNb;
     LabelSlot &s = c.labels[slot];
     hipError_t e = s.ref.ensure(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
@@ -650,6 +652,7 @@ hipError_t sched_label_begin(Context &c, int slot, double budget) {
 }
 
 hipError_t sched_label_track(Context &c) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     for (LabelSlot &s : c.labels) {
         if (!s.active || c.n == 0) continue;
         hipLaunchKernelGGL(k_label_track, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),

@@ -653,116 +653,55 @@ __device__ __forceinline__ bool certainly_intact(float2 brk, float maxLoad, floa
     return L0 >= 1e-30f && lhs + margin < rhs;
 }
 
-// Link sides of particle p with every memory access of side k + 1 issued
-// before the CT stores of side k - 1.  Loads and stores retire in issue order
-// through one counter (vmcnt) on CDNA, so a wait for a load issued after a
-// store also waits for that store to reach memory; issuing the next side's
-// loads first keeps the loop from waiting on its own write-backs.  Same
-// arithmetic and order as the plain loop.
-template <class R, class CT>
-__device__ __forceinline__ void link_loop_deep(const ForceArgs<R, CT, R> &f, const StepRecs<R> &r, int64_t p,
-                                               double xCord, double yCord, double zCord, const double diff[6],
-                                               const R self[7], const double theta[3], float rotSelf, R F[6]) {
-    const int32_t cnt = (int32_t)f.linkCount[p];
-    if (cnt == 0) return;
-    const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + (p & 63);
-    struct Side {
-        int32_t q, word;
-        double4 Q;
-        double od[6];
-        float rot;
-        float2 brk;
-        R ct[6];
-    };
-    auto load_side = [&](int64_t sidx, int32_t q, Side &s) {
-        s.q = q;
-        s.word = f.slotLink[sidx];
-        s.Q = r.rec[q];
-        load_drec(r.drec, q, s.od, s.rot);
-        s.brk = r.slotBrk[sidx];
-        s.ct[0] = (R)f.c0[sidx];
-        s.ct[1] = (R)f.c1[sidx];
-        s.ct[2] = (R)f.c2[sidx];
-        s.ct[3] = (R)f.c3[sidx];
-        s.ct[4] = (R)f.c4[sidx];
-        s.ct[5] = (R)f.c5[sidx];
-    };
-    Side cur, nxt;
-    int32_t qAfter = cnt > 1 ? f.slotOther[base + 64] : 0;
-    load_side(base, f.slotOther[base], cur);
-    R done[6];
-    int64_t doneIdx = -1;
-    for (int32_t k = 0; k < cnt; k++) {
-        const int64_t sidx = base + (int64_t)k * 64;
-        if (k + 1 < cnt) {
-            load_side(sidx + 64, qAfter, nxt);
-            if (k + 2 < cnt) qAfter = f.slotOther[sidx + 128];
-        }
-        if (doneIdx >= 0) {
-            f.c0[doneIdx] = (CT)done[0];
-            f.c1[doneIdx] = (CT)done[1];
-            f.c2[doneIdx] = (CT)done[2];
-            f.c3[doneIdx] = (CT)done[3];
-            f.c4[doneIdx] = (CT)done[4];
-            f.c5[doneIdx] = (CT)done[5];
-        }
-        const double dx = xCord - cur.Q.x, dy = yCord - cur.Q.y, dz = zCord - cur.Q.z;
-        R other[7] = {(R)cur.od[0], (R)cur.od[1], (R)cur.od[2], (R)cur.od[3], (R)cur.od[4], (R)cur.od[5], R(0)};
-        R d[6];
-        diff_delta<R>(diff, cur.od, d);
-        R mask = 1;
-        if (cur.word < 0) {
-            mask = 0;  // broken link (state == 0)
-        } else if (f.breakCheck) {
-            const float maxLoadF = r.perLinkMaxLoad ? cur.brk.y : r.scalarMaxLoad;
-            const bool skip = r.prefilter &&
-                              certainly_intact(cur.brk, maxLoadF, r.bendBreak, dx, dy, dz, rotSelf + cur.rot);
-            if (!skip && link_breaks(f, cur.word, cur.q, dx, dy, dz, theta)) {
-                mask = 0;
-                f.slotLinkW[sidx] = cur.word | SLOT_BROKEN;
-                f.state[cur.word] = 0;
-            }
-        }
-        beam_law<R>(mask, (R)dx, (R)dy, (R)dz, self, other, d, f.fp, cur.ct, F);
-#pragma unroll
-        for (int c = 0; c < 6; c++) done[c] = cur.ct[c];
-        doneIdx = sidx;
-        if (k + 1 < cnt) cur = nxt;
-    }
-    f.c0[doneIdx] = (CT)done[0];
-    f.c1[doneIdx] = (CT)done[1];
-    f.c2[doneIdx] = (CT)done[2];
-    f.c3[doneIdx] = (CT)done[3];
-    f.c4[doneIdx] = (CT)done[4];
-    f.c5[doneIdx] = (CT)done[5];
+// Streaming accesses of the fused step: every CT value, slot word, list entry
+// and per-particle SoA field is touched once per launch, so it is loaded and
+// stored non-temporally and leaves L2 to the gathered partner records, which
+// each particle's neighbours fetch again.
+template <class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+template <class T>
+__device__ __forceinline__ void st_stream(T *p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+__device__ __forceinline__ float2 ld_stream(const float2 *p) {
+    const unsigned long long u = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long *>(p));
+    float2 r;
+    __builtin_memcpy(&r, &u, sizeof(r));
+    return r;
 }
 
 // One timestep of hdm_run in one launch: the link break of the previous
 // timestep (when f.breakCheck), force on particle p from the current state
-// (CombinedAccel::iterateCombined, as k_force), then its integration into the
-// other state buffer (as k_integrate), with the rebuild-test partials and the
-// gather records of the new state.  The force is also stored, as the
-// separate kernels would leave it.
+// (CombinedAccel::iterateCombined, as k_force), then its integration (as
+// k_integrate), with the rebuild-test partials.
 //
-// Partner state comes from the records: one 32 B load of rec decides a
+// The records are the state: a particle's coordinates and Diff are read from
+// its own rec/drec (the same lines its neighbours gather) and the integrated
+// ones are written to the other record buffer only; the SoA coordinates and
+// Diff are unpacked from the records when something else needs them
+// (sync_state).  The rotations stay SoA (ping-pong), read at integration and
+// by the rare exact break test; the prefilter takes their bound from drec.
+//
+// Partner state comes from the records too: one 32 B load of rec decides a
 // contact pair that is provably inert (contact_is_inert: beyond
 // radSum + attractDistance with every Diff bounded by tame, so every term the
 // law adds is +-0); only other pairs, and link partners, load drec.  The
 // break test of an intact side runs exactly (FP64, as k_break) unless the
 // float prefilter proves the link stays intact.
-// The FP32 form fits 4 waves per SIMD (<= 128 VGPRs) without spilling.
-// BLOCK (workgroup size) and XCD (the per-XCD contiguous renumbering of
-// workgroups) are the autotuned launch shape; every shape computes the same
-// per-particle arithmetic, so results do not depend on it.
-//
-// PIPE = 1 software-pipelines the partner loads: the next neighbour's / link
-// side's partner index is loaded two iterations ahead and its records one
-// iteration ahead, so their latency overlaps the current pair's arithmetic.
-// PIPE = 2 also loads the next side's CT ahead and stores each side's CT one
-// iteration late (link_loop_deep).  More registers, fewer waves; the
-// autotuner weighs that.
-template <class R, class CT, int BLOCK, bool XCD, int PIPE>
-__global__ void __launch_bounds__(BLOCK, (sizeof(R) == 4 && PIPE == 0) ? 4 : 1) k_step(ForceArgs<R, CT, R> f, IntegArgs g, StepRecs<R> r) {
+// BLOCK (workgroup size), XCD (the per-XCD contiguous renumbering of
+// workgroups) and OCC (one more wave per SIMD than the register count allows,
+// bought with a few spills) are the autotuned launch shape; every shape
+// computes the same per-particle arithmetic, so results do not depend on it.
+template <class R>
+constexpr int step_waves(bool occ) {
+    return sizeof(R) == 4 ? (occ ? 5 : 4) : (occ ? 4 : 3);
+}
+
+template <class R, class CT, int BLOCK, bool XCD, bool OCC>
+__global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R, CT, R> f, IntegArgs g,
+                                                                    StepRecs<R> r) {
     if (g.guard && *g.guard) return;  // an earlier step found the neighbour list stale
     const int64_t wg = XCD ? xcd_block_index(gridDim.x) : (int64_t)blockIdx.x;
     const int64_t p = wg * BLOCK + threadIdx.x;
@@ -771,40 +710,19 @@ __global__ void __launch_bounds__(BLOCK, (sizeof(R) == 4 && PIPE == 0) ? 4 : 1) 
     if (active) {
         const double4 P = r.rec[p];
         const double xCord = P.x, yCord = P.y, zCord = P.z;
-        double diff[6] = {g.d0[p], g.d1[p], g.d2[p], g.d3[p], g.d4[p], g.d5[p]};
-        const double radius = g.radius[p];
-        double theta[3] = {g.tx[p], g.ty[p], g.tz[p]};
-        const float rotSelf = rotation_bound(theta);
+        double diff[6];
+        float rotSelf;
+        load_drec(r.drec, p, diff, rotSelf);
+        const double radius = ld_stream(g.radius + p);
         R self[7] = {(R)diff[0], (R)diff[1], (R)diff[2], (R)diff[3], (R)diff[4], (R)diff[5], (R)radius};
         R F[6] = {0, 0, 0, 0, 0, 0};
         const int64_t lane = p & 63;
         {
-            const int32_t cnt = f.nbrCount[p];
+            const int32_t cnt = ld_stream(f.nbrCount + p);
             const int32_t *nl = f.nbr + (p >> 6) * (int64_t)64 * f.nbrWidth + lane;
-            int32_t qNext = 0, qAfter = 0;
-            double4 QNext = make_double4(0.0, 0.0, 0.0, 0.0);
-            if (PIPE) {
-                if (cnt > 0) {
-                    qNext = nl[0];
-                    QNext = r.rec[qNext];
-                }
-                if (cnt > 1) qAfter = nl[64];
-            }
             for (int32_t k = 0; k < cnt; k++) {
-                int32_t q;
-                double4 Q;
-                if (PIPE) {
-                    q = qNext;
-                    Q = QNext;
-                    if (k + 1 < cnt) {
-                        qNext = qAfter;
-                        QNext = r.rec[qNext];
-                        if (k + 2 < cnt) qAfter = nl[(int64_t)(k + 2) * 64];
-                    }
-                } else {
-                    q = nl[(int64_t)k * 64];
-                    Q = r.rec[q];
-                }
+                const int32_t q = ld_stream(nl + (int64_t)k * 64);
+                const double4 Q = r.rec[q];
                 const R xDist = (R)(xCord - Q.x);
                 const R yDist = (R)(yCord - Q.y);
                 const R zDist = (R)(zCord - Q.z);
@@ -828,93 +746,60 @@ __global__ void __launch_bounds__(BLOCK, (sizeof(R) == 4 && PIPE == 0) ? 4 : 1) 
                 contact_law<R>(R(1), xDist, yDist, zDist, self, other, d, f.fp, F);
             }
         }
-        if (PIPE == 2) {
-            link_loop_deep(f, r, p, xCord, yCord, zCord, diff, self, theta, rotSelf, F);
-        } else {
-            const int32_t cnt = (int32_t)f.linkCount[p];
+        {
+            const int32_t cnt = (int32_t)ld_stream(f.linkCount + p);
             const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + lane;
-            int32_t qNext = 0, qAfter = 0;
-            double4 QNext = make_double4(0.0, 0.0, 0.0, 0.0);
-            double odNext[6] = {0, 0, 0, 0, 0, 0};
-            float rotNext = 0.0f;
-            if (PIPE) {
-                if (cnt > 0) {
-                    qNext = f.slotOther[base];
-                    QNext = r.rec[qNext];
-                    load_drec(r.drec, qNext, odNext, rotNext);
-                }
-                if (cnt > 1) qAfter = f.slotOther[base + 64];
-            }
             for (int32_t k = 0; k < cnt; k++) {
                 const int64_t sidx = base + (int64_t)k * 64;
-                int32_t q;
-                double4 Q;
+                const int32_t q = ld_stream(f.slotOther + sidx);
+                const double4 Q = r.rec[q];
                 double od[6];
                 float rotOther;
-                if (PIPE) {
-                    q = qNext;
-                    Q = QNext;
-#pragma unroll
-                    for (int c = 0; c < 6; c++) od[c] = odNext[c];
-                    rotOther = rotNext;
-                    if (k + 1 < cnt) {
-                        qNext = qAfter;
-                        QNext = r.rec[qNext];
-                        load_drec(r.drec, qNext, odNext, rotNext);
-                        if (k + 2 < cnt) qAfter = f.slotOther[sidx + 128];
-                    }
-                } else {
-                    q = f.slotOther[sidx];
-                    Q = r.rec[q];
-                    load_drec(r.drec, q, od, rotOther);
-                }
+                load_drec(r.drec, q, od, rotOther);
                 const double dx = xCord - Q.x, dy = yCord - Q.y, dz = zCord - Q.z;
                 R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], R(0)};  // no radius needed
                 R d[6];
                 diff_delta<R>(diff, od, d);
-                const int32_t word = f.slotLink[sidx];
+                const int32_t word = ld_stream(f.slotLink + sidx);
                 R mask = 1;
                 if (word < 0) {
                     mask = 0;  // broken link (state == 0)
                 } else if (f.breakCheck) {
-                    const float maxLoadF = r.perLinkMaxLoad ? r.slotBrk[sidx].y : r.scalarMaxLoad;
-                    const bool skip = r.prefilter && certainly_intact(r.slotBrk[sidx], maxLoadF, r.bendBreak, dx, dy,
-                                                                      dz, rotSelf + rotOther);
-                    if (!skip && link_breaks(f, word, q, dx, dy, dz, theta)) {
-                        // both ends reach the same verdict; each flags its own slot
-                        mask = 0;
-                        f.slotLinkW[sidx] = word | SLOT_BROKEN;
-                        f.state[word] = 0;
+                    const float2 brk = ld_stream(r.slotBrk + sidx);
+                    const float maxLoadF = r.perLinkMaxLoad ? brk.y : r.scalarMaxLoad;
+                    const bool skip = r.prefilter && certainly_intact(brk, maxLoadF, r.bendBreak, dx, dy, dz,
+                                                                      rotSelf + rotOther);
+                    if (!skip) {
+                        const double theta[3] = {g.tx[p], g.ty[p], g.tz[p]};
+                        if (link_breaks(f, word, q, dx, dy, dz, theta)) {
+                            // both ends reach the same verdict; each flags its own slot
+                            mask = 0;
+                            f.slotLinkW[sidx] = word | SLOT_BROKEN;
+                            f.state[word] = 0;
+                        }
                     }
                 }
-                R ct[6] = {(R)f.c0[sidx], (R)f.c1[sidx], (R)f.c2[sidx], (R)f.c3[sidx], (R)f.c4[sidx], (R)f.c5[sidx]};
+                R ct[6] = {(R)ld_stream(f.c0 + sidx), (R)ld_stream(f.c1 + sidx), (R)ld_stream(f.c2 + sidx),
+                           (R)ld_stream(f.c3 + sidx), (R)ld_stream(f.c4 + sidx), (R)ld_stream(f.c5 + sidx)};
                 beam_law<R>(mask, (R)dx, (R)dy, (R)dz, self, other, d, f.fp, ct, F);
-                f.c0[sidx] = (CT)ct[0];
-                f.c1[sidx] = (CT)ct[1];
-                f.c2[sidx] = (CT)ct[2];
-                f.c3[sidx] = (CT)ct[3];
-                f.c4[sidx] = (CT)ct[4];
-                f.c5[sidx] = (CT)ct[5];
+                st_stream(f.c0 + sidx, (CT)ct[0]);
+                st_stream(f.c1 + sidx, (CT)ct[1]);
+                st_stream(f.c2 + sidx, (CT)ct[2]);
+                st_stream(f.c3 + sidx, (CT)ct[3]);
+                st_stream(f.c4 + sidx, (CT)ct[4]);
+                st_stream(f.c5 + sidx, (CT)ct[5]);
             }
         }
         if (r.storeForce) store_force(f, p, F);
         const double Fd[6] = {(double)F[0], (double)F[1], (double)F[2], (double)F[3], (double)F[4], (double)F[5]};
         double cord[3] = {xCord, yCord, zCord};
-        const bool fixed = (g.type[p] & HDM_TYPE_FIXED) != 0;
+        double theta[3] = {ld_stream(g.tx + p), ld_stream(g.ty + p), ld_stream(g.tz + p)};
+        const bool fixed = (ld_stream(g.type + p) & HDM_TYPE_FIXED) != 0;
         const double waterLevel = g.blockWater ? g.blockWater[g.blockId[p]] : g.ip.waterLevel;
         integrate_state(g.ip, fixed, radius, waterLevel, Fd, cord, theta, diff);
-        g.ox[p] = cord[0];
-        g.oy[p] = cord[1];
-        g.oz[p] = cord[2];
-        g.otx[p] = theta[0];
-        g.oty[p] = theta[1];
-        g.otz[p] = theta[2];
-        g.od0[p] = diff[0];
-        g.od1[p] = diff[1];
-        g.od2[p] = diff[2];
-        g.od3[p] = diff[3];
-        g.od4[p] = diff[4];
-        g.od5[p] = diff[5];
+        st_stream(g.otx + p, theta[0]);
+        st_stream(g.oty + p, theta[1]);
+        st_stream(g.otz + p, theta[2]);
         store_records(r, p, cord, theta, diff, radius);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -923,6 +808,27 @@ __global__ void __launch_bounds__(BLOCK, (sizeof(R) == 4 && PIPE == 0) ? 4 : 1) 
         }
     }
     if (g.partials) skin_partials<BLOCK>(g, active, p, cordN, diffN);
+}
+
+// SoA coordinates and Diff from the records (sync_state).
+__global__ void __launch_bounds__(TPB) k_unpack_records(const double4 *__restrict__ rec,
+                                                         const double *__restrict__ drec, int64_t n, double *x,
+                                                         double *y, double *z, double *d0, double *d1, double *d2,
+                                                         double *d3, double *d4, double *d5) {
+    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (p >= n) return;
+    const double4 P = rec[p];
+    x[p] = P.x;
+    y[p] = P.y;
+    z[p] = P.z;
+    const double4 *v = reinterpret_cast<const double4 *>(drec + p * 8);
+    const double4 a = v[0], b = v[1];
+    d0[p] = a.x;
+    d1[p] = a.y;
+    d2[p] = a.z;
+    d3[p] = a.w;
+    d4[p] = b.x;
+    d5[p] = b.y;
 }
 
 // Gather records of the current SoA state (hdm_run entry).
@@ -1095,14 +1001,14 @@ constexpr int FINALIZE_BLOCKS = 32;
 // Launch shapes of the fused step the autotuner chooses from.
 struct StepShape {
     int block;
-    bool xcd;
+    bool xcd, occ;
     const char *name;
 };
 constexpr StepShape kShapes[STEP_VARIANTS] = {
-    {256, true, "wg256-xcd"},       {128, true, "wg128-xcd"},      {512, true, "wg512-xcd"},
-    {256, false, "wg256-linear"},   {256, true, "wg256-xcd-pipe"}, {256, true, "wg256-xcd-deep"},
-    {128, true, "wg128-xcd-deep"}};
-constexpr int kMinBlock = 128;
+    {256, true, false, "wg256-xcd"},      {128, true, false, "wg128-xcd"},   {512, true, false, "wg512-xcd"},
+    {256, false, false, "wg256-linear"},  {128, true, true, "wg128-xcd-occ"}, {256, true, true, "wg256-xcd-occ"},
+    {64, true, false, "wg64-xcd"}};
+constexpr int kMinBlock = 64;
 
 int64_t rows_for(int64_t n, int block) { return (n + block - 1) / block; }
 
@@ -1140,12 +1046,14 @@ hipError_t launch_guard_reset(Context &c) {
 }
 
 hipError_t launch_force(Context &c, const hdm_force_params &fp) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     if (c.n == 0) return hipSuccess;
     if (c.prec == HDM_PREC_FP64) return force_impl<double, double, double>(c, fp);
     return force_impl<float, float, float>(c, fp);
 }
 
 hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *skin, bool withDisplacement) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     if (c.n == 0) return hipSuccess;
     if (skin) {
         hipError_t e = ensure_partials(c);
@@ -1213,6 +1121,7 @@ hipError_t ensure_records(Context &c) {
 }  // namespace
 
 hipError_t launch_pack_records(Context &c, double tame) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     hipError_t e = ensure_records(c);
     if (e != hipSuccess || c.n == 0) return e;
     // pack writes the "out" side; swap so the packed records are current
@@ -1243,11 +1152,30 @@ hipError_t launch_pack_records(Context &c, double tame) {
     return hipSuccess;
 }
 
+void swap_step_state(Context &c) {
+    swap_buf(c.tx, c.alt[3]);
+    swap_buf(c.ty, c.alt[4]);
+    swap_buf(c.tz, c.alt[5]);
+    swap_buf(c.rec, c.recAlt);
+    swap_buf(c.drec, c.drecAlt);
+}
+
+hipError_t sync_state(Context &c) {
+    if (!c.soaStale) return hipSuccess;
+    c.soaStale = false;
+    if (c.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpack_records, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.rec.as<double4>(),
+                       c.drec.as<double>(), c.n, c.x.as<double>(), c.y.as<double>(), c.z.as<double>(),
+                       c.diff[0].as<double>(), c.diff[1].as<double>(), c.diff[2].as<double>(),
+                       c.diff[3].as<double>(), c.diff[4].as<double>(), c.diff[5].as<double>());
+    return hipGetLastError();
+}
+
 hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_params &ip, uint64_t *skin,
                        bool withDisplacement, const hdm_break_params *brk, bool storeForce, const StepGuard *guard) {
     if (c.n == 0) return hipSuccess;
     const size_t b = sizeof(double) * (size_t)c.n;
-    for (int k = 0; k < 12; k++) {
+    for (int k = 3; k < 6; k++) {
         hipError_t e = c.alt[k].ensure(b);
         if (e != hipSuccess) return e;
     }
@@ -1279,25 +1207,23 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
                                             float>::type;
         const dim3 grid((unsigned)rows_for(c.n, shape.block));
         switch (v) {
-        case 0: hipLaunchKernelGGL((k_step<R, R, 256, true, 0>), grid, dim3(256), 0, c.stream, f, g, r); break;
-        case 1: hipLaunchKernelGGL((k_step<R, R, 128, true, 0>), grid, dim3(128), 0, c.stream, f, g, r); break;
-        case 2: hipLaunchKernelGGL((k_step<R, R, 512, true, 0>), grid, dim3(512), 0, c.stream, f, g, r); break;
-        case 3: hipLaunchKernelGGL((k_step<R, R, 256, false, 0>), grid, dim3(256), 0, c.stream, f, g, r); break;
-        case 4: hipLaunchKernelGGL((k_step<R, R, 256, true, 1>), grid, dim3(256), 0, c.stream, f, g, r); break;
-        case 5: hipLaunchKernelGGL((k_step<R, R, 256, true, 2>), grid, dim3(256), 0, c.stream, f, g, r); break;
-        default: hipLaunchKernelGGL((k_step<R, R, 128, true, 2>), grid, dim3(128), 0, c.stream, f, g, r); break;
+        case 0: hipLaunchKernelGGL((k_step<R, R, 256, true, false>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        case 1: hipLaunchKernelGGL((k_step<R, R, 128, true, false>), grid, dim3(128), 0, c.stream, f, g, r); break;
+        case 2: hipLaunchKernelGGL((k_step<R, R, 512, true, false>), grid, dim3(512), 0, c.stream, f, g, r); break;
+        case 3: hipLaunchKernelGGL((k_step<R, R, 256, false, false>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        case 4: hipLaunchKernelGGL((k_step<R, R, 128, true, true>), grid, dim3(128), 0, c.stream, f, g, r); break;
+        case 5: hipLaunchKernelGGL((k_step<R, R, 256, true, true>), grid, dim3(256), 0, c.stream, f, g, r); break;
+        default: hipLaunchKernelGGL((k_step<R, R, 64, true, false>), grid, dim3(64), 0, c.stream, f, g, r); break;
         }
     };
     if (c.prec == HDM_PREC_FP64) launch(force_args<double, double, double>(c, fp), step_recs<double>(c, tame, brk));
     else launch(force_args<float, float, float>(c, fp), step_recs<float>(c, tame, brk));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // the new state becomes current; the old one is the next step's target
-    DevBuf *cur[12] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz,
-                       &c.diff[0], &c.diff[1], &c.diff[2], &c.diff[3], &c.diff[4], &c.diff[5]};
-    for (int k = 0; k < 12; k++) swap_buf(*cur[k], c.alt[k]);
-    swap_buf(c.rec, c.recAlt);
-    swap_buf(c.drec, c.drecAlt);
+    // the new state becomes current (records and rotations); the SoA
+    // coordinates and Diff are unpacked on demand
+    swap_step_state(c);
+    c.soaStale = true;
     if (!skin) return hipSuccess;
     return skin_finalize(c, skin, guard, shape.block);
 }
@@ -1305,6 +1231,7 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
 const char *step_variant_name(int v) { return (v >= 0 && v < STEP_VARIANTS) ? kShapes[v].name : "?"; }
 
 hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     if (c.L == 0) return hipSuccess;
     BreakArgs a;
     a.x = c.x.as<double>(); a.y = c.y.as<double>(); a.z = c.z.as<double>();
@@ -1324,6 +1251,7 @@ hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counte
 }
 
 hipError_t launch_skin_heuristic(Context &c, uint64_t *out3) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     hipError_t e = launch_fill_u64(c.stream, out3, ordered_key(0.0), 3);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_skin_heuristic, dim3(reduce_grid(c, c.n)), dim3(TPB), 0, c.stream, c.diff[0].as<double>(),
@@ -1332,6 +1260,7 @@ hipError_t launch_skin_heuristic(Context &c, uint64_t *out3) {
 }
 
 hipError_t launch_skin_exact(Context &c, uint64_t *out1) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     hipError_t e = launch_fill_u64(c.stream, out1, ordered_key(0.0), 1);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_skin_exact, dim3(reduce_grid(c, c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
@@ -1341,6 +1270,7 @@ hipError_t launch_skin_exact(Context &c, uint64_t *out1) {
 }
 
 hipError_t launch_snapshot(Context &c) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     size_t b = sizeof(double) * (size_t)c.n;
     hipError_t e = hipMemcpyAsync(c.x0.ptr, c.x.ptr, b, hipMemcpyDeviceToDevice, c.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(c.y0.ptr, c.y.ptr, b, hipMemcpyDeviceToDevice, c.stream);
