@@ -436,6 +436,7 @@ int hdm_neighbours_build(hdm_ctx *ctx, double interactDist) {
     HIP_CHECK(ctx, c.nbr.ensure(sizeof(int32_t) * (size_t)c.tiles() * 64 * (size_t)c.maxNeigh));
     HIP_CHECK(ctx, hdm::launch_neigh_fill(c, interactDist));
     c.neighBuilt = true;
+    c.nbrExcludesLinked = hdm::merge_linked_contacts(c);
     return HDM_OK;
 }
 
@@ -505,8 +506,13 @@ int hdm_links_build(hdm_ctx *ctx) {
     HIP_CHECK(ctx, hdm::launch_link_fill(c));
     HIP_CHECK(ctx, hdm::launch_ct_import(c));
     HIP_CHECK(ctx, hdm::launch_side_thresholds(c));
-    HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
+    uint64_t *dupFlag = c.red.as<uint64_t>() + 48;
+    HIP_CHECK(ctx, hdm::launch_partner_check(c, dupFlag));
+    uint64_t dup = 0;
+    HIP_CHECK(ctx, d2h(&dup, dupFlag, sizeof(dup), c.stream));
+    c.partnersUnique = dup == 0;
     c.linksBuilt = true;
+    if (c.nbrExcludesLinked) c.neighBuilt = false;  // the partner set may have changed
     return HDM_OK;
 }
 
@@ -1289,6 +1295,8 @@ int hdm_set_exec(hdm_ctx *ctx, const hdm_exec_params *ep) {
         c.autotune = false;
     }
     if (ep->variant >= 0) c.stepVariant = ep->variant;
+    c.mergeContacts = ep->listWithPartners == 0;
+    if (c.neighBuilt && c.nbrExcludesLinked != hdm::merge_linked_contacts(c)) c.neighBuilt = false;
     return HDM_OK;
 }
 
@@ -1303,6 +1311,7 @@ int hdm_exec_info_get(hdm_ctx *ctx, hdm_exec_info *info) {
     for (int v = 0; v < hdm::STEP_VARIANTS && v < 8; v++) info->tuneMs[v] = c.tuneMs[v];
     info->graphCaptures = c.graphCaptures;
     info->graphReplays = c.graphReplays;
+    info->mergedContacts = c.nbrExcludesLinked ? 1 : 0;
     return HDM_OK;
 }
 

@@ -109,6 +109,11 @@ struct Context {
     bool recCurrent = false;     // rec/drec hold the current state
     bool soaStale = false;       // x, y, z, diff[] lag the records
     bool altHoldsPrev = false;   // drecAlt holds the Diff before the last fused step
+    // FP32 force: the neighbour list leaves out each particle's link partners
+    // and the link loop applies their contact law (merge_linked_contacts)
+    bool partnersUnique = false;     // no particle has two sides to one partner or a side to itself
+    bool mergeContacts = true;       // allowed (hdm_set_exec)
+    bool nbrExcludesLinked = false;  // the current list was built that way
     double recTame = 0;          // Diff bound the radius flags were made with
     // batched form: block id per particle, waterline per block (nBlocks = 0: unused)
     int32_t nBlocks = 0;
@@ -179,6 +184,16 @@ struct Context {
     size_t forceBytes() const { return prec == HDM_PREC_FP64 ? 8 : 4; }
 };
 
+// Whether a neighbour list built now leaves out link partners: FP32 force
+// only (its summation order is its own; the FP64 force keeps the reference's
+// list order), with every particle's partners distinct and not itself.  The
+// contact law of a linked pair then runs in the link loop, so each step skips
+// the ~12 record gathers of a lattice particle's linked neighbours, which are
+// inert contacts while the link holds.
+inline bool merge_linked_contacts(const Context &c) {
+    return c.prec == HDM_PREC_MIXED && c.linksBuilt && c.partnersUnique && c.mergeContacts;
+}
+
 // Link slots hold the link id; the sign bit marks a broken link (state == 0).
 constexpr int32_t SLOT_BROKEN = (int32_t)0x80000000u;
 constexpr int32_t SLOT_LINK_MASK = 0x7fffffff;
@@ -212,6 +227,9 @@ hipError_t launch_link_fill(Context &c);
 hipError_t launch_ct_import(Context &c);   // ctStage (checkpoint order) -> ct (tiled)
 hipError_t launch_ct_export(Context &c);   // ct (tiled) -> ctStage (checkpoint order)
 hipError_t launch_side_thresholds(Context &c);  // initialLen / maxLoad -> slotBrk (after link_fill)
+// *flag != 0 after completion when a particle has two sides to one partner or
+// a side to itself (after link_fill).
+hipError_t launch_partner_check(Context &c, uint64_t *flag);
 hipError_t launch_pack_records(Context &c, double tame);  // SoA state -> rec / drec
 // Brings x, y, z and diff[] up to date with the records after fused steps
 // (no-op otherwise).  Every launcher that reads them calls it first.

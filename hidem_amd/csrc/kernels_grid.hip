@@ -211,7 +211,21 @@ struct NeighArgs {
     int64_t totalCells;
     double interactDist;
     int32_t width;  // ELL width (fill pass)
+    // Link partners left out of the list (nullptr: none): the FP32 force
+    // evaluates their contact law in its link loop instead
+    // (Context::nbrExcludesLinked).
+    const uint32_t *linkCount;
+    const int32_t *slotOther;
+    int32_t linkWidth;
 };
+
+__device__ __forceinline__ bool is_link_partner(const NeighArgs &a, int64_t p, int32_t q) {
+    const uint32_t cnt = a.linkCount[p];
+    const int64_t base = (p >> 6) * (int64_t)64 * a.linkWidth + (p & 63);
+    for (uint32_t k = 0; k < cnt; k++)
+        if (a.slotOther[base + (int64_t)k * 64] == q) return true;
+    return false;
+}
 
 template <bool FILL>
 __global__ void __launch_bounds__(TPB) k_neigh(const double *__restrict__ x, const double *__restrict__ y,
@@ -245,6 +259,7 @@ __global__ void __launch_bounds__(TPB) k_neigh(const double *__restrict__ x, con
             double radSum = rad + radius[nInd];
             double dist = particleDistance - radSum;
             if (dist < a.interactDist) {
+                if (a.linkCount && is_link_partner(a, p, nInd)) continue;
                 if (FILL) {
                     if (neighbours < a.width) out[(int64_t)neighbours * 64] = nInd;
                 }
@@ -341,6 +356,10 @@ static NeighArgs neigh_args(Context &c, double interactDist) {
     a.totalCells = c.grid.totalCells;
     a.interactDist = interactDist;
     a.width = c.maxNeigh;
+    const bool exclude = merge_linked_contacts(c);
+    a.linkCount = exclude ? c.linkCount.as<uint32_t>() : nullptr;
+    a.slotOther = exclude ? c.slotOther.as<int32_t>() : nullptr;
+    a.linkWidth = c.maxLinks;
     return a;
 }
 

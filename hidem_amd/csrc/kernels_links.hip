@@ -139,6 +139,23 @@ __global__ void __launch_bounds__(TPB) k_side_thresholds(const int32_t *__restri
     slotBrk[sidePos[i]] = make_float2((float)initialLen[l], maxLoad ? (float)maxLoad[l] : 0.0f);
 }
 
+// Flags a particle whose link sides name one partner twice, or itself.
+__global__ void __launch_bounds__(TPB) k_partner_check(const uint32_t *__restrict__ linkCount,
+                                                        const int32_t *__restrict__ slotOther, int32_t width,
+                                                        int64_t n, uint64_t *flag) {
+    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t cnt = linkCount[p];
+    const int64_t base = (p >> 6) * (int64_t)64 * width + (p & 63);
+    bool dup = false;
+    for (uint32_t k = 0; k < cnt && !dup; k++) {
+        const int32_t q = slotOther[base + (int64_t)k * 64];
+        dup = q == (int32_t)p;
+        for (uint32_t j = 0; j < k && !dup; j++) dup = slotOther[base + (int64_t)j * 64] == q;
+    }
+    if (dup) atomicOr((unsigned long long *)flag, 1ull);
+}
+
 __global__ void __launch_bounds__(TPB) k_count_broken(const uint8_t *__restrict__ state, int64_t L, uint64_t *out) {
     uint32_t c = 0;
     for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < L; i += (int64_t)gridDim.x * TPB)
@@ -222,6 +239,14 @@ hipError_t launch_side_thresholds(Context &c) {
     hipLaunchKernelGGL(k_side_thresholds, dim3(blocks_for(nSides)), dim3(TPB), 0, c.stream, c.sidePos.as<int32_t>(),
                        nSides, c.initialLen.as<double>(), c.hasMaxLoad ? c.maxLoad.as<double>() : nullptr,
                        c.slotBrk.as<float2>());
+    return hipGetLastError();
+}
+
+hipError_t launch_partner_check(Context &c, uint64_t *flag) {
+    hipError_t e = launch_fill_u64(c.stream, flag, 0ull, 1);
+    if (e != hipSuccess || c.n == 0) return e;
+    hipLaunchKernelGGL(k_partner_check, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.linkCount.as<uint32_t>(),
+                       c.slotOther.as<int32_t>(), c.maxLinks, c.n, flag);
     return hipGetLastError();
 }
 

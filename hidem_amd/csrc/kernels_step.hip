@@ -239,6 +239,10 @@ struct ForceArgs {
     // HiDEMDefLinkBreakFunc on the current state, exactly as k_break does
     // after the previous integration.
     int32_t breakCheck;
+    // The neighbour list leaves out link partners (Context::nbrExcludesLinked,
+    // FP32 only): the link loop applies the contact law of each side's pair,
+    // before its beam law.
+    int32_t linkedContacts;
     int32_t *slotLinkW;
     const double *tx, *ty, *tz, *initialLen, *maxLoad;
     uint8_t *state;
@@ -304,6 +308,11 @@ __device__ __forceinline__ void force_particle(const ForceArgs<R, CT, FO> &a, in
             R xDist = (R)(xCord - a.x[q]);
             R yDist = (R)(yCord - a.y[q]);
             R zDist = (R)(zCord - a.z[q]);
+            if (a.linkedContacts) {
+                R oc[7] = {other[0], other[1], other[2], other[3], other[4], other[5], (R)a.radius[q]};
+                if (!contact_is_inert<R>(xDist, yDist, zDist, self, oc, d, a.fp))
+                    contact_law<R>(R(1), xDist, yDist, zDist, self, oc, d, a.fp, F);
+            }
             R mask = 1;
             if (a.slotLink[sidx] < 0) mask = 0;  // broken link (state == 0)
             R ct[6] = {(R)a.c0[sidx], (R)a.c1[sidx], (R)a.c2[sidx], (R)a.c3[sidx], (R)a.c4[sidx], (R)a.c5[sidx]};
@@ -760,6 +769,18 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                 R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], R(0)};  // no radius needed
                 R d[6];
                 diff_delta<R>(diff, od, d);
+                if (f.linkedContacts) {
+                    // the pair's contact, with the contact loop's record test
+                    const R xDist = (R)dx, yDist = (R)dy, zDist = (R)dz;
+                    const R particleDistance2 = xDist * xDist + yDist * yDist + zDist * zDist;
+                    const R radSum = (R)P.w + (R)Q.w;
+                    const R activeDist = radSum + f.fp.attractDistance;
+                    if (!(particleDistance2 >= activeDist * activeDist && particleDistance2 <= huge_value<R>() &&
+                          particleDistance2 > R(0))) {
+                        R oc[7] = {other[0], other[1], other[2], other[3], other[4], other[5], (R)f.radius[q]};
+                        contact_law<R>(R(1), xDist, yDist, zDist, self, oc, d, f.fp, F);
+                    }
+                }
                 const int32_t word = ld_stream(f.slotLink + sidx);
                 R mask = 1;
                 if (word < 0) {
@@ -980,6 +1001,7 @@ ForceArgs<R, CT, FO> force_args(Context &c, const hdm_force_params &fp) {
     a.f3 = c.force[3].as<FO>(); a.f4 = c.force[4].as<FO>(); a.f5 = c.force[5].as<FO>();
     a.fp = cast_params<R>(fp);
     a.breakCheck = 0;
+    a.linkedContacts = c.nbrExcludesLinked ? 1 : 0;
     a.slotLinkW = c.slotLink.as<int32_t>();
     a.tx = c.tx.as<double>(); a.ty = c.ty.as<double>(); a.tz = c.tz.as<double>();
     a.initialLen = c.initialLen.as<double>();

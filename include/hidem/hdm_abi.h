@@ -300,6 +300,10 @@ typedef struct hdm_exec_params {
     int32_t autotune;
     int32_t graphs;
     int32_t variant;            /* shape to use (0..variants-1); -1 keeps the current one */
+    int32_t listWithPartners;   /* HDM_PREC_MIXED: 1 keeps link partners in the neighbour list
+                                   (the reference's list); 0 (default) leaves them out and the
+                                   link loop applies their contact law (fewer gathers).  FP64
+                                   always uses the reference's list. */
 } hdm_exec_params;
 
 typedef struct hdm_exec_info {
@@ -307,6 +311,7 @@ typedef struct hdm_exec_info {
     char name[32];
     double tuneMs[8];           /* best step time per shape seen by the autotuner (ms) */
     int64_t graphCaptures, graphReplays;
+    int32_t mergedContacts;     /* the current neighbour list leaves out link partners */
 } hdm_exec_info;
 
 int  hdm_set_exec(hdm_ctx *ctx, const hdm_exec_params *ep);

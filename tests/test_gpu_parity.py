@@ -367,3 +367,65 @@ def test_mixed_precision_error_vs_fp64(orc):
     say("mixed max |dx| =", err, "max displacement =", np.max(np.abs(disp)))
     assert err / np.max(np.abs(ref)) <= 1e-5
     assert err <= 1e-3 * np.max(np.abs(disp))
+
+
+def mixed_force(s, p, l, merged):
+    with new_ctx(p, l, A.HDM_PREC_MIXED) as ctx:
+        ctx.set_exec(list_with_partners=not merged)
+        ctx.links_build()
+        ctx.grid_build(s.search_distance)
+        ctx.neighbours_build(s.interact_dist)
+        info = ctx.exec_info()
+        total = ctx.neighbours_download()[0].sum()
+        ctx.force(s.force)
+        return ctx.download_particles(), ctx.download_links(), info["merged_contacts"], total
+
+
+def test_mixed_linked_contacts_in_link_loop(squeezed, orc):
+    """FP32 force with link partners left out of the neighbour list (their
+    contact law applied in the link loop) against the same force from the
+    reference's list, and both against the FP64 oracle, where linked
+    neighbours are in contact (squeezed block)."""
+    s, p0, l0 = squeezed
+    p = perturb(p0, 41)
+    l = l0.copy()
+    rng = np.random.default_rng(42)
+    for c in range(6):
+        l.ct[c][:] = rng.normal(scale=1e3, size=2 * l.n)
+    l.state[rng.random(l.n) < 0.1] = 0
+    say("mixed linked contacts")
+    mp, ml, m_on, m_total = mixed_force(s, p, l, True)
+    lp, ll, l_on, l_total = mixed_force(s, p, l, False)
+    op, ol = oracle_force(orc, s, p, l)
+    assert m_on and not l_on
+    # every particle has linked neighbours, all of them left out of the list
+    assert m_total + 2 * l.n == l_total
+    for f in FORCE_FIELDS:
+        scale = np.max(np.abs(op[f]))
+        assert scale > 0, f
+        for got, name in ((mp, "merged"), (lp, "listed")):
+            err = np.max(np.abs(got[f].astype(np.float64) - op[f]))
+            assert err <= 2e-5 * scale, (name, f, err, scale)
+    for c in range(6):
+        assert np.array_equal(ml.ct[c], ll.ct[c]), c  # CT does not depend on the contacts
+
+
+def test_mixed_fused_step_matches_per_call_force_with_linked_contacts(squeezed):
+    """The fused step (record-based inert test) and k_force (contact_is_inert)
+    apply the same linked contacts: the first step's force is bit-identical."""
+    s, p0, l0 = squeezed
+    p = perturb(p0, 43)
+    say("mixed fused vs per-call, linked contacts")
+    with new_ctx(p, l0, A.HDM_PREC_MIXED) as ctx:
+        ctx.links_build()
+        ctx.run(s.run, 1)
+        assert ctx.exec_info()["merged_contacts"]
+        fused = ctx.download_particles()
+    with new_ctx(p, l0, A.HDM_PREC_MIXED) as ctx:
+        ctx.links_build()
+        ctx.grid_build(s.search_distance)
+        ctx.neighbours_build(s.interact_dist)
+        ctx.force(s.force)
+        per_call = ctx.download_particles()
+    for f in FORCE_FIELDS:
+        assert bits_equal(fused[f], per_call[f]), f
