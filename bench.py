@@ -114,6 +114,10 @@ def run_gpu(setup, p, l, prec, policy, model, steps, warmup):
         exec_info = ctx.exec_info()
         counts, _, _ = ctx.neighbours_download()
         n_neigh = int(counts.sum())
+        if exec_info["merged_contacts"]:
+            # the list leaves out link partners (their contacts run in the link
+            # loop); count the reference list's entries, two per link
+            n_neigh += 2 * l.n
         broken = ctx.count_broken()
         dev_bytes = ctx.device_bytes()
     sched = {k: s1[k] - s0[k] for k in s1}

@@ -224,7 +224,7 @@ int hdm_device_bytes(hdm_ctx *ctx, int64_t *bytes) {
                                  &c.slotOther, &c.sidePos, &c.nbrCount, &c.nbr, &c.colour, &c.order,
                                  &c.cellStart, &c.cellCursor, &c.scanPartials, &c.red, &c.x0, &c.y0, &c.z0,
                                  &c.skinPartials, &c.blockId, &c.blockWater, &c.probe, &c.rec, &c.recAlt,
-                                 &c.drec, &c.drecAlt, &c.slotBrk};
+                                 &c.drec, &c.drecAlt, &c.slotThr};
     for (auto *b : bufs) t += b->bytes;
     for (int k = 0; k < 6; k++) t += c.diff[k].bytes + c.force[k].bytes + c.ct[k].bytes + c.ctStage[k].bytes;
     for (int k = 0; k < 12; k++) t += c.alt[k].bytes;
@@ -502,15 +502,17 @@ int hdm_links_build(hdm_ctx *ctx) {
     HIP_CHECK(ctx, c.slotOther.ensure(sizeof(int32_t) * slots));
     HIP_CHECK(ctx, c.sidePos.ensure(sizeof(int32_t) * (size_t)(2 * c.L > 0 ? 2 * c.L : 1)));
     for (int k = 0; k < 6; k++) HIP_CHECK(ctx, c.ct[k].ensure(c.ctBytes() * slots));
-    HIP_CHECK(ctx, c.slotBrk.ensure(sizeof(float) * 2 * slots));
+    HIP_CHECK(ctx, c.slotThr.ensure(sizeof(float) * slots));
     HIP_CHECK(ctx, hdm::launch_link_fill(c));
     HIP_CHECK(ctx, hdm::launch_ct_import(c));
-    HIP_CHECK(ctx, hdm::launch_side_thresholds(c));
-    uint64_t *dupFlag = c.red.as<uint64_t>() + 48;
-    HIP_CHECK(ctx, hdm::launch_partner_check(c, dupFlag));
-    uint64_t dup = 0;
-    HIP_CHECK(ctx, d2h(&dup, dupFlag, sizeof(dup), c.stream));
-    c.partnersUnique = dup == 0;
+    uint64_t *words = c.red.as<uint64_t>() + 48;  // [0] partner check flag, [1] largest L0 (float bits)
+    HIP_CHECK(ctx, hdm::launch_partner_check(c, words));
+    HIP_CHECK(ctx, hdm::launch_side_thresholds(c, words + 1));
+    uint64_t w[2] = {0, 0};
+    HIP_CHECK(ctx, d2h(w, words, sizeof(w), c.stream));
+    c.partnersUnique = w[0] == 0;
+    const uint32_t l0Bits = (uint32_t)w[1];
+    std::memcpy(&c.l0Max, &l0Bits, sizeof(float));
     c.linksBuilt = true;
     if (c.nbrExcludesLinked) c.neighBuilt = false;  // the partner set may have changed
     return HDM_OK;
@@ -546,7 +548,7 @@ int hdm_linklists_download(hdm_ctx *ctx, int32_t *counts, int64_t *starts, int32
                 size_t s = (size_t)(p >> 6) * 64 * w + (size_t)k * 64 + (size_t)(p & 63);
                 int32_t l = sl[s] & hdm::SLOT_LINK_MASK;
                 if (linkId) linkId[pos] = l;
-                if (other) other[pos] = so[s];
+                if (other) other[pos] = so[s] & hdm::SLOT_LINK_MASK;
                 // the end is whichever of the link's two sides maps to this slot
                 if (end) end[pos] = (sp[(size_t)(2 * l + 1)] == (int32_t)s) ? 1 : 0;
                 pos++;
@@ -816,7 +818,7 @@ std::vector<unsigned char> graph_key(const Context &c, const hdm_run_params *rp,
     };
     const hdm::DevBuf *bufs[] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz, &c.radius, &c.type, &c.rec, &c.recAlt,
                                  &c.drec, &c.drecAlt, &c.nbrCount, &c.nbr, &c.linkCount, &c.slotLink, &c.slotOther,
-                                 &c.state, &c.initialLen, &c.maxLoad, &c.slotBrk, &c.skinPartials, &c.guardState,
+                                 &c.state, &c.initialLen, &c.maxLoad, &c.slotThr, &c.skinPartials, &c.guardState,
                                  &c.x0, &c.y0, &c.z0, &c.blockId, &c.blockWater};
     for (auto *b : bufs) put(&b->ptr, sizeof(void *));
     for (int q = 0; q < 6; q++) {

@@ -128,7 +128,8 @@ struct Context {
     bool linksBuilt = false;
     DevBuf linkCount, linkCursor, slotLink, slotOther, ct[6];
     DevBuf sidePos;        // 2L: tiled slot of side (l, end)
-    DevBuf slotBrk;        // float2 per slot: (initialLen, maxLoad) of the side's link, for the break prefilter
+    DevBuf slotThr;        // float per slot: break prefilter threshold of the side's link (k_side_thresholds)
+    float l0Max = 0.0f;    // largest initialLen (rounded up to float)
     // neighbour list (wave tiles)
     int32_t maxNeigh = 0;
     bool neighBuilt = false;
@@ -226,7 +227,9 @@ hipError_t launch_link_count(Context &c);
 hipError_t launch_link_fill(Context &c);
 hipError_t launch_ct_import(Context &c);   // ctStage (checkpoint order) -> ct (tiled)
 hipError_t launch_ct_export(Context &c);   // ct (tiled) -> ctStage (checkpoint order)
-hipError_t launch_side_thresholds(Context &c);  // initialLen / maxLoad -> slotBrk (after link_fill)
+// initialLen / maxLoad -> slotThr, and the largest initialLen as float bits
+// into *l0MaxBits (after link_fill)
+hipError_t launch_side_thresholds(Context &c, uint64_t *l0MaxBits);
 // *flag != 0 after completion when a particle has two sides to one partner or
 // a side to itself (after link_fill).
 hipError_t launch_partner_check(Context &c, uint64_t *flag);

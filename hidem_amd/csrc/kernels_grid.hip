@@ -223,7 +223,7 @@ __device__ __forceinline__ bool is_link_partner(const NeighArgs &a, int64_t p, i
     const uint32_t cnt = a.linkCount[p];
     const int64_t base = (p >> 6) * (int64_t)64 * a.linkWidth + (p & 63);
     for (uint32_t k = 0; k < cnt; k++)
-        if (a.slotOther[base + (int64_t)k * 64] == q) return true;
+        if ((a.slotOther[base + (int64_t)k * 64] & SLOT_LINK_MASK) == q) return true;
     return false;
 }
 

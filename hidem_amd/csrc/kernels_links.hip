@@ -81,10 +81,11 @@ __global__ void __launch_bounds__(TPB) k_link_sort(const int32_t *__restrict__ A
             other = A[l];
             end = 1;
         }
-        slotOther[s] = other;
         sidePos[2 * (int64_t)l + end] = (int32_t)s;
-        // the slot word carries the link's broken flag in its sign bit, so the
-        // force gather needs no per-link state lookup
+        // both slot words carry the link's broken flag in their sign bit, so
+        // the force gather needs no per-link state lookup (the fused step
+        // reads only the partner word)
+        slotOther[s] = state[l] == 0 ? (other | SLOT_BROKEN) : other;
         if (state[l] == 0) slotLink[s] = l | SLOT_BROKEN;
     }
 }
@@ -126,17 +127,32 @@ __global__ void __launch_bounds__(TPB) k_ct_export(const int32_t *__restrict__ s
     s5[i] = (double)c5[d];
 }
 
-// Per-side copy of the link's fracture parameters in float, in the side's
-// wave-tile slot, so the fused step can prefilter the break test with one
-// coalesced 8 B load per side (the exact test still reads the FP64 values).
+// Per-side float threshold of the break prefilter, in the side's wave-tile
+// slot (one coalesced 4 B load per side; the exact test still reads the FP64
+// values): L0 (1 + maxLoad) with per-link maxLoad, else L0; NaN when L0 is
+// below 1e-30 or not finite.  Also the largest L0 (as float bits, rounded up;
+// +inf when some L0 is not finite), which bounds the bend term.
 __global__ void __launch_bounds__(TPB) k_side_thresholds(const int32_t *__restrict__ sidePos, int64_t nSides,
                                                           const double *__restrict__ initialLen,
                                                           const double *__restrict__ maxLoad,
-                                                          float2 *__restrict__ slotBrk) {
+                                                          float *__restrict__ slotThr, uint64_t *l0MaxBits) {
     int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i >= nSides) return;
-    int64_t l = i >> 1;
-    slotBrk[sidePos[i]] = make_float2((float)initialLen[l], maxLoad ? (float)maxLoad[l] : 0.0f);
+    float big = 0.0f;
+    if (i < nSides) {
+        int64_t l = i >> 1;
+        const double L0 = initialLen[l];
+        const bool ok = L0 >= 1e-30 && L0 <= 1e300;
+        const double T = maxLoad ? L0 * (1.0 + maxLoad[l]) : L0;
+        slotThr[sidePos[i]] = ok ? (float)T : __builtin_nanf("");
+        if (ok) {
+            const float f = (float)L0;
+            big = (double)f < L0 ? __uint_as_float(__float_as_uint(f) + 1u) : f;  // next float up
+        } else if (!(L0 < 1e-30)) {
+            big = __builtin_inff();  // NaN or huge: no useful bound
+        }
+    }
+    const uint32_t bits = wave_max_u32(__float_as_uint(big));  // non-negative floats order as integers
+    if ((threadIdx.x & 63) == 0 && bits) atomicMax((unsigned long long *)l0MaxBits, (unsigned long long)bits);
 }
 
 // Flags a particle whose link sides name one partner twice, or itself.
@@ -149,9 +165,9 @@ __global__ void __launch_bounds__(TPB) k_partner_check(const uint32_t *__restric
     const int64_t base = (p >> 6) * (int64_t)64 * width + (p & 63);
     bool dup = false;
     for (uint32_t k = 0; k < cnt && !dup; k++) {
-        const int32_t q = slotOther[base + (int64_t)k * 64];
+        const int32_t q = slotOther[base + (int64_t)k * 64] & SLOT_LINK_MASK;
         dup = q == (int32_t)p;
-        for (uint32_t j = 0; j < k && !dup; j++) dup = slotOther[base + (int64_t)j * 64] == q;
+        for (uint32_t j = 0; j < k && !dup; j++) dup = (slotOther[base + (int64_t)j * 64] & SLOT_LINK_MASK) == q;
     }
     if (dup) atomicOr((unsigned long long *)flag, 1ull);
 }
@@ -233,12 +249,13 @@ hipError_t launch_ct_export(Context &c) {
     return hipGetLastError();
 }
 
-hipError_t launch_side_thresholds(Context &c) {
+hipError_t launch_side_thresholds(Context &c, uint64_t *l0MaxBits) {
+    hipError_t e = launch_fill_u64(c.stream, l0MaxBits, 0ull, 1);
     int64_t nSides = 2 * c.L;
-    if (nSides == 0) return hipSuccess;
+    if (e != hipSuccess || nSides == 0) return e;
     hipLaunchKernelGGL(k_side_thresholds, dim3(blocks_for(nSides)), dim3(TPB), 0, c.stream, c.sidePos.as<int32_t>(),
                        nSides, c.initialLen.as<double>(), c.hasMaxLoad ? c.maxLoad.as<double>() : nullptr,
-                       c.slotBrk.as<float2>());
+                       c.slotThr.as<float>(), l0MaxBits);
     return hipGetLastError();
 }
 

@@ -243,7 +243,7 @@ struct ForceArgs {
     // FP32 only): the link loop applies the contact law of each side's pair,
     // before its beam law.
     int32_t linkedContacts;
-    int32_t *slotLinkW;
+    int32_t *slotLinkW, *slotOtherW;
     const double *tx, *ty, *tz, *initialLen, *maxLoad;
     uint8_t *state;
     double bendBreak, scalarMaxLoad;
@@ -300,7 +300,8 @@ __device__ __forceinline__ void force_particle(const ForceArgs<R, CT, FO> &a, in
         const int64_t base = (p >> 6) * (int64_t)64 * a.linkWidth + lane;
         for (int32_t k = 0; k < cnt; k++) {
             const int64_t sidx = base + (int64_t)k * 64;
-            const int32_t q = a.slotOther[sidx];
+            const int32_t ow = a.slotOther[sidx];
+            const int32_t q = ow & SLOT_LINK_MASK;
             const double od[6] = {a.d0[q], a.d1[q], a.d2[q], a.d3[q], a.d4[q], a.d5[q]};
             R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], R(0)};
             R d[6];
@@ -308,13 +309,13 @@ __device__ __forceinline__ void force_particle(const ForceArgs<R, CT, FO> &a, in
             R xDist = (R)(xCord - a.x[q]);
             R yDist = (R)(yCord - a.y[q]);
             R zDist = (R)(zCord - a.z[q]);
-            if (a.linkedContacts) {
+            if (sizeof(R) == 4 && a.linkedContacts) {
                 R oc[7] = {other[0], other[1], other[2], other[3], other[4], other[5], (R)a.radius[q]};
                 if (!contact_is_inert<R>(xDist, yDist, zDist, self, oc, d, a.fp))
                     contact_law<R>(R(1), xDist, yDist, zDist, self, oc, d, a.fp, F);
             }
             R mask = 1;
-            if (a.slotLink[sidx] < 0) mask = 0;  // broken link (state == 0)
+            if (ow < 0) mask = 0;  // broken link (state == 0)
             R ct[6] = {(R)a.c0[sidx], (R)a.c1[sidx], (R)a.c2[sidx], (R)a.c3[sidx], (R)a.c4[sidx], (R)a.c5[sidx]};
             beam_law<R>(mask, xDist, yDist, zDist, self, other, d, a.fp, ct, F);
             a.c0[sidx] = (CT)ct[0];
@@ -608,10 +609,13 @@ struct StepRecs {
     double4 *recOut;           // integrated state
     double *drecOut;
     double tame;               // Diff bound behind the radius flag of rec
-    const float2 *slotBrk;     // per side (initialLen, maxLoad) in float
+    // Break prefilter inputs: per side a float threshold, L0 (1 + maxLoad)
+    // with per-link maxLoad (perLinkMaxLoad) else L0, NaN when L0 < 1e-30 or
+    // not finite; the largest L0 of the state (rounded up)
+    const float *slotThr;
     int32_t prefilter;         // 0: every intact side takes the exact break test
     int32_t perLinkMaxLoad;
-    float scalarMaxLoad, bendBreak;
+    float scalarMaxLoad, bendBreak, l0Max;
     int32_t storeForce;        // write x..wForce (the last step of a run; earlier ones are overwritten unseen)
 };
 
@@ -641,25 +645,23 @@ __device__ __forceinline__ void store_records(const StepRecs<R> &r, int64_t p, c
     d[1] = make_double4(diff[4], diff[5], (double)rotation_bound(theta), 0.0);
 }
 
-// Break prefilter: true when the link certainly stays intact, i.e. the float
-// estimate of (s - L0) + bendBreak * bound * L0 is below maxLoad * L0 by more
-// than 1e-5 of the magnitudes involved.  Float rounding of the inputs and
-// operations is below 1e-6 of those magnitudes, the rotation bound is at
-// least the exact maximum bend, and the exact test is
-// (s - L0) / L0 + bendBreak * maxBend >= maxLoad with L0 > 0 and
-// bendBreak >= 0 (the host disables the prefilter otherwise), so a link the
-// exact FP64 test would break never passes.  NaN/inf inputs fail the
-// comparison and go to the exact test.
-__device__ __forceinline__ bool certainly_intact(float2 brk, float maxLoad, float bendBreak, double dx, double dy,
+// Break prefilter: true when the link certainly stays intact.  The exact test
+// (s - L0) / L0 + bendBreak * maxBend >= maxLoad, with L0 > 0, is
+// s + bendBreak * maxBend * L0 >= L0 (1 + maxLoad) =: T.  In float, with the
+// rotation bound (at least maxBend) and the largest L0 standing in for the
+// link's own, the left side can only grow; the link passes when that estimate
+// is below T by more than 1e-5 of the magnitudes.  Float rounding of the
+// inputs and operations is below 1e-6 of those magnitudes and bendBreak >= 0
+// (the host disables the prefilter otherwise), so a link the exact FP64 test
+// would break never passes.  NaN/inf inputs (a NaN threshold for L0 < 1e-30)
+// fail the comparison and go to the exact test.
+__device__ __forceinline__ bool certainly_intact(float T, float bendBreak, float l0Max, double dx, double dy,
                                                  double dz, float rotBound) {
-    const float L0 = brk.x;
     const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
     const float s = sqrtf(fx * fx + fy * fy + fz * fz);
-    const float bend = bendBreak * rotBound * L0;
-    const float lhs = (s - L0) + bend;
-    const float rhs = maxLoad * L0;
-    const float margin = 1e-5f * (s + L0 + bend + fabsf(rhs) + fabsf(lhs));
-    return L0 >= 1e-30f && lhs + margin < rhs;
+    const float lhs = s + bendBreak * rotBound * l0Max;
+    const float margin = 1e-5f * (lhs + fabsf(T));
+    return lhs + margin < T;
 }
 
 // Streaming accesses of the fused step: every CT value, slot word, list entry
@@ -760,7 +762,8 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
             const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + lane;
             for (int32_t k = 0; k < cnt; k++) {
                 const int64_t sidx = base + (int64_t)k * 64;
-                const int32_t q = ld_stream(f.slotOther + sidx);
+                const int32_t ow = ld_stream(f.slotOther + sidx);  // partner; sign bit: broken
+                const int32_t q = ow & SLOT_LINK_MASK;
                 const double4 Q = r.rec[q];
                 double od[6];
                 float rotOther;
@@ -769,8 +772,9 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                 R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], R(0)};  // no radius needed
                 R d[6];
                 diff_delta<R>(diff, od, d);
-                if (f.linkedContacts) {
+                if (sizeof(R) == 4 && f.linkedContacts) {
                     // the pair's contact, with the contact loop's record test
+                    // (FP32 force only: merge_linked_contacts)
                     const R xDist = (R)dx, yDist = (R)dy, zDist = (R)dz;
                     const R particleDistance2 = xDist * xDist + yDist * yDist + zDist * zDist;
                     const R radSum = (R)P.w + (R)Q.w;
@@ -781,21 +785,22 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                         contact_law<R>(R(1), xDist, yDist, zDist, self, oc, d, f.fp, F);
                     }
                 }
-                const int32_t word = ld_stream(f.slotLink + sidx);
                 R mask = 1;
-                if (word < 0) {
+                if (ow < 0) {
                     mask = 0;  // broken link (state == 0)
                 } else if (f.breakCheck) {
-                    const float2 brk = ld_stream(r.slotBrk + sidx);
-                    const float maxLoadF = r.perLinkMaxLoad ? brk.y : r.scalarMaxLoad;
-                    const bool skip = r.prefilter && certainly_intact(brk, maxLoadF, r.bendBreak, dx, dy, dz,
-                                                                      rotSelf + rotOther);
+                    const float thr = ld_stream(r.slotThr + sidx);
+                    const float T = r.perLinkMaxLoad ? thr : thr * (1.0f + r.scalarMaxLoad);
+                    const bool skip = r.prefilter &&
+                                      certainly_intact(T, r.bendBreak, r.l0Max, dx, dy, dz, rotSelf + rotOther);
                     if (!skip) {
+                        const int32_t word = f.slotLink[sidx];  // link id (intact: sign bit clear)
                         const double theta[3] = {g.tx[p], g.ty[p], g.tz[p]};
                         if (link_breaks(f, word, q, dx, dy, dz, theta)) {
                             // both ends reach the same verdict; each flags its own slot
                             mask = 0;
                             f.slotLinkW[sidx] = word | SLOT_BROKEN;
+                            f.slotOtherW[sidx] = ow | SLOT_BROKEN;
                             f.state[word] = 0;
                         }
                     }
@@ -876,7 +881,7 @@ struct BreakArgs {
     const double *x, *y, *z, *tx, *ty, *tz;
     const int32_t *A, *B;
     uint8_t *state;
-    int32_t *slotLink;            // broken flag mirrored into both of the link's slots
+    int32_t *slotLink, *slotOther;  // broken flag mirrored into both of the link's slots
     const int32_t *sidePos;
     const double *initialLen, *maxLoad;
     int64_t L;
@@ -906,8 +911,11 @@ __global__ void __launch_bounds__(TPB) k_break(BreakArgs a) {
         double maxLoad = a.maxLoad ? a.maxLoad[l] : a.scalarMaxLoad;
         if (load >= maxLoad) {
             a.state[l] = 0;
-            a.slotLink[a.sidePos[2 * l]] |= SLOT_BROKEN;
-            a.slotLink[a.sidePos[2 * l + 1]] |= SLOT_BROKEN;
+            const int32_t s0 = a.sidePos[2 * l], s1 = a.sidePos[2 * l + 1];
+            a.slotLink[s0] |= SLOT_BROKEN;
+            a.slotLink[s1] |= SLOT_BROKEN;
+            a.slotOther[s0] |= SLOT_BROKEN;
+            a.slotOther[s1] |= SLOT_BROKEN;
             broke = 1;
         }
     }
@@ -1003,6 +1011,7 @@ ForceArgs<R, CT, FO> force_args(Context &c, const hdm_force_params &fp) {
     a.breakCheck = 0;
     a.linkedContacts = c.nbrExcludesLinked ? 1 : 0;
     a.slotLinkW = c.slotLink.as<int32_t>();
+    a.slotOtherW = c.slotOther.as<int32_t>();
     a.tx = c.tx.as<double>(); a.ty = c.ty.as<double>(); a.tz = c.tz.as<double>();
     a.initialLen = c.initialLen.as<double>();
     a.maxLoad = c.hasMaxLoad ? c.maxLoad.as<double>() : nullptr;
@@ -1118,7 +1127,8 @@ StepRecs<R> step_recs(Context &c, double tame, const hdm_break_params *brk) {
     r.recOut = c.recAlt.as<double4>();
     r.drecOut = c.drecAlt.as<double>();
     r.tame = tame;
-    r.slotBrk = c.slotBrk.as<float2>();
+    r.slotThr = c.slotThr.as<float>();
+    r.l0Max = c.l0Max;
     r.perLinkMaxLoad = c.hasMaxLoad ? 1 : 0;
     r.scalarMaxLoad = brk ? (float)brk->maxLoad : 0.0f;
     r.bendBreak = brk ? (float)brk->bendBreak : 0.0f;
@@ -1263,6 +1273,7 @@ hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counte
     a.initialLen = c.initialLen.as<double>();
     a.maxLoad = c.hasMaxLoad ? c.maxLoad.as<double>() : nullptr;
     a.slotLink = c.slotLink.as<int32_t>();
+    a.slotOther = c.slotOther.as<int32_t>();
     a.sidePos = c.sidePos.as<int32_t>();
     a.L = c.L;
     a.bendBreak = bp.bendBreak;

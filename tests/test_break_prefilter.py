@@ -1,15 +1,37 @@
 """Safety of the fused step's break prefilter (certainly_intact in
-hidem_amd/csrc/kernels_step.hip), restated in float32 NumPy: whenever it
-declares a link intact, the exact FP64 HiDEMDefLinkBreakFunc
-(HiDEMDefaultFunctions.h:217-241) does not break it.  Links are drawn around
-the threshold (where the float estimate is least certain) and at extreme
-magnitudes; the device sqrtf is allowed to be off by 2 ulp either way."""
+hidem_amd/csrc/kernels_step.hip, thresholds from k_side_thresholds in
+kernels_links.hip), restated in float32 NumPy: whenever it declares a link
+intact, the exact FP64 HiDEMDefLinkBreakFunc (HiDEMDefaultFunctions.h:217-241)
+does not break it.  Links are drawn around the threshold (where the float
+estimate is least certain) and at extreme magnitudes, with per-link and
+scalar maxLoad; the device sqrtf is allowed to be off by 2 ulp either way."""
 import numpy as np
 
 f32 = np.float32
 
 
-def prefilter_intact(L0f, Mf, bbf, dx, dy, dz, rot, sqrt_ulps=0):
+def side_threshold(L0, M):
+    """k_side_thresholds: float(L0 (1 + maxLoad)) per link (M None: float(L0)),
+    NaN when L0 < 1e-30 or not finite."""
+    with np.errstate(all="ignore"):
+        ok = (L0 >= 1e-30) & (L0 <= 1e300)
+        T = L0 * (1.0 + M) if M is not None else L0
+        return np.where(ok, T, np.nan).astype(f32)
+
+
+def l0_max(L0):
+    """Largest L0 rounded up to float; inf when some L0 is NaN or huge."""
+    with np.errstate(all="ignore"):
+        ok = (L0 >= 1e-30) & (L0 <= 1e300)
+        bad = ~ok & ~(L0 < 1e-30)
+        if bad.any():
+            return f32(np.inf)
+        f = L0[ok].astype(f32)
+        up = np.where(f.astype(np.float64) < L0[ok], np.nextafter(f, f32(np.inf)), f)
+        return up.max() if up.size else f32(0)
+
+
+def prefilter_intact(thr, scalarM, bbf, l0max, dx, dy, dz, rot, sqrt_ulps=0):
     fx, fy, fz = dx.astype(f32), dy.astype(f32), dz.astype(f32)
     with np.errstate(all="ignore"):
         s = np.sqrt(fx * fx + fy * fy + fz * fz)
@@ -17,11 +39,10 @@ def prefilter_intact(L0f, Mf, bbf, dx, dy, dz, rot, sqrt_ulps=0):
             s = np.nextafter(s, np.where(sqrt_ulps > 0, f32(np.inf), f32(-np.inf)))
             if abs(sqrt_ulps) > 1:
                 s = np.nextafter(s, np.where(sqrt_ulps > 0, f32(np.inf), f32(-np.inf)))
-        bend = bbf * rot * L0f
-        lhs = (s - L0f) + bend
-        rhs = Mf * L0f
-        margin = f32(1e-5) * (s + L0f + bend + np.abs(rhs) + np.abs(lhs))
-        return (L0f >= f32(1e-30)) & (lhs + margin < rhs)
+        T = thr if scalarM is None else thr * (f32(1) + f32(scalarM))
+        lhs = s + bbf * rot * l0max
+        margin = f32(1e-5) * (lhs + np.abs(T))
+        return lhs + margin < T
 
 
 def exact_breaks(L0, M, bb, dx, dy, dz, tp, tq):
@@ -39,14 +60,23 @@ def rot_bound(t):
 
 
 def check(L0, M, bb, dx, dy, dz, tp, tq):
+    """Per-link maxLoad M (array) and, with the same values as one scalar
+    each, the scalar-maxLoad form; returns the per-link verdicts."""
     brk = exact_breaks(L0, M, bb, dx, dy, dz, tp, tq)
     rot = rot_bound(tp) + rot_bound(tq)
+    lm = l0_max(L0)
     ok = None
     for u in (-2, 0, 2):
-        intact = prefilter_intact(L0.astype(f32), M.astype(f32), f32(bb), dx, dy, dz, rot, u)
+        intact = prefilter_intact(side_threshold(L0, M), None, f32(bb), lm, dx, dy, dz, rot, u)
         bad = intact & brk
         assert not bad.any(), (u, np.nonzero(bad)[0][:5])
         ok = intact if ok is None else ok & intact
+    for m in np.unique(M)[:: max(1, len(np.unique(M)) // 5)]:
+        brk_s = exact_breaks(L0, np.full_like(L0, m), bb, dx, dy, dz, tp, tq)
+        for u in (-2, 0, 2):
+            intact = prefilter_intact(side_threshold(L0, None), m, f32(bb), lm, dx, dy, dz, rot, u)
+            bad = intact & brk_s
+            assert not bad.any(), ("scalar", m, u, np.nonzero(bad)[0][:5])
     return ok, brk
 
 
@@ -95,5 +125,25 @@ def test_prefilter_extremes():
     tq = np.zeros((n, 3))
     M = np.array([1e-3, 1e-3, 1e-3, 1e-3, 1e-3, np.nan])
     rot = rot_bound(tp) + rot_bound(tq)
-    intact = prefilter_intact(L0.astype(f32), M.astype(f32), f32(0.35), d[:, 0], d[:, 1], d[:, 2], rot)
+    intact = prefilter_intact(side_threshold(L0, M), None, f32(0.35), l0_max(L0), d[:, 0], d[:, 1], d[:, 2], rot)
     assert not intact.any()
+
+
+def test_prefilter_tight_threshold():
+    """No rotation and no bend term: the float estimate alone separates the
+    links, so the margin is what keeps links within rounding of the
+    threshold on the exact path."""
+    r = np.random.default_rng(2)
+    n = 400_000
+    L0 = r.uniform(0.3, 2.0, n)
+    M = r.uniform(1e-4, 6e-4, n)
+    S = L0 * (1 + M) * (1 + r.uniform(-3e-7, 3e-7, n))
+    S[: n // 10] = (L0 * (1 + M) * (1 - r.uniform(1e-4, 1e-2, n)))[: n // 10]
+    d = r.normal(0, 1, (n, 3))
+    d *= (S / np.linalg.norm(d, axis=1))[:, None]
+    t = np.zeros((n, 3))
+    intact, brk = check(L0, M, 0.0, d[:, 0], d[:, 1], d[:, 2], t, t)
+    assert brk.sum() > n // 4
+    # links 1e-4 (relative) below the threshold pass
+    far = S < L0 * (1 + M) * (1 - 1e-4)
+    assert far.sum() == n // 10 and intact[far].all()
