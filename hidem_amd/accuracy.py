@@ -16,7 +16,10 @@ link strain) and reports, at each checkpoint step:
 With --perturb, a third run (FP64, every coordinate moved by one ulp towards
 +inf at step 0) is compared with the FP64 run the same way: the divergence a
 one-ulp difference grows into, i.e. the floor below which no change of
-arithmetic can stay once the dynamics amplify differences (fracture).
+arithmetic can stay once the dynamics amplify differences (fracture).  With
+--perturb-mixed, a mixed run from the same perturbed coordinates is compared
+with the mixed run: the spread of the mixed form itself, which tells a
+systematic difference between the forms from the chaos of one realisation.
 
 Usage: python -m hidem_amd.accuracy [--config glacier_8m] [--checkpoints 100 1000 10000]
 Prints one JSON line per checkpoint and a final summary line.
@@ -72,6 +75,8 @@ def main(argv=None):
     ap.add_argument("--checkpoints", type=int, nargs="+", default=[100, 1000, 10000])
     ap.add_argument("--chunk", type=int, default=500, help="steps per hdm_run call between progress lines")
     ap.add_argument("--perturb", action="store_true", help="also run FP64 from one-ulp perturbed coordinates")
+    ap.add_argument("--perturb-mixed", action="store_true",
+                    help="also run the mixed form from one-ulp perturbed coordinates")
     args = ap.parse_args(argv)
 
     cfg = params.load_config(args.config)
@@ -81,17 +86,19 @@ def main(argv=None):
     out = []
     t0 = time.perf_counter()
     ctxs = [A.Context(A.HDM_PREC_FP64), A.Context(A.HDM_PREC_MIXED)]
+    role = {}
     if args.perturb:
+        role["fp64_ulp_perturbed"] = len(ctxs)
         ctxs.append(A.Context(A.HDM_PREC_FP64))
+    if args.perturb_mixed:
+        role["mixed_ulp_perturbed"] = len(ctxs)
+        ctxs.append(A.Context(A.HDM_PREC_MIXED))
+    q = p.copy()
+    for f in ("xCord", "yCord", "zCord"):
+        q.arrays[f] = np.nextafter(q[f], np.inf)
     try:
         for i, c in enumerate(ctxs):
-            if i == 2:
-                q = p.copy()
-                for f in ("xCord", "yCord", "zCord"):
-                    q.arrays[f] = np.nextafter(q[f], np.inf)
-                c.upload(q, l)
-            else:
-                c.upload(p, l)
+            c.upload(q if i >= 2 else p, l)
             c.links_build()
         done = 0
         rebuilds = [0] * len(ctxs)
@@ -105,10 +112,13 @@ def main(argv=None):
                 print(f"# step {done}  ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
             p64 = ctxs[0].download_particles()
             b64 = ctxs[0].count_broken()
-            r = compare(p64, ctxs[1].download_particles(), x0, b64, ctxs[1].count_broken())
+            pmx = ctxs[1].download_particles()
+            bmx = ctxs[1].count_broken()
+            r = compare(p64, pmx, x0, b64, bmx)
             r.update(step=cp, rebuilds_fp64=rebuilds[0], rebuilds_mixed=rebuilds[1])
-            if args.perturb:
-                r["fp64_ulp_perturbed"] = compare(p64, ctxs[2].download_particles(), x0, b64, ctxs[2].count_broken())
+            for name, i in role.items():
+                base_p, base_b = (p64, b64) if name.startswith("fp64") else (pmx, bmx)
+                r[name] = compare(base_p, ctxs[i].download_particles(), x0, base_b, ctxs[i].count_broken())
             out.append(r)
             print(json.dumps(r), flush=True)
     finally:

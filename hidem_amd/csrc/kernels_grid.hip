@@ -217,15 +217,11 @@ struct NeighArgs {
     const uint32_t *linkCount;
     const int32_t *slotOther;
     int32_t linkWidth;
+    int32_t partnersInLds;  // copy each lane's partners to LDS first (linkWidth * TPB words)
 };
 
-__device__ __forceinline__ bool is_link_partner(const NeighArgs &a, int64_t p, int32_t q) {
-    const uint32_t cnt = a.linkCount[p];
-    const int64_t base = (p >> 6) * (int64_t)64 * a.linkWidth + (p & 63);
-    for (uint32_t k = 0; k < cnt; k++)
-        if ((a.slotOther[base + (int64_t)k * 64] & SLOT_LINK_MASK) == q) return true;
-    return false;
-}
+// Largest link-slot width whose partner lists are staged in LDS (32 KiB).
+constexpr int32_t NEIGH_LDS_WIDTH = 32;
 
 template <bool FILL>
 __global__ void __launch_bounds__(TPB) k_neigh(const double *__restrict__ x, const double *__restrict__ y,
@@ -234,8 +230,26 @@ __global__ void __launch_bounds__(TPB) k_neigh(const double *__restrict__ x, con
                                                 const int32_t *__restrict__ cellStart,
                                                 const int32_t *__restrict__ order, NeighArgs a,
                                                 int32_t *__restrict__ count, int32_t *__restrict__ list) {
+    extern __shared__ int32_t ldsPartners[];  // [linkWidth][TPB]
     int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
     if (p >= n) return;
+    // p's link partners (list exclusion): from LDS when staged, else global
+    uint32_t pcnt = 0;
+    const int64_t pbase = (p >> 6) * (int64_t)64 * a.linkWidth + (p & 63);
+    if (a.linkCount) {
+        pcnt = a.linkCount[p];
+        if (a.partnersInLds)
+            for (uint32_t k = 0; k < pcnt; k++)
+                ldsPartners[k * TPB + threadIdx.x] = a.slotOther[pbase + (int64_t)k * 64] & SLOT_LINK_MASK;
+    }
+    auto is_partner = [&](int32_t q) {
+        for (uint32_t k = 0; k < pcnt; k++) {
+            const int32_t v = a.partnersInLds ? ldsPartners[k * TPB + threadIdx.x]
+                                              : (a.slotOther[pbase + (int64_t)k * 64] & SLOT_LINK_MASK);
+            if (v == q) return true;
+        }
+        return false;
+    };
     const double xCord = x[p], yCord = y[p], zCord = z[p], rad = radius[p];
     const int col = colour[p];
     int32_t neighbours = 0;
@@ -259,7 +273,7 @@ __global__ void __launch_bounds__(TPB) k_neigh(const double *__restrict__ x, con
             double radSum = rad + radius[nInd];
             double dist = particleDistance - radSum;
             if (dist < a.interactDist) {
-                if (a.linkCount && is_link_partner(a, p, nInd)) continue;
+                if (pcnt && is_partner(nInd)) continue;
                 if (FILL) {
                     if (neighbours < a.width) out[(int64_t)neighbours * 64] = nInd;
                 }
@@ -360,13 +374,18 @@ static NeighArgs neigh_args(Context &c, double interactDist) {
     a.linkCount = exclude ? c.linkCount.as<uint32_t>() : nullptr;
     a.slotOther = exclude ? c.slotOther.as<int32_t>() : nullptr;
     a.linkWidth = c.maxLinks;
+    a.partnersInLds = exclude && c.maxLinks <= NEIGH_LDS_WIDTH ? 1 : 0;
     return a;
+}
+
+static size_t neigh_lds(const NeighArgs &a) {
+    return a.partnersInLds ? sizeof(int32_t) * (size_t)a.linkWidth * TPB : 0;
 }
 
 hipError_t launch_neigh_count(Context &c, double interactDist) {
     if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     NeighArgs a = neigh_args(c, interactDist);
-    hipLaunchKernelGGL(k_neigh<false>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
+    hipLaunchKernelGGL(k_neigh<false>, dim3(blocks_for(c.n)), dim3(TPB), neigh_lds(a), c.stream, c.x.as<double>(),
                        c.y.as<double>(), c.z.as<double>(), c.radius.as<double>(), c.n, c.colour.as<int32_t>(),
                        c.cellStart.as<int32_t>(), c.order.as<int32_t>(), a, c.nbrCount.as<int32_t>(),
                        (int32_t *)nullptr);
@@ -376,7 +395,7 @@ hipError_t launch_neigh_count(Context &c, double interactDist) {
 hipError_t launch_neigh_fill(Context &c, double interactDist) {
     if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     NeighArgs a = neigh_args(c, interactDist);
-    hipLaunchKernelGGL(k_neigh<true>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
+    hipLaunchKernelGGL(k_neigh<true>, dim3(blocks_for(c.n)), dim3(TPB), neigh_lds(a), c.stream, c.x.as<double>(),
                        c.y.as<double>(), c.z.as<double>(), c.radius.as<double>(), c.n, c.colour.as<int32_t>(),
                        c.cellStart.as<int32_t>(), c.order.as<int32_t>(), a, (int32_t *)nullptr,
                        c.nbr.as<int32_t>());

@@ -136,23 +136,32 @@ __global__ void __launch_bounds__(TPB) k_side_thresholds(const int32_t *__restri
                                                           const double *__restrict__ initialLen,
                                                           const double *__restrict__ maxLoad,
                                                           float *__restrict__ slotThr, uint64_t *l0MaxBits) {
-    int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    __shared__ uint32_t part[TPB / 64];
     float big = 0.0f;
-    if (i < nSides) {
-        int64_t l = i >> 1;
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < nSides; i += (int64_t)gridDim.x * TPB) {
+        const int64_t l = i >> 1;
         const double L0 = initialLen[l];
         const bool ok = L0 >= 1e-30 && L0 <= 1e300;
         const double T = maxLoad ? L0 * (1.0 + maxLoad[l]) : L0;
         slotThr[sidePos[i]] = ok ? (float)T : __builtin_nanf("");
+        float b = 0.0f;
         if (ok) {
             const float f = (float)L0;
-            big = (double)f < L0 ? __uint_as_float(__float_as_uint(f) + 1u) : f;  // next float up
+            b = (double)f < L0 ? __uint_as_float(__float_as_uint(f) + 1u) : f;  // next float up
         } else if (!(L0 < 1e-30)) {
-            big = __builtin_inff();  // NaN or huge: no useful bound
+            b = __builtin_inff();  // NaN or huge: no useful bound
         }
+        big = b > big ? b : big;
     }
-    const uint32_t bits = wave_max_u32(__float_as_uint(big));  // non-negative floats order as integers
-    if ((threadIdx.x & 63) == 0 && bits) atomicMax((unsigned long long *)l0MaxBits, (unsigned long long)bits);
+    // non-negative floats order as their bit patterns
+    const uint32_t bits = wave_max_u32(__float_as_uint(big));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = bits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = part[0];
+        for (int w = 1; w < TPB / 64; w++) m = part[w] > m ? part[w] : m;
+        if (m) atomicMax((unsigned long long *)l0MaxBits, (unsigned long long)m);
+    }
 }
 
 // Flags a particle whose link sides name one partner twice, or itself.
@@ -253,7 +262,9 @@ hipError_t launch_side_thresholds(Context &c, uint64_t *l0MaxBits) {
     hipError_t e = launch_fill_u64(c.stream, l0MaxBits, 0ull, 1);
     int64_t nSides = 2 * c.L;
     if (e != hipSuccess || nSides == 0) return e;
-    hipLaunchKernelGGL(k_side_thresholds, dim3(blocks_for(nSides)), dim3(TPB), 0, c.stream, c.sidePos.as<int32_t>(),
+    const int64_t nb = blocks_for(nSides);
+    const int cap = c.numCU > 0 ? c.numCU * 8 : 2048;
+    hipLaunchKernelGGL(k_side_thresholds, dim3((unsigned)(nb < cap ? nb : cap)), dim3(TPB), 0, c.stream, c.sidePos.as<int32_t>(),
                        nSides, c.initialLen.as<double>(), c.hasMaxLoad ? c.maxLoad.as<double>() : nullptr,
                        c.slotThr.as<float>(), l0MaxBits);
     return hipGetLastError();

@@ -731,8 +731,12 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
         {
             const int32_t cnt = ld_stream(f.nbrCount + p);
             const int32_t *nl = f.nbr + (p >> 6) * (int64_t)64 * f.nbrWidth + lane;
+            // each entry's index is loaded one iteration ahead, so only the
+            // record gather waits inside the iteration
+            int32_t qNext = cnt > 0 ? ld_stream(nl) : 0;
             for (int32_t k = 0; k < cnt; k++) {
-                const int32_t q = ld_stream(nl + (int64_t)k * 64);
+                const int32_t q = qNext;
+                if (k + 1 < cnt) qNext = ld_stream(nl + (int64_t)(k + 1) * 64);
                 const double4 Q = r.rec[q];
                 const R xDist = (R)(xCord - Q.x);
                 const R yDist = (R)(yCord - Q.y);
@@ -760,9 +764,11 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
         {
             const int32_t cnt = (int32_t)ld_stream(f.linkCount + p);
             const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + lane;
+            int32_t owNext = cnt > 0 ? ld_stream(f.slotOther + base) : 0;  // one side ahead, as above
             for (int32_t k = 0; k < cnt; k++) {
                 const int64_t sidx = base + (int64_t)k * 64;
-                const int32_t ow = ld_stream(f.slotOther + sidx);  // partner; sign bit: broken
+                const int32_t ow = owNext;  // partner; sign bit: broken
+                if (k + 1 < cnt) owNext = ld_stream(f.slotOther + sidx + 64);
                 const int32_t q = ow & SLOT_LINK_MASK;
                 const double4 Q = r.rec[q];
                 double od[6];
