@@ -192,7 +192,7 @@ struct Context {
 // the ~12 record gathers of a lattice particle's linked neighbours, which are
 // inert contacts while the link holds.
 inline bool merge_linked_contacts(const Context &c) {
-    return c.prec == HDM_PREC_MIXED && c.linksBuilt && c.partnersUnique && c.mergeContacts;
+    return c.prec == HDM_PREC_MIXED && c.linksBuilt && c.partnersUnique && c.mergeContacts && c.maxLinks <= 64;
 }
 
 // Link slots hold the link id; the sign bit marks a broken link (state == 0).

@@ -309,11 +309,6 @@ __device__ __forceinline__ void force_particle(const ForceArgs<R, CT, FO> &a, in
             R xDist = (R)(xCord - a.x[q]);
             R yDist = (R)(yCord - a.y[q]);
             R zDist = (R)(zCord - a.z[q]);
-            if (sizeof(R) == 4 && a.linkedContacts) {
-                R oc[7] = {other[0], other[1], other[2], other[3], other[4], other[5], (R)a.radius[q]};
-                if (!contact_is_inert<R>(xDist, yDist, zDist, self, oc, d, a.fp))
-                    contact_law<R>(R(1), xDist, yDist, zDist, self, oc, d, a.fp, F);
-            }
             R mask = 1;
             if (ow < 0) mask = 0;  // broken link (state == 0)
             R ct[6] = {(R)a.c0[sidx], (R)a.c1[sidx], (R)a.c2[sidx], (R)a.c3[sidx], (R)a.c4[sidx], (R)a.c5[sidx]};
@@ -324,6 +319,22 @@ __device__ __forceinline__ void force_particle(const ForceArgs<R, CT, FO> &a, in
             a.c3[sidx] = (CT)ct[3];
             a.c4[sidx] = (CT)ct[4];
             a.c5[sidx] = (CT)ct[5];
+        }
+        // the contacts of linked pairs (list without link partners), after
+        // every beam, in side order (as the fused step)
+        if (sizeof(R) == 4 && a.linkedContacts) {
+            for (int32_t k = 0; k < cnt; k++) {
+                const int32_t q = a.slotOther[base + (int64_t)k * 64] & SLOT_LINK_MASK;
+                const double od[6] = {a.d0[q], a.d1[q], a.d2[q], a.d3[q], a.d4[q], a.d5[q]};
+                R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], (R)a.radius[q]};
+                R d[6];
+                diff_delta<R>(sd, od, d);
+                R xDist = (R)(xCord - a.x[q]);
+                R yDist = (R)(yCord - a.y[q]);
+                R zDist = (R)(zCord - a.z[q]);
+                if (!contact_is_inert<R>(xDist, yDist, zDist, self, other, d, a.fp))
+                    contact_law<R>(R(1), xDist, yDist, zDist, self, other, d, a.fp, F);
+            }
         }
     }
 }
@@ -683,6 +694,29 @@ __device__ __forceinline__ float2 ld_stream(const float2 *p) {
     return r;
 }
 
+// Contacts of the linked pairs flagged in `sides` (bit k: side k's pair is
+// not provably inert), after every beam, in side order: outside the link
+// loop, so that loop does not carry the contact law's registers.
+template <class R, class CT>
+__device__ __forceinline__ void contact_sides(const ForceArgs<R, CT, R> &f, const StepRecs<R> &r,
+                                                        uint64_t sides, int64_t base, double xCord, double yCord,
+                                                        double zCord, const double diff[6], const R self[7],
+                                                        R F[6]) {
+    while (sides) {
+        const int k = __builtin_ctzll(sides);
+        sides &= sides - 1;
+        const int32_t q = f.slotOther[base + (int64_t)k * 64] & SLOT_LINK_MASK;
+        const double4 Q = r.rec[q];
+        double od[6];
+        float rot;
+        load_drec(r.drec, q, od, rot);
+        R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], (R)f.radius[q]};
+        R d[6];
+        diff_delta<R>(diff, od, d);
+        contact_law<R>(R(1), (R)(xCord - Q.x), (R)(yCord - Q.y), (R)(zCord - Q.z), self, other, d, f.fp, F);
+    }
+}
+
 // One timestep of hdm_run in one launch: the link break of the previous
 // timestep (when f.breakCheck), force on particle p from the current state
 // (CombinedAccel::iterateCombined, as k_force), then its integration (as
@@ -765,6 +799,7 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
             const int32_t cnt = (int32_t)ld_stream(f.linkCount + p);
             const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + lane;
             int32_t owNext = cnt > 0 ? ld_stream(f.slotOther + base) : 0;  // one side ahead, as above
+            uint64_t activeSides = 0;  // linked pairs in contact (FP32, merged list; cnt <= 64)
             for (int32_t k = 0; k < cnt; k++) {
                 const int64_t sidx = base + (int64_t)k * 64;
                 const int32_t ow = owNext;  // partner; sign bit: broken
@@ -779,17 +814,16 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                 R d[6];
                 diff_delta<R>(diff, od, d);
                 if (sizeof(R) == 4 && f.linkedContacts) {
-                    // the pair's contact, with the contact loop's record test
-                    // (FP32 force only: merge_linked_contacts)
+                    // the pair's contact test (the contact loop's record test);
+                    // the rare active ones are applied after the loop, which
+                    // keeps the contact law out of this loop's registers
                     const R xDist = (R)dx, yDist = (R)dy, zDist = (R)dz;
                     const R particleDistance2 = xDist * xDist + yDist * yDist + zDist * zDist;
                     const R radSum = (R)P.w + (R)Q.w;
                     const R activeDist = radSum + f.fp.attractDistance;
                     if (!(particleDistance2 >= activeDist * activeDist && particleDistance2 <= huge_value<R>() &&
-                          particleDistance2 > R(0))) {
-                        R oc[7] = {other[0], other[1], other[2], other[3], other[4], other[5], (R)f.radius[q]};
-                        contact_law<R>(R(1), xDist, yDist, zDist, self, oc, d, f.fp, F);
-                    }
+                          particleDistance2 > R(0)))
+                        activeSides |= 1ull << k;
                 }
                 R mask = 1;
                 if (ow < 0) {
@@ -821,6 +855,7 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                 st_stream(f.c4 + sidx, (CT)ct[4]);
                 st_stream(f.c5 + sidx, (CT)ct[5]);
             }
+            if (sizeof(R) == 4 && activeSides) contact_sides(f, r, activeSides, base, xCord, yCord, zCord, diff, self, F);
         }
         if (r.storeForce) store_force(f, p, F);
         const double Fd[6] = {(double)F[0], (double)F[1], (double)F[2], (double)F[3], (double)F[4], (double)F[5]};
