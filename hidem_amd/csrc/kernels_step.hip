@@ -798,12 +798,10 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
         {
             const int32_t cnt = (int32_t)ld_stream(f.linkCount + p);
             const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + lane;
-            int32_t owNext = cnt > 0 ? ld_stream(f.slotOther + base) : 0;  // one side ahead, as above
             uint64_t activeSides = 0;  // linked pairs in contact (FP32, merged list; cnt <= 64)
             for (int32_t k = 0; k < cnt; k++) {
                 const int64_t sidx = base + (int64_t)k * 64;
-                const int32_t ow = owNext;  // partner; sign bit: broken
-                if (k + 1 < cnt) owNext = ld_stream(f.slotOther + sidx + 64);
+                const int32_t ow = ld_stream(f.slotOther + sidx);  // partner; sign bit: broken
                 const int32_t q = ow & SLOT_LINK_MASK;
                 const double4 Q = r.rec[q];
                 double od[6];
@@ -813,18 +811,6 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                 R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], R(0)};  // no radius needed
                 R d[6];
                 diff_delta<R>(diff, od, d);
-                if (sizeof(R) == 4 && f.linkedContacts) {
-                    // the pair's contact test (the contact loop's record test);
-                    // the rare active ones are applied after the loop, which
-                    // keeps the contact law out of this loop's registers
-                    const R xDist = (R)dx, yDist = (R)dy, zDist = (R)dz;
-                    const R particleDistance2 = xDist * xDist + yDist * yDist + zDist * zDist;
-                    const R radSum = (R)P.w + (R)Q.w;
-                    const R activeDist = radSum + f.fp.attractDistance;
-                    if (!(particleDistance2 >= activeDist * activeDist && particleDistance2 <= huge_value<R>() &&
-                          particleDistance2 > R(0)))
-                        activeSides |= 1ull << k;
-                }
                 R mask = 1;
                 if (ow < 0) {
                     mask = 0;  // broken link (state == 0)
@@ -848,6 +834,20 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                 R ct[6] = {(R)ld_stream(f.c0 + sidx), (R)ld_stream(f.c1 + sidx), (R)ld_stream(f.c2 + sidx),
                            (R)ld_stream(f.c3 + sidx), (R)ld_stream(f.c4 + sidx), (R)ld_stream(f.c5 + sidx)};
                 beam_law<R>(mask, (R)dx, (R)dy, (R)dz, self, other, d, f.fp, ct, F);
+                if (sizeof(R) == 4 && f.linkedContacts) {
+                    // the pair's contact test (the contact loop's record test);
+                    // the rare active ones are applied after the loop, which
+                    // keeps the contact law out of this loop's registers
+                    // (placed after the beam law, where it does not delay the
+                    // side's loads)
+                    const R xDist = (R)dx, yDist = (R)dy, zDist = (R)dz;
+                    const R particleDistance2 = xDist * xDist + yDist * yDist + zDist * zDist;
+                    const R radSum = (R)P.w + (R)Q.w;
+                    const R activeDist = radSum + f.fp.attractDistance;
+                    if (!(particleDistance2 >= activeDist * activeDist && particleDistance2 <= huge_value<R>() &&
+                          particleDistance2 > R(0)))
+                        activeSides |= 1ull << k;
+                }
                 st_stream(f.c0 + sidx, (CT)ct[0]);
                 st_stream(f.c1 + sidx, (CT)ct[1]);
                 st_stream(f.c2 + sidx, (CT)ct[2]);
