@@ -979,10 +979,7 @@ int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t 
             bool done = true;
             for (int v = 0; v < hdm::STEP_VARIANTS; v++) done &= c.tuneCount[v] >= 2;
             if (done) {
-                int best = 0;
-                for (int v = 1; v < hdm::STEP_VARIANTS; v++)
-                    if (c.tuneMs[v] < c.tuneMs[best]) best = v;
-                c.stepVariant = best;
+                c.stepVariant = hdm::pick_step_variant(c.tuneMs);
                 c.tuned = true;
             } else {
                 c.stepVariant = 0;

@@ -248,6 +248,8 @@ hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *sk
 // Launch shapes of the fused step (autotuner candidates).
 constexpr int STEP_VARIANTS = 7;
 const char *step_variant_name(int v);
+// Autotuner choice from the per-shape step times (ms).
+int pick_step_variant(const double ms[STEP_VARIANTS]);
 
 // Guard words in Context::guardState.
 enum { GUARD_TICKET = 0, GUARD_FLAG = 1, GUARD_STEPS = 2, GUARD_WORDS = 4 };
