@@ -62,6 +62,13 @@ def step_bytes(n, n_neigh_total, n_links, prec, exact=True):
     return n * per_particle + 4 * n_neigh_total + 2 * n_links * per_side + 16 * n_links
 
 
+def reference_list_entries(n_list, n_links, exec_info):
+    """Entries of the reference's neighbour list: the mixed form's list leaves
+    out link partners (their contacts run in the link loop), which the
+    reference lists, two entries per link."""
+    return n_list + (2 * n_links if exec_info["merged_contacts"] else 0)
+
+
 def break_bytes(n_links):
     """Compulsory bytes of the standalone k_break: A, B, state, initialLen,
     maxLoad per link (endpoint coordinates come through the cache)."""
@@ -113,11 +120,7 @@ def run_gpu(setup, p, l, prec, policy, model, steps, warmup):
         s1 = ctx.sched_stats()
         exec_info = ctx.exec_info()
         counts, _, _ = ctx.neighbours_download()
-        n_neigh = int(counts.sum())
-        if exec_info["merged_contacts"]:
-            # the list leaves out link partners (their contacts run in the link
-            # loop); count the reference list's entries, two per link
-            n_neigh += 2 * l.n
+        n_neigh = reference_list_entries(int(counts.sum()), l.n, exec_info)
         broken = ctx.count_broken()
         dev_bytes = ctx.device_bytes()
     sched = {k: s1[k] - s0[k] for k in s1}
@@ -157,9 +160,10 @@ def probe_traffic(setup, p, l, forms):
             log(f"launch shape {ctx.exec_info()['name']}")
             ctx.sync()
             counts, _, _ = ctx.neighbours_download()
-            b = step_bytes(p.n, int(counts.sum()), l.n, prec)
+            n_neigh = reference_list_entries(int(counts.sum()), l.n, ctx.exec_info())
+            b = step_bytes(p.n, n_neigh, l.n, prec)
             print(json.dumps({"precision": "fp64" if prec == A.HDM_PREC_FP64 else "mixed",
-                              "step_algorithmic_bytes": b, "neighbours": int(counts.sum()), "links": l.n}),
+                              "step_algorithmic_bytes": b, "neighbours": n_neigh, "links": l.n}),
                   flush=True)
 
 
