@@ -55,13 +55,15 @@ ForceParams<R> cast_params(const hdm_force_params &f) {
 }
 
 // Diff differences of a pair, s - o: in the FP64 form exactly the law's own
-// (self[c] - other[c]); in the FP32 form the FP64 difference rounded once,
-// because neighbouring Diffs are nearly equal and rounding each to FP32 first
-// would cancel most of the significant bits of their difference.
+// (self[c] - other[c]).  In the FP32 form the partner's Diff is the FP32 value
+// its gather record carries, and the difference with the particle's own FP64
+// Diff is formed in FP64 and rounded once: neighbouring Diffs are nearly
+// equal, and rounding both to FP32 before subtracting would cancel most of the
+// significant bits of their difference.
 template <class R>
 __device__ __forceinline__ void diff_delta(const double s[6], const double o[6], R d[6]) {
 #pragma unroll
-    for (int c = 0; c < 6; c++) d[c] = (R)(s[c] - o[c]);
+    for (int c = 0; c < 6; c++) d[c] = (R)(s[c] - (sizeof(R) == 4 ? (double)(float)o[c] : o[c]));
 }
 
 // HiDEMDefNeighForceFunc::operator() (HiDEMDefaultFunctions.h:38-86); the
@@ -612,12 +614,27 @@ __global__ void __launch_bounds__(TPB) k_integrate(IntegArgs a, const FO *__rest
 }
 
 // ---- fused step ------------------------------------------------------------------
+// Gather record of a particle in the FP32 form: coordinates in FP64, Diff
+// rounded to FP32, the rotation bound and the radius (NaN when not tame), in
+// one 64 B line: a link partner costs four 16 B loads.  (The FP64 form gathers
+// rec = double4 {x, y, z, radius} and the FP64 Diff from drec.)
+struct __attribute__((aligned(16))) MixedRec {
+    double x, y, z;
+    float d[6];
+    float rot, radius;
+};
+template <class R> struct RecOf { using type = double4; };
+template <> struct RecOf<float> { using type = MixedRec; };
+__device__ __forceinline__ double rec_radius(const double4 &P) { return P.w; }
+__device__ __forceinline__ float rec_radius(const MixedRec &P) { return P.radius; }
+
 // Gather records (see Context::rec / drec) and the break prefilter inputs.
 template <class R>
 struct StepRecs {
-    const double4 *rec;        // current state
+    using Rec = typename RecOf<R>::type;
+    const Rec *rec;            // current state (drec: every particle's FP64 Diff)
     const double *drec;
-    double4 *recOut;           // integrated state
+    Rec *recOut;               // integrated state
     double *drecOut;
     double tame;               // Diff bound behind the radius flag of rec
     // Break prefilter inputs: per side a float threshold, L0 (1 + maxLoad)
@@ -650,10 +667,43 @@ __device__ __forceinline__ void store_records(const StepRecs<R> &r, int64_t p, c
     bool tame = fabs(radius) <= 5e5;
 #pragma unroll
     for (int c = 0; c < 6; c++) tame &= fabs(diff[c]) <= r.tame;
-    r.recOut[p] = make_double4(cord[0], cord[1], cord[2], tame ? radius : __builtin_nan(""));
+    if constexpr (sizeof(R) == 4) {
+        MixedRec m;
+        m.x = cord[0];
+        m.y = cord[1];
+        m.z = cord[2];
+#pragma unroll
+        for (int c = 0; c < 6; c++) m.d[c] = (float)diff[c];
+        m.rot = rotation_bound(theta);
+        m.radius = tame ? (float)radius : __builtin_nanf("");
+        r.recOut[p] = m;
+    } else {
+        r.recOut[p] = make_double4(cord[0], cord[1], cord[2], tame ? radius : __builtin_nan(""));
+    }
     double4 *d = reinterpret_cast<double4 *>(r.drecOut + p * 8);
     d[0] = make_double4(diff[0], diff[1], diff[2], diff[3]);
     d[1] = make_double4(diff[4], diff[5], (double)rotation_bound(theta), 0.0);
+}
+
+// A partner's Diff (as the laws take it) and the pair differences, from its
+// gathered record: FP32 from the record itself, FP64 from drec.
+template <class R>
+__device__ __forceinline__ void partner_diff(const StepRecs<R> &r, int32_t q, const typename RecOf<R>::type &Q,
+                                             const double diff[6], R other[6], R d[6], float &rot) {
+    if constexpr (sizeof(R) == 4) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            other[c] = Q.d[c];
+            d[c] = (R)(diff[c] - (double)Q.d[c]);
+        }
+        rot = Q.rot;
+    } else {
+        double od[6];
+        load_drec(r.drec, q, od, rot);
+#pragma unroll
+        for (int c = 0; c < 6; c++) other[c] = (R)od[c];
+        diff_delta<R>(diff, od, d);
+    }
 }
 
 // Break prefilter: true when the link certainly stays intact.  The exact test
@@ -706,13 +756,11 @@ __device__ __forceinline__ void contact_sides(const ForceArgs<R, CT, R> &f, cons
         const int k = __builtin_ctzll(sides);
         sides &= sides - 1;
         const int32_t q = f.slotOther[base + (int64_t)k * 64] & SLOT_LINK_MASK;
-        const double4 Q = r.rec[q];
-        double od[6];
+        const auto Q = r.rec[q];
+        R other[7], d[6];
         float rot;
-        load_drec(r.drec, q, od, rot);
-        R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], (R)f.radius[q]};
-        R d[6];
-        diff_delta<R>(diff, od, d);
+        partner_diff<R>(r, q, Q, diff, other, d, rot);
+        other[6] = (R)f.radius[q];
         contact_law<R>(R(1), (R)(xCord - Q.x), (R)(yCord - Q.y), (R)(zCord - Q.z), self, other, d, f.fp, F);
     }
 }
@@ -753,7 +801,7 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
     const bool active = p < f.n;
     double cordN[3] = {0, 0, 0}, diffN[3] = {0, 0, 0};
     if (active) {
-        const double4 P = r.rec[p];
+        const auto P = r.rec[p];
         const double xCord = P.x, yCord = P.y, zCord = P.z;
         double diff[6];
         float rotSelf;
@@ -771,7 +819,7 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
             for (int32_t k = 0; k < cnt; k++) {
                 const int32_t q = qNext;
                 if (k + 1 < cnt) qNext = ld_stream(nl + (int64_t)(k + 1) * 64);
-                const double4 Q = r.rec[q];
+                const auto Q = r.rec[q];
                 const R xDist = (R)(xCord - Q.x);
                 const R yDist = (R)(yCord - Q.y);
                 const R zDist = (R)(zCord - Q.z);
@@ -780,18 +828,16 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                     // record radius is NaN unless both the Diff (damping terms)
                     // and the radius are within their bounds
                     const R particleDistance2 = xDist * xDist + yDist * yDist + zDist * zDist;
-                    const R radSum = (R)P.w + (R)Q.w;
+                    const R radSum = (R)rec_radius(P) + (R)rec_radius(Q);
                     const R activeDist = radSum + f.fp.attractDistance;
                     if (particleDistance2 >= activeDist * activeDist && particleDistance2 <= huge_value<R>() &&
                         particleDistance2 > R(0))
                         continue;
                 }
-                double od[6];
+                R other[7], d[6];
                 float rotOther;
-                load_drec(r.drec, q, od, rotOther);
-                R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], (R)f.radius[q]};
-                R d[6];
-                diff_delta<R>(diff, od, d);
+                partner_diff<R>(r, q, Q, diff, other, d, rotOther);
+                other[6] = (R)f.radius[q];
                 contact_law<R>(R(1), xDist, yDist, zDist, self, other, d, f.fp, F);
             }
         }
@@ -803,14 +849,12 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                 const int64_t sidx = base + (int64_t)k * 64;
                 const int32_t ow = ld_stream(f.slotOther + sidx);  // partner; sign bit: broken
                 const int32_t q = ow & SLOT_LINK_MASK;
-                const double4 Q = r.rec[q];
-                double od[6];
+                const auto Q = r.rec[q];
+                R other[7], d[6];
                 float rotOther;
-                load_drec(r.drec, q, od, rotOther);
+                partner_diff<R>(r, q, Q, diff, other, d, rotOther);
+                other[6] = R(0);  // no radius needed
                 const double dx = xCord - Q.x, dy = yCord - Q.y, dz = zCord - Q.z;
-                R other[7] = {(R)od[0], (R)od[1], (R)od[2], (R)od[3], (R)od[4], (R)od[5], R(0)};  // no radius needed
-                R d[6];
-                diff_delta<R>(diff, od, d);
                 R mask = 1;
                 if (ow < 0) {
                     mask = 0;  // broken link (state == 0)
@@ -842,7 +886,7 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                     // side's loads)
                     const R xDist = (R)dx, yDist = (R)dy, zDist = (R)dz;
                     const R particleDistance2 = xDist * xDist + yDist * yDist + zDist * zDist;
-                    const R radSum = (R)P.w + (R)Q.w;
+                    const R radSum = (R)rec_radius(P) + (R)rec_radius(Q);
                     const R activeDist = radSum + f.fp.attractDistance;
                     if (!(particleDistance2 >= activeDist * activeDist && particleDistance2 <= huge_value<R>() &&
                           particleDistance2 > R(0)))
@@ -878,13 +922,14 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
 }
 
 // SoA coordinates and Diff from the records (sync_state).
-__global__ void __launch_bounds__(TPB) k_unpack_records(const double4 *__restrict__ rec,
+template <class Rec>
+__global__ void __launch_bounds__(TPB) k_unpack_records(const Rec *__restrict__ rec,
                                                          const double *__restrict__ drec, int64_t n, double *x,
                                                          double *y, double *z, double *d0, double *d1, double *d2,
                                                          double *d3, double *d4, double *d5) {
     const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (p >= n) return;
-    const double4 P = rec[p];
+    const Rec P = rec[p];
     x[p] = P.x;
     y[p] = P.y;
     z[p] = P.z;
@@ -1163,9 +1208,10 @@ double tame_bound(const Context &c, const hdm_force_params &fp) {
 template <class R>
 StepRecs<R> step_recs(Context &c, double tame, const hdm_break_params *brk) {
     StepRecs<R> r;
-    r.rec = c.rec.as<double4>();
+    using Rec = typename StepRecs<R>::Rec;
+    r.rec = c.rec.as<Rec>();
     r.drec = c.drec.as<double>();
-    r.recOut = c.recAlt.as<double4>();
+    r.recOut = c.recAlt.as<Rec>();
     r.drecOut = c.drecAlt.as<double>();
     r.tame = tame;
     r.slotThr = c.slotThr.as<float>();
@@ -1180,8 +1226,9 @@ StepRecs<R> step_recs(Context &c, double tame, const hdm_break_params *brk) {
 
 hipError_t ensure_records(Context &c) {
     const size_t n = (size_t)(c.n > 0 ? c.n : 1);
+    const size_t recBytes = c.prec == HDM_PREC_FP64 ? sizeof(double4) : sizeof(MixedRec);
     for (DevBuf *b : {&c.rec, &c.recAlt}) {
-        hipError_t e = b->ensure(32 * n);
+        hipError_t e = b->ensure(recBytes * n);
         if (e != hipSuccess) return e;
     }
     for (DevBuf *b : {&c.drec, &c.drecAlt}) {
@@ -1214,7 +1261,7 @@ hipError_t launch_pack_records(Context &c, double tame) {
         go(r);
     } else {
         StepRecs<float> r = step_recs<float>(c, tame, nullptr);
-        r.recOut = c.rec.as<double4>();
+        r.recOut = c.rec.as<MixedRec>();
         r.drecOut = c.drec.as<double>();
         go(r);
     }
@@ -1237,10 +1284,15 @@ hipError_t sync_state(Context &c) {
     if (!c.soaStale) return hipSuccess;
     c.soaStale = false;
     if (c.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_unpack_records, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.rec.as<double4>(),
-                       c.drec.as<double>(), c.n, c.x.as<double>(), c.y.as<double>(), c.z.as<double>(),
-                       c.diff[0].as<double>(), c.diff[1].as<double>(), c.diff[2].as<double>(),
-                       c.diff[3].as<double>(), c.diff[4].as<double>(), c.diff[5].as<double>());
+    auto go = [&](auto *rec) {
+        using Rec = typename std::remove_pointer<decltype(rec)>::type;
+        hipLaunchKernelGGL(k_unpack_records<Rec>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, rec,
+                           c.drec.as<double>(), c.n, c.x.as<double>(), c.y.as<double>(), c.z.as<double>(),
+                           c.diff[0].as<double>(), c.diff[1].as<double>(), c.diff[2].as<double>(),
+                           c.diff[3].as<double>(), c.diff[4].as<double>(), c.diff[5].as<double>());
+    };
+    if (c.prec == HDM_PREC_FP64) go(c.rec.as<double4>());
+    else go(c.rec.as<MixedRec>());
     return hipGetLastError();
 }
 
