@@ -97,11 +97,15 @@ struct Context {
     // tx, ty, tz); the other entries are unused
     DevBuf alt[12];
     // gather records of the fused step, ping-pong like the state:
-    //   rec   double4 {x, y, z, radius}; the radius reads NaN when a Diff
-    //         component exceeds recTame (then the contact skip test fails)
-    //   drec  8 x f64: Diff0..5 (FP64 in both forms: pair differences are
-    //         formed before any rounding to FP32), rotation bound
-    //         |xTheta| + |yTheta| + |zTheta| (as a float), 0
+    //   rec   FP64 form: double4 {x, y, z, radius}; FP32 form: MixedRec
+    //         {x, y, z, Diff0..5 in FP32, rotation bound, radius} (64 B); the
+    //         radius reads NaN when a Diff component exceeds recTame (then
+    //         the contact skip test fails)
+    //   drec  FP64 Diff0..5 of every particle; the FP64 form adds the
+    //         rotation bound |xTheta| + |yTheta| + |zTheta| (as a float) and
+    //         a 0 (8 doubles, gathered by partners), the FP32 form stores 6
+    //         (read only by the particle; partners gather its MixedRec:
+    //         coordinates, FP32 Diff, rotation bound, radius flag)
     // The fused step reads and writes only the records for coordinates and
     // Diff: after it the SoA x, y, z and diff[] are stale (soaStale) until
     // sync_state unpacks them.

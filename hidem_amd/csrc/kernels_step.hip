@@ -654,6 +654,22 @@ __device__ __forceinline__ float rotation_bound(const double t[3]) {
     return (float)(fabs(t[0]) + fabs(t[1]) + fabs(t[2]));
 }
 
+// Doubles per particle in drec: the FP64 form's partners gather it (Diff0..5,
+// rotation bound, 0); in the FP32 form only the particle itself reads it
+// (Diff0..5), the rest travelling in its MixedRec.
+template <class R>
+constexpr int64_t drec_stride() {
+    return sizeof(R) == 4 ? 6 : 8;
+}
+inline int64_t drec_stride(int32_t prec) { return prec == HDM_PREC_FP64 ? 8 : 6; }
+
+// A particle's own FP64 Diff in the FP32 form (48 B, three 16 B loads).
+__device__ __forceinline__ void load_own_diff6(const double *drec, int64_t p, double o[6]) {
+    const double2 *v = reinterpret_cast<const double2 *>(drec + p * 6);
+    const double2 a = v[0], b = v[1], c = v[2];
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y; o[4] = c.x; o[5] = c.y;
+}
+
 __device__ __forceinline__ void load_drec(const double *drec, int64_t q, double o[6], float &rot) {
     const double4 *v = reinterpret_cast<const double4 *>(drec + q * 8);
     const double4 a = v[0], b = v[1];
@@ -680,9 +696,16 @@ __device__ __forceinline__ void store_records(const StepRecs<R> &r, int64_t p, c
     } else {
         r.recOut[p] = make_double4(cord[0], cord[1], cord[2], tame ? radius : __builtin_nan(""));
     }
-    double4 *d = reinterpret_cast<double4 *>(r.drecOut + p * 8);
-    d[0] = make_double4(diff[0], diff[1], diff[2], diff[3]);
-    d[1] = make_double4(diff[4], diff[5], (double)rotation_bound(theta), 0.0);
+    if constexpr (sizeof(R) == 4) {
+        double2 *d = reinterpret_cast<double2 *>(r.drecOut + p * 6);
+        d[0] = make_double2(diff[0], diff[1]);
+        d[1] = make_double2(diff[2], diff[3]);
+        d[2] = make_double2(diff[4], diff[5]);
+    } else {
+        double4 *d = reinterpret_cast<double4 *>(r.drecOut + p * 8);
+        d[0] = make_double4(diff[0], diff[1], diff[2], diff[3]);
+        d[1] = make_double4(diff[4], diff[5], (double)rotation_bound(theta), 0.0);
+    }
 }
 
 // A partner's Diff (as the laws take it) and the pair differences, from its
@@ -805,7 +828,12 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
         const double xCord = P.x, yCord = P.y, zCord = P.z;
         double diff[6];
         float rotSelf;
-        load_drec(r.drec, p, diff, rotSelf);
+        if constexpr (sizeof(R) == 4) {
+            load_own_diff6(r.drec, p, diff);
+            rotSelf = P.rot;
+        } else {
+            load_drec(r.drec, p, diff, rotSelf);
+        }
         const double radius = ld_stream(g.radius + p);
         R self[7] = {(R)diff[0], (R)diff[1], (R)diff[2], (R)diff[3], (R)diff[4], (R)diff[5], (R)radius};
         R F[6] = {0, 0, 0, 0, 0, 0};
@@ -922,7 +950,7 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
 }
 
 // SoA coordinates and Diff from the records (sync_state).
-template <class Rec>
+template <class Rec, int STRIDE>
 __global__ void __launch_bounds__(TPB) k_unpack_records(const Rec *__restrict__ rec,
                                                          const double *__restrict__ drec, int64_t n, double *x,
                                                          double *y, double *z, double *d0, double *d1, double *d2,
@@ -933,14 +961,13 @@ __global__ void __launch_bounds__(TPB) k_unpack_records(const Rec *__restrict__ 
     x[p] = P.x;
     y[p] = P.y;
     z[p] = P.z;
-    const double4 *v = reinterpret_cast<const double4 *>(drec + p * 8);
-    const double4 a = v[0], b = v[1];
-    d0[p] = a.x;
-    d1[p] = a.y;
-    d2[p] = a.z;
-    d3[p] = a.w;
-    d4[p] = b.x;
-    d5[p] = b.y;
+    const double *v = drec + p * STRIDE;
+    d0[p] = v[0];
+    d1[p] = v[1];
+    d2[p] = v[2];
+    d3[p] = v[3];
+    d4[p] = v[4];
+    d5[p] = v[5];
 }
 
 // Gather records of the current SoA state (hdm_run entry).
@@ -1232,7 +1259,7 @@ hipError_t ensure_records(Context &c) {
         if (e != hipSuccess) return e;
     }
     for (DevBuf *b : {&c.drec, &c.drecAlt}) {
-        hipError_t e = b->ensure(64 * n);
+        hipError_t e = b->ensure(sizeof(double) * (size_t)drec_stride(c.prec) * n);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -1286,7 +1313,8 @@ hipError_t sync_state(Context &c) {
     if (c.n == 0) return hipSuccess;
     auto go = [&](auto *rec) {
         using Rec = typename std::remove_pointer<decltype(rec)>::type;
-        hipLaunchKernelGGL(k_unpack_records<Rec>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, rec,
+        constexpr int stride = std::is_same<Rec, double4>::value ? 8 : 6;
+        hipLaunchKernelGGL((k_unpack_records<Rec, stride>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, rec,
                            c.drec.as<double>(), c.n, c.x.as<double>(), c.y.as<double>(), c.z.as<double>(),
                            c.diff[0].as<double>(), c.diff[1].as<double>(), c.diff[2].as<double>(),
                            c.diff[3].as<double>(), c.diff[4].as<double>(), c.diff[5].as<double>());

@@ -429,3 +429,32 @@ def test_mixed_fused_step_matches_per_call_force_with_linked_contacts(squeezed):
         per_call = ctx.download_particles()
     for f in FORCE_FIELDS:
         assert bits_equal(fused[f], per_call[f]), f
+
+
+def test_mixed_runs_with_and_without_partner_list(squeezed, orc):
+    """Fused mixed runs (rebuilds included) with the partner-free list and
+    with the reference's list stay within FP32 rounding of each other and of
+    the FP64 oracle over 60 steps of the squeezed block, where linked
+    neighbours are in contact from the start."""
+    s, p, l = squeezed
+    say("mixed runs: partner-free list vs reference list")
+    sim = orc.OracleSim(p, l, s)
+    sim.run(60)
+    out = {}
+    for merged in (True, False):
+        with new_ctx(p, l, A.HDM_PREC_MIXED) as ctx:
+            ctx.set_exec(list_with_partners=not merged)
+            ctx.links_build()
+            st = ctx.run(s.run, 60)
+            assert ctx.exec_info()["merged_contacts"] == merged
+            out[merged] = (ctx.download_particles(), ctx.count_broken(), st.rebuilds)
+    ref = np.stack([sim.p["xCord"], sim.p["yCord"], sim.p["zCord"]])
+    disp = np.max(np.abs(ref - np.stack([p["xCord"], p["yCord"], p["zCord"]])))
+    assert disp > 1e-4  # the squeezed block expands
+    for merged, (gp, broken, rebuilds) in out.items():
+        got = np.stack([gp["xCord"], gp["yCord"], gp["zCord"]])
+        err = np.max(np.abs(got - ref))
+        say("  merged" if merged else "  listed", "max |dx| =", err, "displacement", disp, "broken", broken)
+        assert err <= 1e-3 * disp, (merged, err, disp)
+        ob = int(np.sum(sim.l.state == 0))
+        assert abs(broken - ob) <= max(2, ob // 100), (merged, broken, ob)
