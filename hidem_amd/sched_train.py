@@ -159,6 +159,32 @@ def evaluate(setup, state, ts, steps, model, chunk=1000):
     return results
 
 
+def train(X, T, Cn, epochs=8, max_rows=4_000_000, seed=0):
+    """Standardises the rows, initialises the network (He normal weights, the
+    output bias at the mean uncensored target) and trains it on the device
+    (hdm_mlp_train).  Returns (weights, mean loss before, mean loss after)."""
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(len(X))[:max_rows]
+    X, T, Cn = X[order], T[order], Cn[order]
+    mean, std = X.mean(0), X.std(0)
+    std = np.where(std > 1e-6, std, 1.0).astype(np.float32)
+    w0 = np.zeros(A.C.sizeof(A.HdmMlpWeights) // 4, np.float32)
+    r = np.random.default_rng(seed + 1)
+    w0[:16 * 64] = r.normal(0, np.sqrt(2 / 16), 16 * 64)
+    o = 16 * 64 + 64
+    w0[o:o + 64 * 64] = r.normal(0, np.sqrt(2 / 64), 64 * 64)
+    o += 64 * 64 + 64
+    w0[o:o + 64] = r.normal(0, np.sqrt(1 / 64), 64)
+    w0[o + 64] = float(T[Cn == 0].mean()) if (Cn == 0).any() else float(T.mean())
+    w0[o + 65:o + 81] = mean
+    w0[o + 81:o + 97] = std
+    w = A.HdmMlpWeights.from_numpy(w0)
+    with A.Context(A.HDM_PREC_MIXED) as c:
+        _, loss0 = c.mlp_gradient(w, X, T, Cn)
+        loss = c.mlp_train(w, X, T, Cn, epochs=epochs, batch=4096, lr=1e-3)
+    return w, loss0 / len(X), loss
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--config", default="glacier_8m")
@@ -177,31 +203,13 @@ def main(argv=None):
     t0 = time.perf_counter()
     X, T, Cn, rebuilds, state, ts = generate(setup, p, l, args.train_steps, args.every)
     log(f"# {len(X)} rows, {Cn.mean():.3f} censored, {rebuilds} rebuilds, {time.perf_counter() - t0:.0f} s")
-    rng = np.random.default_rng(0)
-    order = rng.permutation(len(X))[:args.max_rows]
-    X, T, Cn = X[order], T[order], Cn[order]
-    mean, std = X.mean(0), X.std(0)
-    std = np.where(std > 1e-6, std, 1.0).astype(np.float32)
-    w0 = np.zeros(A.C.sizeof(A.HdmMlpWeights) // 4, np.float32)
-    r = np.random.default_rng(1)
-    w0[:16 * 64] = r.normal(0, np.sqrt(2 / 16), 16 * 64)
-    o = 16 * 64 + 64
-    w0[o:o + 64 * 64] = r.normal(0, np.sqrt(2 / 64), 64 * 64)
-    o += 64 * 64 + 64
-    w0[o:o + 64] = r.normal(0, np.sqrt(1 / 64), 64)
-    w0[o + 64] = float(T[Cn == 0].mean()) if (Cn == 0).any() else float(T.mean())
-    w0[o + 65:o + 81] = mean
-    w0[o + 81:o + 97] = std
-    w = A.HdmMlpWeights.from_numpy(w0)
-    with A.Context(A.HDM_PREC_MIXED) as c:
-        _, loss0 = c.mlp_gradient(w, X, T, Cn)
-        loss = c.mlp_train(w, X, T, Cn, epochs=args.epochs, batch=4096, lr=1e-3)
-    log(f"# training: mean loss {loss0 / len(X):.4f} -> {loss:.4f}")
+    w, loss0, loss = train(X, T, Cn, epochs=args.epochs, max_rows=args.max_rows)
+    log(f"# training: mean loss {loss0:.4f} -> {loss:.4f}")
     out = args.out or os.path.join(MODEL_DIR, f"sched_{args.config}.bin")
     ev = evaluate(setup, state, ts, args.eval_steps, w)
     report = {"config": args.config, "particles": p.n, "links": l.n, "train_steps": args.train_steps,
-              "eval_steps": args.eval_steps, "rows": int(len(X)), "censored_fraction": float(Cn.mean()),
-              "rebuilds_in_training": rebuilds, "loss_initial": loss0 / len(X),
+              "eval_steps": args.eval_steps, "rows": int(min(len(X), args.max_rows)), "censored_fraction": float(Cn.mean()),
+              "rebuilds_in_training": rebuilds, "loss_initial": loss0,
               "loss_trained": loss, "evaluation": ev, "seconds": time.perf_counter() - t0}
     save_model(out, w, report)
     print(json.dumps(report), flush=True)

@@ -229,3 +229,34 @@ def test_label_tracking_matches_host():
         want = np.where((want < 0) & hit, k, want)
     assert (want > 0).sum() > 0   # free-falling particles cover the budget in ~50 steps
     assert np.array_equal(first, want)
+
+
+def test_trained_network_gains_over_fixed_batches():
+    """What the trained network gains: a network trained on the device
+    (sched_train.train) on the first steps of a small glacier slab, run on
+    the following steps, against the same device-side exact test with fixed
+    batches (no network) and with a host check after every step.  All
+    three end bit-identical; the network needs far fewer host checks than
+    the per-step test and wastes fewer launches (steps launched after the
+    guard tripped) than fixed batches."""
+    from hidem_amd import sched_train
+    cfg = params.load_config("glacier_8m")
+    setup, p, l = params.build_inputs(cfg, [40, 6, 8])
+    # a thin skin: rebuilds every few dozen steps, so both the labels and the
+    # evaluation see many trips
+    setup.run.interactDist = setup.run.force.attractDistance + 0.004
+    say("trained network gain", p.n, l.n)
+    X, T, Cn, rebuilds, state, ts = sched_train.generate(setup, p, l, 1500, 25)
+    say("  rows", len(X), "censored", float(Cn.mean()), "rebuilds", rebuilds)
+    assert rebuilds >= 5 and (Cn == 0).sum() > 100
+    w, loss0, loss = sched_train.train(X, T, Cn, epochs=20)
+    say("  loss", loss0, "->", loss)
+    assert loss < loss0
+    ev = sched_train.evaluate(setup, state, ts, 1500, w, chunk=500)
+    for k, v in ev.items():
+        say("  ", k, v)
+    assert ev["exact"]["rebuilds"] >= 5
+    assert ev["learned"]["bit_identical_to_exact"] and ev["learned_no_model"]["bit_identical_to_exact"]
+    assert ev["learned"]["rebuilds"] == ev["exact"]["rebuilds"]
+    assert ev["learned"]["host_checks"] * 4 < ev["exact"]["host_checks"]
+    assert ev["learned"]["wasted_launches"] < ev["learned_no_model"]["wasted_launches"]
