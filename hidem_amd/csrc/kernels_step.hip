@@ -17,8 +17,8 @@
 // (coordinates, Diff, radius) is gathered through L2; particles are stored in
 // spatial (Morton) order and blocks are renumbered per XCD, so partners of a
 // tile were usually fetched by a neighbouring tile on the same XCD.  A link's
-// broken flag rides in the sign bit of its slot word (no per-link gather), and
-// contact pairs beyond the attraction range are skipped when that provably
+// broken flag rides in the sign bit of its slot words (no per-link gather),
+// and contact pairs beyond the attraction range are skipped when that provably
 // leaves the force unchanged (contact_is_inert).
 #include <cmath>
 #include <type_traits>
@@ -794,18 +794,19 @@ __device__ __forceinline__ void contact_sides(const ForceArgs<R, CT, R> &f, cons
 // k_integrate), with the rebuild-test partials.
 //
 // The records are the state: a particle's coordinates and Diff are read from
-// its own rec/drec (the same lines its neighbours gather) and the integrated
-// ones are written to the other record buffer only; the SoA coordinates and
-// Diff are unpacked from the records when something else needs them
-// (sync_state).  The rotations stay SoA (ping-pong), read at integration and
-// by the rare exact break test; the prefilter takes their bound from drec.
+// its own records (the lines its neighbours gather) and the integrated ones
+// are written to the other record buffers only; the SoA coordinates and Diff
+// are unpacked from the records when something else needs them (sync_state).
+// The rotations stay SoA (ping-pong), read at integration and by the rare
+// exact break test; the prefilter takes their bound from the records.
 //
-// Partner state comes from the records too: one 32 B load of rec decides a
-// contact pair that is provably inert (contact_is_inert: beyond
+// Partner state comes from the records too.  FP64 form: one 32 B load of rec
+// decides a contact pair that is provably inert (contact_is_inert: beyond
 // radSum + attractDistance with every Diff bounded by tame, so every term the
-// law adds is +-0); only other pairs, and link partners, load drec.  The
-// break test of an intact side runs exactly (FP64, as k_break) unless the
-// float prefilter proves the link stays intact.
+// law adds is +-0); only other pairs, and link partners, load drec.  FP32
+// form: one 64 B MixedRec per partner.  The break test of an intact side runs
+// exactly (FP64, as k_break) unless the float prefilter proves the link stays
+// intact.
 // BLOCK (workgroup size), XCD (the per-XCD contiguous renumbering of
 // workgroups) and OCC (one more wave per SIMD than the register count allows,
 // bought with a few spills) are the autotuned launch shape; every shape
