@@ -20,6 +20,9 @@ arithmetic can stay once the dynamics amplify differences (fracture).  With
 --perturb-mixed, a mixed run from the same perturbed coordinates is compared
 with the mixed run: the spread of the mixed form itself, which tells a
 systematic difference between the forms from the chaos of one realisation.
+With --ensemble K, K more runs of each form start from coordinates moved by
+2..K+1 ulps, and each checkpoint reports the broken-link counts of every
+realisation of both forms (mean and spread per form).
 
 Usage: python -m hidem_amd.accuracy [--config glacier_8m] [--checkpoints 100 1000 10000]
 Prints one JSON line per checkpoint and a final summary line.
@@ -77,6 +80,8 @@ def main(argv=None):
     ap.add_argument("--perturb", action="store_true", help="also run FP64 from one-ulp perturbed coordinates")
     ap.add_argument("--perturb-mixed", action="store_true",
                     help="also run the mixed form from one-ulp perturbed coordinates")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="K more runs of each form from coordinates moved by 2..K+1 ulps (broken counts only)")
     args = ap.parse_args(argv)
 
     cfg = params.load_config(args.config)
@@ -93,12 +98,25 @@ def main(argv=None):
     if args.perturb_mixed:
         role["mixed_ulp_perturbed"] = len(ctxs)
         ctxs.append(A.Context(A.HDM_PREC_MIXED))
-    q = p.copy()
-    for f in ("xCord", "yCord", "zCord"):
-        q.arrays[f] = np.nextafter(q[f], np.inf)
+    def moved(ulps):
+        q = p.copy()
+        for f in ("xCord", "yCord", "zCord"):
+            v = q[f]
+            for _ in range(ulps):
+                v = np.nextafter(v, np.inf)
+            q.arrays[f] = v
+        return q
+
+    start = [p, p] + [moved(1)] * len(role)
+    ens = []  # (precision name, context index)
+    for k in range(args.ensemble):
+        for prec, name in ((A.HDM_PREC_FP64, "fp64"), (A.HDM_PREC_MIXED, "mixed")):
+            ens.append((name, len(ctxs)))
+            ctxs.append(A.Context(prec))
+            start.append(moved(k + 2))
     try:
         for i, c in enumerate(ctxs):
-            c.upload(q if i >= 2 else p, l)
+            c.upload(start[i], l)
             c.links_build()
         done = 0
         rebuilds = [0] * len(ctxs)
@@ -119,6 +137,14 @@ def main(argv=None):
             for name, i in role.items():
                 base_p, base_b = (p64, b64) if name.startswith("fp64") else (pmx, bmx)
                 r[name] = compare(base_p, ctxs[i].download_particles(), x0, base_b, ctxs[i].count_broken())
+            if ens:
+                counts = {"fp64": [b64], "mixed": [bmx]}
+                for name, i in role.items():
+                    counts["fp64" if name.startswith("fp64") else "mixed"].append(ctxs[i].count_broken())
+                for name, i in ens:
+                    counts[name].append(ctxs[i].count_broken())
+                r["ensemble_broken"] = {k: {"counts": v, "mean": float(np.mean(v)), "std": float(np.std(v))}
+                                        for k, v in counts.items()}
             out.append(r)
             print(json.dumps(r), flush=True)
     finally:
