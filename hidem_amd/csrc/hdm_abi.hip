@@ -224,9 +224,10 @@ int hdm_device_bytes(hdm_ctx *ctx, int64_t *bytes) {
                                  &c.slotOther, &c.sidePos, &c.nbrCount, &c.nbr, &c.colour, &c.order,
                                  &c.cellStart, &c.cellCursor, &c.scanPartials, &c.red, &c.x0, &c.y0, &c.z0,
                                  &c.skinPartials, &c.blockId, &c.blockWater, &c.probe, &c.rec, &c.recAlt,
-                                 &c.drec, &c.drecAlt, &c.slotThr};
+                                 &c.drec, &c.drecAlt, &c.dtail, &c.dtailAlt, &c.slotThr};
     for (auto *b : bufs) t += b->bytes;
-    for (int k = 0; k < 6; k++) t += c.diff[k].bytes + c.force[k].bytes + c.ct[k].bytes + c.ctStage[k].bytes;
+    t += c.ct.bytes;
+    for (int k = 0; k < 6; k++) t += c.diff[k].bytes + c.force[k].bytes + c.ctStage[k].bytes;
     for (int k = 0; k < 12; k++) t += c.alt[k].bytes;
     *bytes = (int64_t)t;
     return HDM_OK;
@@ -501,7 +502,7 @@ int hdm_links_build(hdm_ctx *ctx) {
     HIP_CHECK(ctx, c.slotLink.ensure(sizeof(int32_t) * slots));
     HIP_CHECK(ctx, c.slotOther.ensure(sizeof(int32_t) * slots));
     HIP_CHECK(ctx, c.sidePos.ensure(sizeof(int32_t) * (size_t)(2 * c.L > 0 ? 2 * c.L : 1)));
-    for (int k = 0; k < 6; k++) HIP_CHECK(ctx, c.ct[k].ensure(c.ctBytes() * slots));
+    HIP_CHECK(ctx, c.ct.ensure(6 * c.ctBytes() * slots));
     HIP_CHECK(ctx, c.slotThr.ensure(sizeof(float) * slots));
     HIP_CHECK(ctx, hdm::launch_link_fill(c));
     HIP_CHECK(ctx, hdm::launch_ct_import(c));
@@ -817,15 +818,15 @@ std::vector<unsigned char> graph_key(const Context &c, const hdm_run_params *rp,
         k.insert(k.end(), b, b + n);
     };
     const hdm::DevBuf *bufs[] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz, &c.radius, &c.type, &c.rec, &c.recAlt,
-                                 &c.drec, &c.drecAlt, &c.nbrCount, &c.nbr, &c.linkCount, &c.slotLink, &c.slotOther,
+                                 &c.drec, &c.drecAlt, &c.dtail, &c.dtailAlt, &c.nbrCount, &c.nbr, &c.linkCount, &c.slotLink, &c.slotOther,
                                  &c.state, &c.initialLen, &c.maxLoad, &c.slotThr, &c.skinPartials, &c.guardState,
                                  &c.x0, &c.y0, &c.z0, &c.blockId, &c.blockWater};
     for (auto *b : bufs) put(&b->ptr, sizeof(void *));
     for (int q = 0; q < 6; q++) {
         put(&c.diff[q].ptr, sizeof(void *));
         put(&c.force[q].ptr, sizeof(void *));
-        put(&c.ct[q].ptr, sizeof(void *));
     }
+    put(&c.ct.ptr, sizeof(void *));
     for (int q = 0; q < 12; q++) put(&c.alt[q].ptr, sizeof(void *));
     put(&c.n, sizeof(c.n));
     put(&c.L, sizeof(c.L));

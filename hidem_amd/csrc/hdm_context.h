@@ -98,21 +98,27 @@ struct Context {
     DevBuf alt[12];
     // gather records of the fused step, ping-pong like the state:
     //   rec   FP64 form: double4 {x, y, z, radius}; FP32 form: MixedRec
-    //         {x, y, z, Diff0..5 in FP32, rotation bound, radius} (64 B); the
+    //         {x, y, z, Diff0..5 in FP32, rotation bound, radius, low parts
+    //         of Diff0 and Diff1} (64 B); the
     //         radius reads NaN when a Diff component exceeds recTame (then
     //         the contact skip test fails)
-    //   drec  FP64 Diff0..5 of every particle; the FP64 form adds the
-    //         rotation bound |xTheta| + |yTheta| + |zTheta| (as a float) and
-    //         a 0 (8 doubles, gathered by partners), the FP32 form stores 6
-    //         (read only by the particle; partners gather its MixedRec:
-    //         coordinates, FP32 Diff, rotation bound, radius flag)
+    //   drec  FP64 form: FP64 Diff0..5 of every particle, the rotation
+    //         bound |xTheta| + |yTheta| + |zTheta| (as a float) and a 0 (8
+    //         doubles, gathered by partners).  FP32 form: 3 doubles, the
+    //         FP64 translation Diff of the particles whose Diff does not
+    //         split exactly into the record's FP32 part and low parts
+    //         (split_diff3); others leave their entry untouched
+    //   dtail FP32 form: MixedTail, the FP64 rotation Diff and the low part
+    //         of Diff2 (NaN when the translation Diff is in drec), 32 B.
+    //         Only the particle reads its own; partners gather its MixedRec
+    //         (coordinates, FP32 Diff, rotation bound, radius flag)
     // The fused step reads and writes only the records for coordinates and
     // Diff: after it the SoA x, y, z and diff[] are stale (soaStale) until
     // sync_state unpacks them.
-    DevBuf rec, recAlt, drec, drecAlt;
-    bool recCurrent = false;     // rec/drec hold the current state
+    DevBuf rec, recAlt, drec, drecAlt, dtail, dtailAlt;
+    bool recCurrent = false;     // rec/drec/dtail hold the current state
     bool soaStale = false;       // x, y, z, diff[] lag the records
-    bool altHoldsPrev = false;   // drecAlt holds the Diff before the last fused step
+    bool altHoldsPrev = false;   // recAlt/drecAlt/dtailAlt hold the state before the last fused step
     // FP32 force: the neighbour list leaves out each particle's link partners
     // and the link loop applies their contact law (merge_linked_contacts)
     bool partnersUnique = false;     // no particle has two sides to one partner or a side to itself
@@ -130,7 +136,8 @@ struct Context {
     // link slots (wave tiles) and side store
     int32_t maxLinks = 0;
     bool linksBuilt = false;
-    DevBuf linkCount, linkCursor, slotLink, slotOther, ct[6];
+    DevBuf linkCount, linkCursor, slotLink, slotOther;
+    DevBuf ct;             // CT0..5 of every slot, 6 x 64 values per (wave tile, side) block (ct_offset)
     DevBuf sidePos;        // 2L: tiled slot of side (l, end)
     DevBuf slotThr;        // float per slot: break prefilter threshold of the side's link (k_side_thresholds)
     float l0Max = 0.0f;    // largest initialLen (rounded up to float)

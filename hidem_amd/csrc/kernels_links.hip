@@ -95,36 +95,33 @@ __global__ void __launch_bounds__(TPB) k_ct_import(const int32_t *__restrict__ s
                                                     const double *__restrict__ s0, const double *__restrict__ s1,
                                                     const double *__restrict__ s2, const double *__restrict__ s3,
                                                     const double *__restrict__ s4, const double *__restrict__ s5,
-                                                    CT *__restrict__ c0, CT *__restrict__ c1, CT *__restrict__ c2,
-                                                    CT *__restrict__ c3, CT *__restrict__ c4, CT *__restrict__ c5) {
+                                                    CT *__restrict__ ct) {
     int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= nSides) return;
     int64_t d = sidePos[i];
-    c0[d] = (CT)s0[i];
-    c1[d] = (CT)s1[i];
-    c2[d] = (CT)s2[i];
-    c3[d] = (CT)s3[i];
-    c4[d] = (CT)s4[i];
-    c5[d] = (CT)s5[i];
+    ct[ct_offset<CT>(d, 0)] = (CT)s0[i];
+    ct[ct_offset<CT>(d, 1)] = (CT)s1[i];
+    ct[ct_offset<CT>(d, 2)] = (CT)s2[i];
+    ct[ct_offset<CT>(d, 3)] = (CT)s3[i];
+    ct[ct_offset<CT>(d, 4)] = (CT)s4[i];
+    ct[ct_offset<CT>(d, 5)] = (CT)s5[i];
 }
 
 template <class CT>
 __global__ void __launch_bounds__(TPB) k_ct_export(const int32_t *__restrict__ sidePos, int64_t nSides,
-                                                    const CT *__restrict__ c0, const CT *__restrict__ c1,
-                                                    const CT *__restrict__ c2, const CT *__restrict__ c3,
-                                                    const CT *__restrict__ c4, const CT *__restrict__ c5,
-                                                    double *__restrict__ s0, double *__restrict__ s1,
-                                                    double *__restrict__ s2, double *__restrict__ s3,
-                                                    double *__restrict__ s4, double *__restrict__ s5) {
+                                                    const CT *__restrict__ ct, double *__restrict__ s0,
+                                                    double *__restrict__ s1, double *__restrict__ s2,
+                                                    double *__restrict__ s3, double *__restrict__ s4,
+                                                    double *__restrict__ s5) {
     int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= nSides) return;
     int64_t d = sidePos[i];
-    s0[i] = (double)c0[d];
-    s1[i] = (double)c1[d];
-    s2[i] = (double)c2[d];
-    s3[i] = (double)c3[d];
-    s4[i] = (double)c4[d];
-    s5[i] = (double)c5[d];
+    s0[i] = (double)ct[ct_offset<CT>(d, 0)];
+    s1[i] = (double)ct[ct_offset<CT>(d, 1)];
+    s2[i] = (double)ct[ct_offset<CT>(d, 2)];
+    s3[i] = (double)ct[ct_offset<CT>(d, 3)];
+    s4[i] = (double)ct[ct_offset<CT>(d, 4)];
+    s5[i] = (double)ct[ct_offset<CT>(d, 5)];
 }
 
 // Per-side float threshold of the break prefilter, in the side's wave-tile
@@ -221,19 +218,12 @@ hipError_t launch_ct_import(Context &c) {
     if (nSides == 0) return hipSuccess;
     const double *s[6];
     for (int k = 0; k < 6; k++) s[k] = c.ctStage[k].as<double>();
-    if (c.prec == HDM_PREC_FP64) {
-        double *d[6];
-        for (int k = 0; k < 6; k++) d[k] = c.ct[k].as<double>();
+    if (c.prec == HDM_PREC_FP64)
         hipLaunchKernelGGL(k_ct_import<double>, dim3(blocks_for(nSides)), dim3(TPB), 0, c.stream,
-                           c.sidePos.as<int32_t>(), nSides, s[0], s[1], s[2], s[3], s[4], s[5], d[0], d[1], d[2],
-                           d[3], d[4], d[5]);
-    } else {
-        float *d[6];
-        for (int k = 0; k < 6; k++) d[k] = c.ct[k].as<float>();
+                           c.sidePos.as<int32_t>(), nSides, s[0], s[1], s[2], s[3], s[4], s[5], c.ct.as<double>());
+    else
         hipLaunchKernelGGL(k_ct_import<float>, dim3(blocks_for(nSides)), dim3(TPB), 0, c.stream,
-                           c.sidePos.as<int32_t>(), nSides, s[0], s[1], s[2], s[3], s[4], s[5], d[0], d[1], d[2],
-                           d[3], d[4], d[5]);
-    }
+                           c.sidePos.as<int32_t>(), nSides, s[0], s[1], s[2], s[3], s[4], s[5], c.ct.as<float>());
     return hipGetLastError();
 }
 
@@ -242,19 +232,12 @@ hipError_t launch_ct_export(Context &c) {
     if (nSides == 0) return hipSuccess;
     double *s[6];
     for (int k = 0; k < 6; k++) s[k] = c.ctStage[k].as<double>();
-    if (c.prec == HDM_PREC_FP64) {
-        const double *d[6];
-        for (int k = 0; k < 6; k++) d[k] = c.ct[k].as<double>();
+    if (c.prec == HDM_PREC_FP64)
         hipLaunchKernelGGL(k_ct_export<double>, dim3(blocks_for(nSides)), dim3(TPB), 0, c.stream,
-                           c.sidePos.as<int32_t>(), nSides, d[0], d[1], d[2], d[3], d[4], d[5], s[0], s[1], s[2],
-                           s[3], s[4], s[5]);
-    } else {
-        const float *d[6];
-        for (int k = 0; k < 6; k++) d[k] = c.ct[k].as<float>();
+                           c.sidePos.as<int32_t>(), nSides, c.ct.as<double>(), s[0], s[1], s[2], s[3], s[4], s[5]);
+    else
         hipLaunchKernelGGL(k_ct_export<float>, dim3(blocks_for(nSides)), dim3(TPB), 0, c.stream,
-                           c.sidePos.as<int32_t>(), nSides, d[0], d[1], d[2], d[3], d[4], d[5], s[0], s[1], s[2],
-                           s[3], s[4], s[5]);
-    }
+                           c.sidePos.as<int32_t>(), nSides, c.ct.as<float>(), s[0], s[1], s[2], s[3], s[4], s[5]);
     return hipGetLastError();
 }
 

@@ -66,8 +66,12 @@ __global__ void __launch_bounds__(TPB) k_sched_blocks(const double *__restrict__
 struct AccArgs {
     const double *x, *y, *z, *x0, *y0, *z0;
     const double *d0, *d1, *d2, *d3, *d4, *d5;
-    const double *prev;                // drec before the last fused step (nullptr: unknown)
-    int64_t prevStride;                // doubles per particle in prev
+    // Diff before the last fused step (prev and prevLo nullptr: unknown):
+    // FP64 form prev = that drec (8 doubles per particle); FP32 form the
+    // records, tails and escapes of that state
+    const double *prev;
+    const MixedRec *prevRec;
+    const MixedTail *prevTail;
     const int32_t *blockOf;
     const uint32_t *linkCount;
     const int32_t *slotLink;
@@ -106,7 +110,13 @@ __global__ void __launch_bounds__(TPB) k_sched_accumulate(AccArgs a) {
         const double D = sqrt(dx * dx + dy * dy + dz * dz);
         double Ac = 0.0;
         if (a.prev) {
-            const double *q = a.prev + p * a.prevStride;
+            double q[6];
+            if (a.prevTail) {
+                join_diff6(a.prevRec[p], a.prevTail[p], a.prev, p, q);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++) q[c] = a.prev[p * 8 + c];
+            }
             const double ax = dx - q[0], ay = dy - q[1], az = dz - q[2];
             Ac = sqrt(ax * ax + ay * ay + az * az);
         }
@@ -563,8 +573,10 @@ hipError_t sched_features(Context &c, const SchedGeometry &g, const SchedFeature
     a.x0 = c.x0.as<double>(); a.y0 = c.y0.as<double>(); a.z0 = c.z0.as<double>();
     a.d0 = c.diff[0].as<double>(); a.d1 = c.diff[1].as<double>(); a.d2 = c.diff[2].as<double>();
     a.d3 = c.diff[3].as<double>(); a.d4 = c.diff[4].as<double>(); a.d5 = c.diff[5].as<double>();
+    const bool fp32 = c.prec != HDM_PREC_FP64;
     a.prev = c.altHoldsPrev ? c.drecAlt.as<double>() : nullptr;
-    a.prevStride = c.prec == HDM_PREC_FP64 ? 8 : 6;
+    a.prevRec = c.altHoldsPrev && fp32 ? c.recAlt.as<MixedRec>() : nullptr;
+    a.prevTail = c.altHoldsPrev && fp32 ? c.dtailAlt.as<MixedTail>() : nullptr;
     a.blockOf = c.schedBlockOf.as<int32_t>();
     a.linkCount = c.linkCount.as<uint32_t>();
     a.slotLink = c.slotLink.as<int32_t>();

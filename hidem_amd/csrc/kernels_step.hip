@@ -233,7 +233,7 @@ struct ForceArgs {
     const uint32_t *linkCount;
     const int32_t *slotLink, *slotOther;
     int32_t linkWidth;
-    CT *c0, *c1, *c2, *c3, *c4, *c5;
+    CT *ct;  // side store (ct_offset layout)
     FO *f0, *f1, *f2, *f3, *f4, *f5;
     ForceParams<R> fp;
     // Fused link break of the previous timestep (fused step only; breakCheck
@@ -313,14 +313,13 @@ __device__ __forceinline__ void force_particle(const ForceArgs<R, CT, FO> &a, in
             R zDist = (R)(zCord - a.z[q]);
             R mask = 1;
             if (ow < 0) mask = 0;  // broken link (state == 0)
-            R ct[6] = {(R)a.c0[sidx], (R)a.c1[sidx], (R)a.c2[sidx], (R)a.c3[sidx], (R)a.c4[sidx], (R)a.c5[sidx]};
+            CT cs[6];
+            ct_load(a.ct, sidx, cs);
+            R ct[6] = {(R)cs[0], (R)cs[1], (R)cs[2], (R)cs[3], (R)cs[4], (R)cs[5]};
             beam_law<R>(mask, xDist, yDist, zDist, self, other, d, a.fp, ct, F);
-            a.c0[sidx] = (CT)ct[0];
-            a.c1[sidx] = (CT)ct[1];
-            a.c2[sidx] = (CT)ct[2];
-            a.c3[sidx] = (CT)ct[3];
-            a.c4[sidx] = (CT)ct[4];
-            a.c5[sidx] = (CT)ct[5];
+#pragma unroll
+            for (int c = 0; c < 6; c++) cs[c] = (CT)ct[c];
+            ct_store(a.ct, sidx, cs);
         }
         // the contacts of linked pairs (list without link partners), after
         // every beam, in side order (as the fused step)
@@ -614,15 +613,8 @@ __global__ void __launch_bounds__(TPB) k_integrate(IntegArgs a, const FO *__rest
 }
 
 // ---- fused step ------------------------------------------------------------------
-// Gather record of a particle in the FP32 form: coordinates in FP64, Diff
-// rounded to FP32, the rotation bound and the radius (NaN when not tame), in
-// one 64 B line: a link partner costs four 16 B loads.  (The FP64 form gathers
-// rec = double4 {x, y, z, radius} and the FP64 Diff from drec.)
-struct __attribute__((aligned(16))) MixedRec {
-    double x, y, z;
-    float d[6];
-    float rot, radius;
-};
+// Gather records: MixedRec (kernel_common.h) in the FP32 form; the FP64 form
+// gathers rec = double4 {x, y, z, radius} and the FP64 Diff from drec.
 template <class R> struct RecOf { using type = double4; };
 template <> struct RecOf<float> { using type = MixedRec; };
 __device__ __forceinline__ double rec_radius(const double4 &P) { return P.w; }
@@ -632,10 +624,12 @@ __device__ __forceinline__ float rec_radius(const MixedRec &P) { return P.radius
 template <class R>
 struct StepRecs {
     using Rec = typename RecOf<R>::type;
-    const Rec *rec;            // current state (drec: every particle's FP64 Diff)
-    const double *drec;
+    const Rec *rec;            // current state
+    const double *drec;        // FP64 form: every particle's FP64 Diff; FP32 form: the escapes
+    const MixedTail *tail;     // FP32 form: rest of the own Diff (split_diff3)
     Rec *recOut;               // integrated state
     double *drecOut;
+    MixedTail *tailOut;
     double tame;               // Diff bound behind the radius flag of rec
     // Break prefilter inputs: per side a float threshold, L0 (1 + maxLoad)
     // with per-link maxLoad (perLinkMaxLoad) else L0, NaN when L0 < 1e-30 or
@@ -655,20 +649,11 @@ __device__ __forceinline__ float rotation_bound(const double t[3]) {
 }
 
 // Doubles per particle in drec: the FP64 form's partners gather it (Diff0..5,
-// rotation bound, 0); in the FP32 form only the particle itself reads it
-// (Diff0..5), the rest travelling in its MixedRec.
-template <class R>
-constexpr int64_t drec_stride() {
-    return sizeof(R) == 4 ? 6 : 8;
-}
-inline int64_t drec_stride(int32_t prec) { return prec == HDM_PREC_FP64 ? 8 : 6; }
-
-// A particle's own FP64 Diff in the FP32 form (48 B, three 16 B loads).
-__device__ __forceinline__ void load_own_diff6(const double *drec, int64_t p, double o[6]) {
-    const double2 *v = reinterpret_cast<const double2 *>(drec + p * 6);
-    const double2 a = v[0], b = v[1], c = v[2];
-    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y; o[4] = c.x; o[5] = c.y;
-}
+// rotation bound, 0); in the FP32 form it is the escape array of
+// split_diff3 (Diff0..2, written and read only for particles whose
+// translation Diff does not split exactly), the rest travelling in the
+// MixedRec and the MixedTail.
+inline int64_t drec_stride(int32_t prec) { return prec == HDM_PREC_FP64 ? 8 : 3; }
 
 __device__ __forceinline__ void load_drec(const double *drec, int64_t q, double o[6], float &rot) {
     const double4 *v = reinterpret_cast<const double4 *>(drec + q * 8);
@@ -688,20 +673,27 @@ __device__ __forceinline__ void store_records(const StepRecs<R> &r, int64_t p, c
         m.x = cord[0];
         m.y = cord[1];
         m.z = cord[2];
+        float lo[3];
+        const bool exact = split_diff3(diff, m.d, lo);
 #pragma unroll
-        for (int c = 0; c < 6; c++) m.d[c] = (float)diff[c];
+        for (int c = 3; c < 6; c++) m.d[c] = (float)diff[c];
         m.rot = rotation_bound(theta);
         m.radius = tame ? (float)radius : __builtin_nanf("");
+        m.lo[0] = lo[0];
+        m.lo[1] = lo[1];
         r.recOut[p] = m;
+        MixedTail t;
+#pragma unroll
+        for (int c = 0; c < 3; c++) t.dr[c] = diff[3 + c];
+        t.lo2 = lo[2];
+        t.pad = 0.0f;
+        r.tailOut[p] = t;
+        if (!exact) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) r.drecOut[p * 3 + c] = diff[c];
+        }
     } else {
         r.recOut[p] = make_double4(cord[0], cord[1], cord[2], tame ? radius : __builtin_nan(""));
-    }
-    if constexpr (sizeof(R) == 4) {
-        double2 *d = reinterpret_cast<double2 *>(r.drecOut + p * 6);
-        d[0] = make_double2(diff[0], diff[1]);
-        d[1] = make_double2(diff[2], diff[3]);
-        d[2] = make_double2(diff[4], diff[5]);
-    } else {
         double4 *d = reinterpret_cast<double4 *>(r.drecOut + p * 8);
         d[0] = make_double4(diff[0], diff[1], diff[2], diff[3]);
         d[1] = make_double4(diff[4], diff[5], (double)rotation_bound(theta), 0.0);
@@ -830,7 +822,7 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
         double diff[6];
         float rotSelf;
         if constexpr (sizeof(R) == 4) {
-            load_own_diff6(r.drec, p, diff);
+            join_diff6(P, r.tail[p], r.drec, p, diff);
             rotSelf = P.rot;
         } else {
             load_drec(r.drec, p, diff, rotSelf);
@@ -904,8 +896,9 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                         }
                     }
                 }
-                R ct[6] = {(R)ld_stream(f.c0 + sidx), (R)ld_stream(f.c1 + sidx), (R)ld_stream(f.c2 + sidx),
-                           (R)ld_stream(f.c3 + sidx), (R)ld_stream(f.c4 + sidx), (R)ld_stream(f.c5 + sidx)};
+                CT cs[6];
+                ct_load(f.ct, sidx, cs);
+                R ct[6] = {(R)cs[0], (R)cs[1], (R)cs[2], (R)cs[3], (R)cs[4], (R)cs[5]};
                 beam_law<R>(mask, (R)dx, (R)dy, (R)dz, self, other, d, f.fp, ct, F);
                 if (sizeof(R) == 4 && f.linkedContacts) {
                     // the pair's contact test (the contact loop's record test);
@@ -921,12 +914,9 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                           particleDistance2 > R(0)))
                         activeSides |= 1ull << k;
                 }
-                st_stream(f.c0 + sidx, (CT)ct[0]);
-                st_stream(f.c1 + sidx, (CT)ct[1]);
-                st_stream(f.c2 + sidx, (CT)ct[2]);
-                st_stream(f.c3 + sidx, (CT)ct[3]);
-                st_stream(f.c4 + sidx, (CT)ct[4]);
-                st_stream(f.c5 + sidx, (CT)ct[5]);
+#pragma unroll
+                for (int c = 0; c < 6; c++) cs[c] = (CT)ct[c];
+                ct_store(f.ct, sidx, cs);
             }
             if (sizeof(R) == 4 && activeSides) contact_sides(f, r, activeSides, base, xCord, yCord, zCord, diff, self, F);
         }
@@ -951,9 +941,10 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
 }
 
 // SoA coordinates and Diff from the records (sync_state).
-template <class Rec, int STRIDE>
+template <class Rec>
 __global__ void __launch_bounds__(TPB) k_unpack_records(const Rec *__restrict__ rec,
-                                                         const double *__restrict__ drec, int64_t n, double *x,
+                                                         const double *__restrict__ drec,
+                                                         const MixedTail *__restrict__ tail, int64_t n, double *x,
                                                          double *y, double *z, double *d0, double *d1, double *d2,
                                                          double *d3, double *d4, double *d5) {
     const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
@@ -962,7 +953,13 @@ __global__ void __launch_bounds__(TPB) k_unpack_records(const Rec *__restrict__ 
     x[p] = P.x;
     y[p] = P.y;
     z[p] = P.z;
-    const double *v = drec + p * STRIDE;
+    double v[6];
+    if constexpr (std::is_same<Rec, MixedRec>::value) {
+        join_diff6(P, tail[p], drec, p, v);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 6; c++) v[c] = drec[p * 8 + c];
+    }
     d0[p] = v[0];
     d1[p] = v[1];
     d2[p] = v[2];
@@ -1117,8 +1114,7 @@ ForceArgs<R, CT, FO> force_args(Context &c, const hdm_force_params &fp) {
     a.nbrCount = c.nbrCount.as<int32_t>(); a.nbr = c.nbr.as<int32_t>(); a.nbrWidth = c.maxNeigh;
     a.linkCount = c.linkCount.as<uint32_t>(); a.slotLink = c.slotLink.as<int32_t>();
     a.slotOther = c.slotOther.as<int32_t>(); a.linkWidth = c.maxLinks;
-    a.c0 = c.ct[0].as<CT>(); a.c1 = c.ct[1].as<CT>(); a.c2 = c.ct[2].as<CT>();
-    a.c3 = c.ct[3].as<CT>(); a.c4 = c.ct[4].as<CT>(); a.c5 = c.ct[5].as<CT>();
+    a.ct = c.ct.as<CT>();
     a.f0 = c.force[0].as<FO>(); a.f1 = c.force[1].as<FO>(); a.f2 = c.force[2].as<FO>();
     a.f3 = c.force[3].as<FO>(); a.f4 = c.force[4].as<FO>(); a.f5 = c.force[5].as<FO>();
     a.fp = cast_params<R>(fp);
@@ -1239,8 +1235,10 @@ StepRecs<R> step_recs(Context &c, double tame, const hdm_break_params *brk) {
     using Rec = typename StepRecs<R>::Rec;
     r.rec = c.rec.as<Rec>();
     r.drec = c.drec.as<double>();
+    r.tail = c.dtail.as<MixedTail>();
     r.recOut = c.recAlt.as<Rec>();
     r.drecOut = c.drecAlt.as<double>();
+    r.tailOut = c.dtailAlt.as<MixedTail>();
     r.tame = tame;
     r.slotThr = c.slotThr.as<float>();
     r.l0Max = c.l0Max;
@@ -1263,6 +1261,12 @@ hipError_t ensure_records(Context &c) {
         hipError_t e = b->ensure(sizeof(double) * (size_t)drec_stride(c.prec) * n);
         if (e != hipSuccess) return e;
     }
+    if (c.prec != HDM_PREC_FP64) {
+        for (DevBuf *b : {&c.dtail, &c.dtailAlt}) {
+            hipError_t e = b->ensure(sizeof(MixedTail) * n);
+            if (e != hipSuccess) return e;
+        }
+    }
     return hipSuccess;
 }
 
@@ -1275,6 +1279,7 @@ hipError_t launch_pack_records(Context &c, double tame) {
     // pack writes the "out" side; swap so the packed records are current
     swap_buf(c.rec, c.recAlt);
     swap_buf(c.drec, c.drecAlt);
+    swap_buf(c.dtail, c.dtailAlt);
     auto go = [&](auto r) {
         hipLaunchKernelGGL(k_pack_records, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
                            c.y.as<double>(), c.z.as<double>(), c.tx.as<double>(), c.ty.as<double>(),
@@ -1291,6 +1296,7 @@ hipError_t launch_pack_records(Context &c, double tame) {
         StepRecs<float> r = step_recs<float>(c, tame, nullptr);
         r.recOut = c.rec.as<MixedRec>();
         r.drecOut = c.drec.as<double>();
+        r.tailOut = c.dtail.as<MixedTail>();
         go(r);
     }
     e = hipGetLastError();
@@ -1306,6 +1312,7 @@ void swap_step_state(Context &c) {
     swap_buf(c.tz, c.alt[5]);
     swap_buf(c.rec, c.recAlt);
     swap_buf(c.drec, c.drecAlt);
+    swap_buf(c.dtail, c.dtailAlt);
 }
 
 hipError_t sync_state(Context &c) {
@@ -1314,9 +1321,8 @@ hipError_t sync_state(Context &c) {
     if (c.n == 0) return hipSuccess;
     auto go = [&](auto *rec) {
         using Rec = typename std::remove_pointer<decltype(rec)>::type;
-        constexpr int stride = std::is_same<Rec, double4>::value ? 8 : 6;
-        hipLaunchKernelGGL((k_unpack_records<Rec, stride>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, rec,
-                           c.drec.as<double>(), c.n, c.x.as<double>(), c.y.as<double>(), c.z.as<double>(),
+        hipLaunchKernelGGL((k_unpack_records<Rec>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, rec,
+                           c.drec.as<double>(), c.dtail.as<MixedTail>(), c.n, c.x.as<double>(), c.y.as<double>(), c.z.as<double>(),
                            c.diff[0].as<double>(), c.diff[1].as<double>(), c.diff[2].as<double>(),
                            c.diff[3].as<double>(), c.diff[4].as<double>(), c.diff[5].as<double>());
     };

@@ -431,6 +431,57 @@ def test_mixed_fused_step_matches_per_call_force_with_linked_contacts(squeezed):
         assert bits_equal(fused[f], per_call[f]), f
 
 
+def split_is_exact(D):
+    """Whether an FP64 Diff comes back bit for bit from its FP32 rounding and
+    the FP32 rounding of the remainder (else the fused FP32 form keeps the
+    particle's translation Diff in its FP64 escape array)."""
+    hi = D.astype(np.float32).astype(np.float64)
+    lo = (D - hi).astype(np.float32).astype(np.float64)
+    return (hi + lo).view(np.uint64) == D.view(np.uint64)
+
+
+def test_mixed_fused_steps_match_per_call_steps(squeezed):
+    """The fused FP32-form step keeps each particle's FP64 translation Diff as
+    an FP32 high and low part, or in an FP64 escape array when the two do not
+    give it back exactly.  Fused steps stay bit-identical to per-call force +
+    integrate steps on FP64 SoA state, checked every 5 steps over 25: from
+    random FP64 Diffs (most do not split at the first step), with three
+    particles starting at exactly 0 in x, y and z (near zero a Diff needs
+    more than 48 bits, so they keep taking the escape)."""
+    s, p0, l0 = squeezed
+    p = perturb(p0, 47)
+    for f, k in (("xCord", 11), ("yCord", 222), ("zCord", 1333)):
+        p.arrays[f][:] -= p.arrays[f][k]
+    assert split_is_exact(np.asarray(p["xDiff"])).mean() < 0.5
+    rp = A.HdmRunParams.from_buffer_copy(s.run)
+    rp.policy = A.HDM_REBUILD_EXACT
+    rp.breakTime = 1e30
+    say("mixed fused steps vs per-call steps, escaped Diffs")
+    fused, per_call, rebuilds = [], [], 0
+    with new_ctx(p, l0, A.HDM_PREC_MIXED) as ctx:
+        ctx.links_build()
+        for _ in range(5):
+            rebuilds += ctx.run(rp, 5).rebuilds
+            fused.append(ctx.download_particles())
+    assert rebuilds == 1
+    with new_ctx(p, l0, A.HDM_PREC_MIXED) as ctx:
+        ctx.links_build()
+        ctx.grid_build(s.search_distance)
+        ctx.neighbours_build(s.interact_dist)
+        for _ in range(5):
+            for _ in range(5):
+                ctx.force(s.force)
+                ctx.integrate(s.integ)
+            per_call.append(ctx.download_particles())
+    inexact = 0
+    for k, (a, b) in enumerate(zip(fused, per_call)):
+        for f in A.PARTICLE_FIELDS:
+            assert bits_equal(a[f], b[f]), (5 * (k + 1), f)
+        inexact += sum(int((~split_is_exact(np.asarray(b[f]))).sum()) for f in ("xDiff", "yDiff", "zDiff"))
+    say("  translation Diff components without an exact split at the checks:", inexact)
+    assert inexact > 0
+
+
 def test_mixed_runs_with_and_without_partner_list(squeezed, orc):
     """Fused mixed runs (rebuilds included) with the partner-free list and
     with the reference's list stay within FP32 rounding of each other and of
