@@ -138,6 +138,52 @@ class HdmSchedParams(C.Structure):
     _fields_ = [("blockCells", C.c_int32), ("minBatch", C.c_int32), ("maxBatch", C.c_int32), ("safety", C.c_float)]
 
 
+class HdmMap2d(C.Structure):
+    _fields_ = [("xPoints", C.c_int32), ("yPoints", C.c_int32)] + \
+               [(f, C.c_float) for f in ("xMin", "yMin", "xMax", "yMax", "xOffset", "yOffset")] + \
+               [("data", C.POINTER(C.c_double))]
+
+
+HDM_MAP_BED, HDM_MAP_FRICTION, HDM_MAP_EXT, HDM_MAP_COUNT = 0, 1, 2, 8
+
+
+@dataclass
+class Map2d:
+    """A 2-D field on a regular grid (hdm_map2d): values[iy, ix] over
+    [x_min, x_max] x [y_min, y_max]; offsets as Map2d::setOffset."""
+    values: np.ndarray
+    x_min: float
+    y_min: float
+    x_max: float
+    y_max: float
+    x_offset: float = 0.0
+    y_offset: float = 0.0
+
+    @classmethod
+    def constant(cls, v):
+        """One point: the single-value form of the reference's maps."""
+        return cls(np.array([[float(v)]]), 0.0, 0.0, 1.0, 1.0)
+
+    def struct(self):
+        """hdm_map2d over this map's values (kept alive by the returned pair)."""
+        v = np.ascontiguousarray(self.values, dtype=np.float64)
+        m = HdmMap2d(v.shape[1], v.shape[0], self.x_min, self.y_min, self.x_max, self.y_max,
+                     self.x_offset, self.y_offset, v.ctypes.data_as(C.POINTER(C.c_double)))
+        return m, v
+
+
+def load_map(path: str, field: int) -> Map2d:
+    """Map2dLoader::readInput + get through hdm_map2d_load."""
+    L = lib()
+    m = HdmMap2d()
+    check(L.hdm_map2d_load(os.fsencode(path), field, C.byref(m)), what=f"hdm_map2d_load({path}, {field})")
+    try:
+        v = np.ctypeslib.as_array(m.data, (m.yPoints, m.xPoints)).copy()
+    finally:
+        L.hdm_map2d_free(C.byref(m))
+    return Map2d(v, m.xMin, m.yMin, m.xMax, m.yMax)
+
+
 class HdmMlpTrainParams(C.Structure):
     _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
                 ("beta2", C.c_float), ("eps", C.c_float)]
@@ -212,6 +258,9 @@ def _declare(L):
                            C.POINTER(HdmMlpTrainParams), _dp], C.c_int),
         "hdm_sched_stats": ([P, _i64p, _i64p, _i64p], C.c_int),
         "hdm_set_exec": ([P, C.POINTER(HdmExecParams)], C.c_int),
+        "hdm_set_maps": ([P, C.POINTER(C.POINTER(HdmMap2d))], C.c_int),
+        "hdm_map2d_load": ([C.c_char_p, C.c_int32, C.POINTER(HdmMap2d)], C.c_int),
+        "hdm_map2d_free": ([C.POINTER(HdmMap2d)], None),
         "hdm_sched_label_begin": ([P, C.c_int32, C.POINTER(HdmRunParams)], C.c_int),
         "hdm_sched_label_track": ([P], C.c_int),
         "hdm_sched_label_end": ([P, C.c_int32, C.c_int32, _i32p, _i32p], C.c_int),
@@ -560,6 +609,17 @@ class Context:
         self._c(lib().hdm_sched_label_end(self._h, slot, capacity, _ptr(out, C.c_int32), C.byref(nb)),
                 "hdm_sched_label_end")
         return out[:nb.value]
+
+    def set_maps(self, bed: "Map2d | None" = None, friction: "Map2d | None" = None, ext=None):
+        """GlacierDem map inputs (hdm_set_maps); ext: six maps or None entries
+        (x, y, z, u, v, w force).  Replaces every slot."""
+        slots = [bed, friction] + list(ext if ext is not None else [None] * 6)
+        assert len(slots) == HDM_MAP_COUNT
+        keep = [m.struct() if m is not None else None for m in slots]
+        arr = (C.POINTER(HdmMap2d) * HDM_MAP_COUNT)()
+        for k, kv in enumerate(keep):
+            arr[k] = C.pointer(kv[0]) if kv is not None else C.POINTER(HdmMap2d)()
+        self._c(lib().hdm_set_maps(self._h, arr), "hdm_set_maps")
 
     def set_exec(self, autotune=False, graphs=False, variant=-1, list_with_partners=False):
         ep = HdmExecParams(int(autotune), int(graphs), variant, int(list_with_partners))

@@ -3,6 +3,8 @@
 // on device data owned by the context; see include/hidem/hdm_abi.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -666,6 +668,130 @@ int hdm_set_blocks(hdm_ctx *ctx, const int32_t *blockId, int32_t nBlocks, const 
     return HDM_OK;
 }
 
+int hdm_set_maps(hdm_ctx *ctx, const hdm_map2d *const *maps) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    hdm::MapDev desc[HDM_MAP_COUNT] = {};
+    int32_t mask = 0;
+    for (int k = 0; maps && k < HDM_MAP_COUNT; k++) {
+        const hdm_map2d *m = maps[k];
+        if (!m) continue;
+        if (m->xPoints < 1 || m->yPoints < 1 || !m->data || (int64_t)m->xPoints * m->yPoints > (1ll << 30))
+            return fail(ctx, HDM_E_INVALID, "hdm_set_maps: a map needs at least one point per axis and its data");
+        const size_t n = (size_t)m->xPoints * (size_t)m->yPoints;
+        HIP_CHECK(ctx, c.mapData[k].ensure(sizeof(double) * n));
+        HIP_CHECK(ctx, hipMemcpyAsync(c.mapData[k].ptr, m->data, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+        hdm::MapDev &d = desc[k];
+        d.xPoints = m->xPoints;
+        d.yPoints = m->yPoints;
+        d.xMin = m->xMin;
+        d.yMin = m->yMin;
+        // Map2d constructor (Map2d.h:56-59), float arithmetic
+        d.xRes = (m->xMax - m->xMin) / (float)(m->xPoints - 1);
+        d.yRes = (m->yMax - m->yMin) / (float)(m->yPoints - 1);
+        d.invXMaxMin = 1.0f / (m->xMax - m->xMin);
+        d.invYMaxMin = 1.0f / (m->yMax - m->yMin);
+        d.xOffset = m->xOffset;
+        d.yOffset = m->yOffset;
+        d.data = c.mapData[k].as<double>();
+        mask |= 1 << k;
+    }
+    HIP_CHECK(ctx, c.mapDesc.ensure(sizeof(desc)));
+    HIP_CHECK(ctx, hipMemcpyAsync(c.mapDesc.ptr, desc, sizeof(desc), hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
+    c.mapMask = mask;
+    return HDM_OK;
+}
+
+int hdm_map2d_load(const char *path, int32_t fieldID, hdm_map2d *map) {
+    if (!path || !map || fieldID < 0) return HDM_E_INVALID;
+    *map = hdm_map2d{};
+    FILE *f = std::fopen(path, "r");
+    if (!f) return HDM_E_INVALID;
+    // Map2dLoader::readInput (Map2dLoader.cpp:8-86): whitespace-separated
+    // floats, every row with the first row's column count; the extents from
+    // the x and y columns; a grid row per distinct y, all of equal length
+    std::vector<std::vector<float>> cols;
+    std::vector<float> row;
+    std::vector<std::pair<float, int64_t>> rowCounts;  // distinct y values in first-seen order
+    char line[1 << 14];
+    bool ok = true;
+    while (ok && std::fgets(line, sizeof(line), f)) {
+        row.clear();
+        for (char *t = std::strtok(line, " \t\r\n"); t; t = std::strtok(nullptr, " \t\r\n")) {
+            char *end = nullptr;
+            const float v = std::strtof(t, &end);
+            if (end == t) {
+                ok = false;
+                break;
+            }
+            row.push_back(v);
+        }
+        if (!ok) break;
+        if (row.empty()) continue;
+        if (cols.empty()) cols.resize(row.size());
+        if (row.size() != cols.size() || row.size() < 2) {
+            ok = false;
+            break;
+        }
+        for (size_t j = 0; j < row.size(); j++) cols[j].push_back(row[j]);
+        bool seen = false;
+        for (auto &rc : rowCounts)
+            if (rc.first == row[1]) {
+                rc.second++;
+                seen = true;
+            }
+        if (!seen) rowCounts.emplace_back(row[1], 1);
+    }
+    std::fclose(f);
+    if (!ok || cols.empty() || (size_t)fieldID >= cols.size()) return HDM_E_INVALID;
+    const int64_t xPoints = rowCounts.front().second, yPoints = (int64_t)rowCounts.size();
+    for (auto &rc : rowCounts)
+        if (rc.second != xPoints) return HDM_E_INVALID;
+    // the extents start where the reference's do: min from the largest
+    // float, max from the smallest positive normal float (so the max of an
+    // all-negative column reads 1.2e-38 there too)
+    float xMin = FLT_MAX, xMax = FLT_MIN, yMin = FLT_MAX, yMax = FLT_MIN;
+    for (size_t i = 0; i < cols[0].size(); i++) {
+        xMin = std::min(xMin, cols[0][i]);
+        xMax = std::max(xMax, cols[0][i]);
+        yMin = std::min(yMin, cols[1][i]);
+        yMax = std::max(yMax, cols[1][i]);
+    }
+    if (xPoints * yPoints != (int64_t)cols[0].size()) return HDM_E_INVALID;
+    double *data = new (std::nothrow) double[(size_t)(xPoints * yPoints)];
+    if (!data) return HDM_E_NOMEM;
+    // Map2d constructor: every point -1 until set; Map2d::setPointCord
+    // (Map2d.h:65-74) places each row at its rounded grid index
+    for (int64_t i = 0; i < xPoints * yPoints; i++) data[i] = -1;
+    const float xRes = (xMax - xMin) / (float)(xPoints - 1), yRes = (yMax - yMin) / (float)(yPoints - 1);
+    for (size_t i = 0; i < cols[0].size(); i++) {
+        float x = (cols[0][i] - xMin) / xRes, y = (cols[1][i] - yMin) / yRes;
+        x = std::round(x);
+        y = std::round(y);
+        const int64_t ix = xPoints > 1 ? (int64_t)x : 0, iy = yPoints > 1 ? (int64_t)y : 0;
+        if (ix < 0 || ix >= xPoints || iy < 0 || iy >= yPoints) {
+            delete[] data;
+            return HDM_E_INVALID;
+        }
+        data[ix + iy * xPoints] = (double)cols[(size_t)fieldID][i];
+    }
+    map->xPoints = (int32_t)xPoints;
+    map->yPoints = (int32_t)yPoints;
+    map->xMin = xMin;
+    map->yMin = yMin;
+    map->xMax = xMax;
+    map->yMax = yMax;
+    map->data = data;
+    return HDM_OK;
+}
+
+void hdm_map2d_free(hdm_map2d *map) {
+    if (!map) return;
+    delete[] map->data;
+    map->data = nullptr;
+}
+
 int hdm_bw_probe(hdm_ctx *ctx, int32_t kind, int64_t bytes, int32_t iters, double *gbps) {
     if (!valid(ctx) || (kind != HDM_PROBE_COPY && kind != HDM_PROBE_CT_RMW) || bytes < (1 << 20) || iters < 1)
         return HDM_E_INVALID;
@@ -833,6 +959,8 @@ std::vector<unsigned char> graph_key(const Context &c, const hdm_run_params *rp,
     put(&c.maxNeigh, sizeof(c.maxNeigh));
     put(&c.maxLinks, sizeof(c.maxLinks));
     put(&c.nBlocks, sizeof(c.nBlocks));
+    put(&c.mapMask, sizeof(c.mapMask));
+    put(&c.mapDesc.ptr, sizeof(void *));
     put(&c.stepVariant, sizeof(c.stepVariant));
     put(&c.recTame, sizeof(c.recTame));
     put(&c.hasMaxLoad, sizeof(c.hasMaxLoad));

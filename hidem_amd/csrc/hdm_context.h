@@ -29,6 +29,14 @@
 namespace hdm {
 
 // Raw device allocation that grows on demand (contents are not preserved).
+// Device descriptor of an hdm_map2d (Map2d's fields, with xRes, yRes and
+// the inverse extents computed in float as its constructor does).
+struct MapDev {
+    int32_t xPoints, yPoints;
+    float xMin, yMin, xRes, yRes, invXMaxMin, invYMaxMin, xOffset, yOffset;
+    const double *data;
+};
+
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
@@ -128,6 +136,10 @@ struct Context {
     // batched form: block id per particle, waterline per block (nBlocks = 0: unused)
     int32_t nBlocks = 0;
     DevBuf blockId, blockWater;
+    // GlacierDem map inputs (hdm_set_maps): descriptors (MapDev[HDM_MAP_COUNT])
+    // and their values; bit k of mapMask: slot k holds a map
+    int32_t mapMask = 0;
+    DevBuf mapDesc, mapData[HDM_MAP_COUNT];
     // links
     int64_t L = 0;
     bool hasMaxLoad = false;

@@ -45,6 +45,19 @@ struct GeneralParams {
     hdm_integ_params integrator(ArgParser &parser) const;
 };
 
+// GlacierDem map inputs named by a config: the bed's base and friction fields
+// ([[collisionGeometry]] type = "bed") and external forces given as
+// {path, fieldID} tables ([externalForce]); slot = HDM_MAP_*.
+struct MapFile {
+    int slot;
+    std::string path;
+    int32_t fieldID;
+};
+std::vector<MapFile> map_inputs(ArgParser &parser);
+// Loads the files (hdm_map2d_load) and binds them to the context
+// (hdm_set_maps); throws TomlError naming what failed.
+void set_map_inputs(hdm_ctx *ctx, const std::vector<MapFile> &files);
+
 // The pure part of HiDEMLib::setup: everything derived from the config.
 struct SetupInfo {
     hdm_setup_info info;

@@ -11,7 +11,62 @@ namespace hdm {
 
 namespace {
 template <class T> T arg_or(ArgParser &p, const std::string &name, T dflt) { return p.getArg<T>(name).value_or(dflt); }
+
+// The TOML table at "section.key" when it is a {path, fieldID} map table
+// (ExternalForceBuilder::createForce, ExternalForces.h:87-123), else null.
+const TomlValue *map_spec(ArgParser &a, const std::string &name) {
+    const size_t dot = name.find('.');
+    const std::string sec = name.substr(0, dot), key = name.substr(dot + 1);
+    if (!a.table.contains(sec) || !a.table.at(sec).contains(key)) return nullptr;
+    const TomlValue &v = a.table.at(sec).at(key);
+    return v.is_table() ? &v : nullptr;
+}
 }  // namespace
+
+std::vector<MapFile> map_inputs(ArgParser &a) {
+    std::vector<MapFile> out;
+    // [[collisionGeometry]] type = "bed": base and friction fields of one table
+    // (GeometryCollection::parseCollisionTable, GeometryCollection.cpp:281-289)
+    if (a.table.contains("collisionGeometry")) {
+        const TomlValue &arr = a.table.at("collisionGeometry");
+        for (const TomlValue &g : arr.arr) {
+            if (!g.contains("type") || g.at("type").s != "bed") continue;
+            if (!g.contains("path") || !g.contains("baseFieldID") || !g.contains("frictionFieldID"))
+                throw TomlError("collisionGeometry bed needs path, baseFieldID and frictionFieldID");
+            out.push_back({HDM_MAP_BED, g.at("path").s, (int32_t)toml_as<int64_t>(g.at("baseFieldID"))});
+            out.push_back({HDM_MAP_FRICTION, g.at("path").s, (int32_t)toml_as<int64_t>(g.at("frictionFieldID"))});
+        }
+    }
+    const char *en[6] = {"externalForce.xForce", "externalForce.yForce", "externalForce.zForce",
+                         "externalForce.uForce", "externalForce.vForce", "externalForce.wForce"};
+    for (int k = 0; k < 6; k++) {
+        const TomlValue *m = map_spec(a, en[k]);
+        if (!m) continue;
+        if (!m->contains("path") || !m->contains("fieldID")) throw TomlError(std::string(en[k]) + " needs path and fieldID");
+        out.push_back({HDM_MAP_EXT + k, m->at("path").s, (int32_t)toml_as<int64_t>(m->at("fieldID"))});
+    }
+    return out;
+}
+
+void set_map_inputs(hdm_ctx *ctx, const std::vector<MapFile> &files) {
+    hdm_map2d maps[HDM_MAP_COUNT] = {};
+    const hdm_map2d *slots[HDM_MAP_COUNT] = {};
+    int rc = HDM_OK;
+    std::string failed;
+    for (const MapFile &f : files) {
+        hdm_map2d_free(&maps[f.slot]);
+        rc = hdm_map2d_load(f.path.c_str(), f.fieldID, &maps[f.slot]);
+        if (rc != HDM_OK) {
+            failed = f.path;
+            break;
+        }
+        slots[f.slot] = &maps[f.slot];
+    }
+    if (rc == HDM_OK) rc = hdm_set_maps(ctx, slots);
+    for (auto &m : maps) hdm_map2d_free(&m);
+    if (!failed.empty()) throw TomlError("cannot read map " + failed);
+    if (rc != HDM_OK) throw TomlError(std::string("hdm_set_maps: ") + hdm_last_error(ctx));
+}
 
 // MainIceBlock.cpp:25-51 (defaults) and :99-121 (scaling / derived values);
 // MainDem.cpp:116-138 is identical.  Evaluated in double, in the reference's
@@ -45,7 +100,8 @@ GeneralParams GeneralParams::read(ArgParser &a) {
     for (int k = 0; k < 6; k++) g.bound[k] = arg_or<bool>(a, bn[k], false);
     const char *en[6] = {"externalForce.xForce", "externalForce.yForce", "externalForce.zForce",
                          "externalForce.uForce", "externalForce.vForce", "externalForce.wForce"};
-    for (int k = 0; k < 6; k++) g.ext[k] = arg_or<double>(a, en[k], 0.0);
+    // a single value each; a {path, fieldID} table is a map (load_maps)
+    for (int k = 0; k < 6; k++) g.ext[k] = map_spec(a, en[k]) ? 0.0 : arg_or<double>(a, en[k], 0.0);
 
     const double scale = g.scale;
     g.beamWidth *= scale;
@@ -367,6 +423,17 @@ int hdm_sim_create(int32_t precision, int32_t argc, const char *const *argv, hdm
         s->integ = si.info.run.integ;
         s->lib.maxMinutes = s->args.getArg<unsigned int>("iteration.maxMinutes").value_or((unsigned int)-1);
         s->lib.setup(si.info);
+        if (s->integ.variant == HDM_INTEG_GLACIER) {
+            std::vector<hdm::MapFile> files = hdm::map_inputs(s->args);
+            if (!files.empty()) {
+                try {
+                    hdm::set_map_inputs(s->lib.accel.ctx, files);
+                } catch (...) {
+                    delete s;
+                    throw;
+                }
+            }
+        }
         *out = s;
         return HDM_OK;
     } catch (const std::exception &e) {

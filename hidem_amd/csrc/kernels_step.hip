@@ -381,6 +381,8 @@ struct IntegArgs {
     const double *x0, *y0, *z0;   // nullptr: no displacement reduction
     const int32_t *blockId;       // batched form: waterline of the particle's block
     const double *blockWater;     // (nullptr: ip.waterLevel for every particle)
+    const MapDev *maps;           // GlacierDem map inputs, slot k valid when bit k of mapMask
+    int32_t mapMask;
 };
 
 // Damped central difference (HiDEMDefTimeIntegration, HiDEMDefaultFunctions.h
@@ -412,14 +414,66 @@ __device__ __forceinline__ void central_difference(double dragCoeff, double mass
     }
 }
 
+// Map2d::getIndexes (Map2d.h:92-108) for one axis: floor of the scaled
+// coordinate, held within [-1, n] before the int conversion (so NaN and huge
+// coordinates stay defined; the clamp to [0, n - 1] below treats them as the
+// reference does any value beyond the grid).
+__device__ __forceinline__ int32_t map_floor(float f, int32_t n) {
+    f = floorf(f);
+    if (!(f >= -1.0f)) f = -1.0f;
+    if (f > (float)n) f = (float)n;
+    return (int32_t)f;
+}
+
+__device__ __forceinline__ int32_t map_clamp(int32_t v, int32_t lo, int32_t hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Map2d::interp (Map2d.h:110-145): FP32 bilinear interpolation in the
+// reference's expression order; the grid value itself where both axes clamp
+// to one point.
+__device__ double map_interp(const MapDev &m, float x, float y) {
+    x += m.xOffset;
+    y += m.yOffset;
+    const int32_t xPoint = map_floor(((x - m.xMin) * m.invXMaxMin) * (float)(m.xPoints - 1), m.xPoints);
+    const int32_t yPoint = map_floor(((y - m.yMin) * m.invYMaxMin) * (float)(m.yPoints - 1), m.yPoints);
+    const int32_t xLow = map_clamp(xPoint, 0, m.xPoints - 1), yLow = map_clamp(yPoint, 0, m.yPoints - 1);
+    const int32_t xHigh = map_clamp(xPoint + 1, 0, m.xPoints - 1), yHigh = map_clamp(yPoint + 1, 0, m.yPoints - 1);
+    if (xLow == xHigh && yLow == yHigh) return m.data[xLow + yLow * m.xPoints];
+    const float v00 = (float)m.data[xLow + yLow * m.xPoints];
+    const float v01 = (float)m.data[xLow + yHigh * m.xPoints];
+    const float v10 = (float)m.data[xHigh + yLow * m.xPoints];
+    const float v11 = (float)m.data[xHigh + yHigh * m.xPoints];
+    const float x0 = x - m.xMin;
+    const float y0 = y - m.yMin;
+    const float x1 = (float)xLow * m.xRes;
+    const float x2 = (float)xHigh * m.xRes;
+    const float y1 = (float)yLow * m.yRes;
+    const float y2 = (float)yHigh * m.yRes;
+    const float invX21 = 1.0f / (x2 - x1);
+    const float c1 = (x2 - x0) * invX21;
+    const float c2 = (x0 - x1) * invX21;
+    float fxy1 = (c1 * v00) + (c2 * v10);
+    float fxy2 = (c1 * v01) + (c2 * v11);
+    if (xLow == xHigh) {
+        fxy1 = v00;
+        fxy2 = v01;
+    } else if (yLow == yHigh) {
+        return fxy1;
+    }
+    const float invY21 = 1.0f / (y2 - y1);
+    return (((y2 - y0) * invY21) * fxy1) + (((y0 - y1) * invY21) * fxy2);
+}
+
 // applyForce: IceBlock (MainIceBlock.cpp:299-400, clamp = HDM_TYPE_FIXED) or
-// GlacierDem (MainDem.cpp:145-259 with an analytic bed, constant external
-// force, HiDEMDefVelocityLimit and HiDEMDefBoundaryCondition), on one
+// GlacierDem (MainDem.cpp:145-259: bed height, friction and external force
+// from maps (bit k of mapMask: maps[k]) or the analytic bed and constant
+// force of ip, HiDEMDefVelocityLimit and HiDEMDefBoundaryCondition), on one
 // particle's state in registers: cord/theta/diff are replaced by the new
 // coordinates, rotations and Diff.
-__device__ __forceinline__ void integrate_state(const hdm_integ_params &ip, bool fixed, double radius,
-                                                double waterLevel, const double Fin[6], double cord[3],
-                                                double theta[3], double diff[6]) {
+__device__ __forceinline__ void integrate_state(const hdm_integ_params &ip, const MapDev *maps, int32_t mapMask,
+                                                bool fixed, double radius, double waterLevel, const double Fin[6],
+                                                double cord[3], double theta[3], double diff[6]) {
     if (ip.variant == HDM_INTEG_ICEBLOCK && fixed) {
 #pragma unroll
         for (int c = 0; c < 6; c++) diff[c] = 0;
@@ -439,8 +493,13 @@ __device__ __forceinline__ void integrate_state(const hdm_integ_params &ip, bool
         central_difference(dragCoeff, particleMass, particleMomentInertia, ip, F, diff, cord, cordPrev, theta,
                            thetaPrev, vel);
     } else {
-        const double bedzCord = (cord[0] < ip.bedXEnd) ? ip.bedZ : ip.deepZ;
-        const double fric = (cord[0] < ip.frictionXEnd) ? 1.0 : 0.0;
+        // InputBed::getHeight / getFric (InputGeometry.h:57-67): the
+        // coordinates go to Map2d::interp as float
+        const float fx = (float)cord[0], fy = (float)cord[1];
+        const double bedzCord = (mapMask & (1 << HDM_MAP_BED)) ? map_interp(maps[HDM_MAP_BED], fx, fy)
+                                                               : ((cord[0] < ip.bedXEnd) ? ip.bedZ : ip.deepZ);
+        const double fric = (mapMask & (1 << HDM_MAP_FRICTION)) ? map_interp(maps[HDM_MAP_FRICTION], fx, fy)
+                                                                : ((cord[0] < ip.frictionXEnd) ? 1.0 : 0.0);
         if ((zCord + radius) - bedzCord < ip.bedInteract) {
             double d = ip.bedDrag * fric;
             dragCoeff = (ip.frictionCap < d) ? ip.frictionCap : d;
@@ -452,8 +511,13 @@ __device__ __forceinline__ void integrate_state(const hdm_integ_params &ip, bool
         if (zCord > waterLevel) F[2] += -ip.g * particleMass;
         if (zCord <= waterLevel) F[2] += ip.lift * particleMass;
         if (zCord + radius < bedzCord) F[2] += ip.bedStiffness * (bedzCord - (zCord + radius));
+        if (mapMask >> HDM_MAP_EXT) {
+            for (int c = 0; c < 6; c++)
+                F[c] += (mapMask & (1 << (HDM_MAP_EXT + c))) ? map_interp(maps[HDM_MAP_EXT + c], fx, fy) : ip.ext[c];
+        } else {
 #pragma unroll
-        for (int c = 0; c < 6; c++) F[c] += ip.ext[c];
+            for (int c = 0; c < 6; c++) F[c] += ip.ext[c];
+        }
         central_difference(dragCoeff, particleMass, particleMomentInertia, ip, F, diff, cord, cordPrev, theta,
                            thetaPrev, vel);
         if (ip.velocityLimit) {
@@ -490,7 +554,7 @@ __device__ __forceinline__ void integrate_particle(const IntegArgs &a, int64_t i
     double diff[6] = {a.d0[i], a.d1[i], a.d2[i], a.d3[i], a.d4[i], a.d5[i]};
     const bool fixed = (a.type[i] & HDM_TYPE_FIXED) != 0;
     const double waterLevel = a.blockWater ? a.blockWater[a.blockId[i]] : a.ip.waterLevel;
-    integrate_state(a.ip, fixed, a.radius[i], waterLevel, Fin, cord, theta, diff);
+    integrate_state(a.ip, a.maps, a.mapMask, fixed, a.radius[i], waterLevel, Fin, cord, theta, diff);
     a.ox[i] = cord[0];
     a.oy[i] = cord[1];
     a.oz[i] = cord[2];
@@ -926,7 +990,7 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
         double theta[3] = {ld_stream(g.tx + p), ld_stream(g.ty + p), ld_stream(g.tz + p)};
         const bool fixed = (ld_stream(g.type + p) & HDM_TYPE_FIXED) != 0;
         const double waterLevel = g.blockWater ? g.blockWater[g.blockId[p]] : g.ip.waterLevel;
-        integrate_state(g.ip, fixed, radius, waterLevel, Fd, cord, theta, diff);
+        integrate_state(g.ip, g.maps, g.mapMask, fixed, radius, waterLevel, Fd, cord, theta, diff);
         st_stream(g.otx + p, theta[0]);
         st_stream(g.oty + p, theta[1]);
         st_stream(g.otz + p, theta[2]);
@@ -1100,6 +1164,8 @@ IntegArgs integ_args(Context &c, const hdm_integ_params &ip, uint64_t *skin, boo
     a.z0 = withDisplacement ? c.z0.as<double>() : nullptr;
     a.blockId = c.nBlocks > 0 ? c.blockId.as<int32_t>() : nullptr;
     a.blockWater = c.nBlocks > 0 ? c.blockWater.as<double>() : nullptr;
+    a.mapMask = ip.variant == HDM_INTEG_GLACIER ? c.mapMask : 0;
+    a.maps = a.mapMask ? c.mapDesc.as<MapDev>() : nullptr;
     return a;
 }
 

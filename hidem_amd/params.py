@@ -34,6 +34,28 @@ GENERAL_DEFAULTS = {
 POLICIES = {"heuristic": A.HDM_REBUILD_HEURISTIC, "exact": A.HDM_REBUILD_EXACT, "interval": A.HDM_REBUILD_INTERVAL}
 
 
+EXT_FORCES = ("xForce", "yForce", "zForce", "uForce", "vForce", "wForce")
+
+
+def map_inputs(cfg: dict) -> list:
+    """GlacierDem map slots a config names (hdm_set_maps order: bed, friction,
+    external force x..w; None where the analytic input applies): the bed's
+    base and friction fields of a [[collisionGeometry]] table with type
+    "bed", external forces given as {path, fieldID} tables.  Paths as written
+    (relative to the working directory, like the reference)."""
+    slots = [None] * A.HDM_MAP_COUNT
+    for g in cfg.get("collisionGeometry", []):
+        if g.get("type") == "bed":
+            slots[A.HDM_MAP_BED] = A.load_map(g["path"], int(g["baseFieldID"]))
+            slots[A.HDM_MAP_FRICTION] = A.load_map(g["path"], int(g["frictionFieldID"]))
+    ext = cfg.get("externalForce", {})
+    for k, name in enumerate(EXT_FORCES):
+        v = ext.get(name)
+        if isinstance(v, dict):
+            slots[A.HDM_MAP_EXT + k] = A.load_map(v["path"], int(v["fieldID"]))
+    return slots
+
+
 def load_config(name_or_path: str) -> dict:
     path = name_or_path
     if not os.path.exists(path):
@@ -101,8 +123,9 @@ def derive(cfg: dict) -> Setup:
     ip.deepZ = hdm.get("deepZ", -1e300)
     ip.frictionXEnd = hdm.get("frictionXEnd", 1e300)
     ext = cfg.get("externalForce", {})
-    for k, name in enumerate(("xForce", "yForce", "zForce", "uForce", "vForce", "wForce")):
-        ip.ext[k] = float(ext.get(name, 0.0))
+    for k, name in enumerate(EXT_FORCES):
+        v = ext.get(name, 0.0)
+        ip.ext[k] = 0.0 if isinstance(v, dict) else float(v)  # a {path, fieldID} table is a map (map_inputs)
     ip.maxVel, ip.maxRot = g["maxVel"], g["maxRot"]
     ip.velocityLimit = 1 if ip.variant == A.HDM_INTEG_GLACIER else 0
     bounds = ("xBoundaryMin", "xBoundaryMax", "yBoundaryMin", "yBoundaryMax", "zBoundaryMin", "zBoundaryMax")

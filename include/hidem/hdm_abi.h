@@ -387,6 +387,45 @@ int  hdm_gen_batch_thresholds(const hdm_batch_spec *spec, const hdm_batch_block 
  * (replaces hdm_integ_params.waterLevel per particle); nBlocks = 0 clears. */
 int  hdm_set_blocks(hdm_ctx *ctx, const int32_t *blockId, int32_t nBlocks, const double *waterLevel);
 
+/* ---- GlacierDem map inputs ----------------------------------------------------
+ * A 2-D field on a regular xPoints x yPoints grid over [xMin, xMax] x
+ * [yMin, yMax] (Map2d, Map2d.h:22-61), data[ix + iy * xPoints] (host
+ * memory, lpReal_t values), read at (x, y) by Map2d::interp (Map2d.h:110-145):
+ * bilinear interpolation in FP32 of the four grid values around the point,
+ * indexes clamped to the grid, the value itself where both axes clamp to one
+ * point (so a 1 x 1 map is a constant, as ExternalForceBuilder makes for a
+ * single value, ExternalForces.h:111-121).  xOffset, yOffset: Map2d::setOffset. */
+typedef struct hdm_map2d {
+    int32_t xPoints, yPoints;
+    float xMin, yMin, xMax, yMax;
+    float xOffset, yOffset;
+    const double *data;
+} hdm_map2d;
+
+/* Slots of hdm_set_maps: bed height and friction (InputBed, InputGeometry.h
+ * :53-67), external force x, y, z, u, v, w (ExternalForces, ExternalForces.h
+ * :24-70). */
+enum { HDM_MAP_BED = 0, HDM_MAP_FRICTION = 1, HDM_MAP_EXT = 2, HDM_MAP_COUNT = 8 };
+
+/* Map-driven inputs of the HDM_INTEG_GLACIER integrator (MainDem.cpp:176-217:
+ * bed->getHeight, bed->getFric, extForcePtr->xForce->interp ...).  maps[k]
+ * NULL keeps the analytic input of hdm_integ_params for that slot (bedZ /
+ * bedXEnd / deepZ, frictionXEnd, ext[k]).  The data is copied to the device;
+ * a later call replaces every slot (all NULL, or maps NULL, clears).
+ * HDM_E_INVALID for a map with fewer than 1 point per axis or no data. */
+int  hdm_set_maps(hdm_ctx *ctx, const hdm_map2d *const *maps);
+
+/* Map2dLoader::readInput + get (Map2dLoader.cpp:8-86, Map2dLoader.h:29-40):
+ * reads a whitespace-separated table whose first two columns are x and y
+ * (rows with the same y form one grid row, every row the same length) and
+ * fills *map with column fieldID on the grid the x and y columns span (each
+ * row placed at its rounded grid index, Map2d::setPointCord).  map->data is
+ * owned by the library until hdm_map2d_free.  HDM_E_INVALID for a missing or
+ * malformed file (a row with another column count, rows of unequal length, a
+ * field index beyond the columns). */
+int  hdm_map2d_load(const char *path, int32_t fieldID, hdm_map2d *map);
+void hdm_map2d_free(hdm_map2d *map);
+
 /* ---- TOML-driven simulation (HiDEMLib::setup + the sample main loop) --------
  * argv holds the command line after the program name, as the reference's
  * ArgParser expects it (ArgParser.h:170-196): optional --section.key=value

@@ -442,6 +442,73 @@ void hdmo_force(hdmo_particles *p, hdmo_links *l, const hdmo_neigh *nl,
 /* Damped central-difference update shared by both samples
  * (HiDEMDefTimeIntegration, HiDEMDefaultFunctions.h:317-370, identical to the
  * inline IceBlock form MainIceBlock.cpp:353-385). Returns the velocities. */
+/* ---- 2-D maps (Map2d.h) --------------------------------------------------- */
+void hdmo_map_init(hdmo_map2d *m)
+{
+    /* Map2d.h:56-59, float arithmetic */
+    m->xRes = (m->xMax - m->xMin) / (float)(m->xPoints - 1);
+    m->yRes = (m->yMax - m->yMin) / (float)(m->yPoints - 1);
+    m->invXMaxMin = 1.0f / (m->xMax - m->xMin);
+    m->invYMaxMin = 1.0f / (m->yMax - m->yMin);
+}
+
+/* floor of a float as an int, for indexes that the clamp below bounds: values
+ * beyond [-1, n] (and NaN) clamp to those, so the conversion stays defined. */
+static int32_t map_floor(float f, int32_t n)
+{
+    f = floorf(f);
+    if (!(f >= -1.0f)) f = -1.0f;
+    if (f > (float)n) f = (float)n;
+    return (int32_t)f;
+}
+
+static int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+void hdmo_map_indexes(const hdmo_map2d *m, float x, float y, int32_t out[4])
+{
+    /* Map2d.h:96-106 */
+    int32_t xPoint = map_floor(((x - m->xMin) * m->invXMaxMin) * (float)(m->xPoints - 1), m->xPoints);
+    int32_t yPoint = map_floor(((y - m->yMin) * m->invYMaxMin) * (float)(m->yPoints - 1), m->yPoints);
+    out[0] = clampi(xPoint, 0, m->xPoints - 1);
+    out[1] = clampi(yPoint, 0, m->yPoints - 1);
+    out[2] = clampi(xPoint + 1, 0, m->xPoints - 1);
+    out[3] = clampi(yPoint + 1, 0, m->yPoints - 1);
+}
+
+double hdmo_map_interp(const hdmo_map2d *m, float x, float y)
+{
+    /* Map2d.h:110-145 */
+    x += m->xOffset;
+    y += m->yOffset;
+    int32_t ix[4];
+    hdmo_map_indexes(m, x, y, ix);
+    int32_t xLow = ix[0], yLow = ix[1], xHigh = ix[2], yHigh = ix[3];
+    if (xLow == xHigh && yLow == yHigh) return m->data[xLow + yLow * m->xPoints];
+    float v00 = (float)m->data[xLow + yLow * m->xPoints];
+    float v01 = (float)m->data[xLow + yHigh * m->xPoints];
+    float v10 = (float)m->data[xHigh + yLow * m->xPoints];
+    float v11 = (float)m->data[xHigh + yHigh * m->xPoints];
+    float x0 = x - m->xMin;
+    float y0 = y - m->yMin;
+    float x1 = (float)xLow * m->xRes;
+    float x2 = (float)xHigh * m->xRes;
+    float y1 = (float)yLow * m->yRes;
+    float y2 = (float)yHigh * m->yRes;
+    float invX21 = 1.0f / (x2 - x1);
+    float c1 = (x2 - x0) * invX21;
+    float c2 = (x0 - x1) * invX21;
+    float fxy1 = (c1 * v00) + (c2 * v10);
+    float fxy2 = (c1 * v01) + (c2 * v11);
+    if (xLow == xHigh) {
+        fxy1 = v00;
+        fxy2 = v01;
+    } else if (yLow == yHigh) {
+        return fxy1;
+    }
+    float invY21 = 1.0f / (y2 - y1);
+    return (((y2 - y0) * invY21) * fxy1) + (((y0 - y1) * invY21) * fxy2);
+}
+
 static void central_difference(double dragCoeff, double mass, double inertia, const hdmo_integ_params *ip,
                                const double F[6], const double diff[6], double cord[3], double cordPrev[3],
                                double theta[3], double thetaPrev[3], double vel[6])
@@ -500,8 +567,14 @@ void hdmo_integrate(hdmo_particles *p, const hdmo_integ_params *ip)
             if (zCord <= waterLevel) F[2] += ip->lift * particleMass;
             central_difference(dragCoeff, particleMass, particleMomentInertia, ip, F, diff, cord, cordPrev, theta, thetaPrev, vel);
         } else {
-            double bedzCord = (cord[0] < ip->bedXEnd) ? ip->bedZ : ip->deepZ;
-            double fric = (cord[0] < ip->frictionXEnd) ? 1.0 : 0.0;
+            /* InputBed::getHeight / getFric (InputGeometry.h:57-67) take the
+             * coordinates as lpReal_t and Map2d::interp as float */
+            double bedzCord = ip->maps[HDMO_MAP_BED]
+                                  ? hdmo_map_interp(ip->maps[HDMO_MAP_BED], (float)cord[0], (float)cord[1])
+                                  : ((cord[0] < ip->bedXEnd) ? ip->bedZ : ip->deepZ);
+            double fric = ip->maps[HDMO_MAP_FRICTION]
+                              ? hdmo_map_interp(ip->maps[HDMO_MAP_FRICTION], (float)cord[0], (float)cord[1])
+                              : ((cord[0] < ip->frictionXEnd) ? 1.0 : 0.0);
             if ((zCord + radius) - bedzCord < ip->bedInteract) {
                 double a = ip->bedDrag * fric;
                 dragCoeff = (ip->frictionCap < a) ? ip->frictionCap : a;   /* std::min(a, cap) */
@@ -513,7 +586,11 @@ void hdmo_integrate(hdmo_particles *p, const hdmo_integ_params *ip)
             if (zCord > waterLevel) F[2] += -ip->g * particleMass;
             if (zCord <= waterLevel) F[2] += ip->lift * particleMass;
             if (zCord + radius < bedzCord) F[2] += ip->bedStiffness * (bedzCord - (zCord + radius));
-            for (int c = 0; c < 6; c++) F[c] += ip->ext[c];
+            /* ExternalForces maps (MainDem.cpp:210-216) */
+            for (int c = 0; c < 6; c++) {
+                const hdmo_map2d *m = ip->maps[HDMO_MAP_EXT + c];
+                F[c] += m ? hdmo_map_interp(m, (float)cord[0], (float)cord[1]) : ip->ext[c];
+            }
             central_difference(dragCoeff, particleMass, particleMomentInertia, ip, F, diff, cord, cordPrev, theta, thetaPrev, vel);
             if (ip->velocityLimit) {
                 for (int c = 0; c < 3; c++) {

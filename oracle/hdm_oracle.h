@@ -98,6 +98,29 @@ typedef struct hdmo_force_params {
     double E, W2, W4, G;
 } hdmo_force_params;
 
+/* A 2-D field on a regular xPoints x yPoints grid over [xMin, xMax] x
+ * [yMin, yMax] (Map2d, Map2d.h:22-61): data[ix + iy * xPoints]; xRes, yRes,
+ * invXMaxMin, invYMaxMin as its constructor computes them (in float, see
+ * hdmo_map_init); xOffset, yOffset as set by Map2d::setOffset. */
+typedef struct hdmo_map2d {
+    int32_t xPoints, yPoints;
+    float xMin, yMin, xMax, yMax;
+    float xRes, yRes, invXMaxMin, invYMaxMin;
+    float xOffset, yOffset;
+    const double *data;
+} hdmo_map2d;
+
+/* Map2d constructor arithmetic (Map2d.h:43-61) for the derived fields. */
+void hdmo_map_init(hdmo_map2d *m);
+/* Map2d::getIndexes (Map2d.h:92-108): xLow, yLow, xHigh, yHigh. */
+void hdmo_map_indexes(const hdmo_map2d *m, float x, float y, int32_t out[4]);
+/* Map2d::interp (Map2d.h:110-145), FP32 bilinear interpolation; the value
+ * itself (not rounded to float) where both axes clamp to one point. */
+double hdmo_map_interp(const hdmo_map2d *m, float x, float y);
+
+/* GlacierDem map inputs, indexes into hdmo_integ_params.maps. */
+enum { HDMO_MAP_BED = 0, HDMO_MAP_FRICTION = 1, HDMO_MAP_EXT = 2, HDMO_MAP_COUNT = 8 };
+
 /* Time integration parameters: IceBlock applyForce (MainIceBlock.cpp:299-400)
  * and GlacierDem applyForce (MainDem.cpp:145-259). */
 typedef struct hdmo_integ_params {
@@ -116,6 +139,9 @@ typedef struct hdmo_integ_params {
      * blockWater[blockId[i]] when blockWater != NULL */
     const int32_t *blockId;
     const double *blockWater;
+    /* GlacierDem map inputs (NULL: the analytic bed / friction above, ext[k]):
+     * bed height, friction, external force x, y, z, u, v, w */
+    const hdmo_map2d *maps[HDMO_MAP_COUNT];
 } hdmo_integ_params;
 
 /* HiDEMDefLinkBreakFunc (HiDEMDefaultFunctions.h:207-242). */

@@ -54,6 +54,15 @@ class ForceParams(C.Structure):
                                           "dampingRotation", "scaleE", "E", "W2", "W4", "G")]
 
 
+class Map2d(C.Structure):
+    _fields_ = [("xPoints", C.c_int32), ("yPoints", C.c_int32)] + \
+               [(f, C.c_float) for f in ("xMin", "yMin", "xMax", "yMax", "xRes", "yRes", "invXMaxMin", "invYMaxMin",
+                                         "xOffset", "yOffset")] + [("data", _dp)]
+
+
+MAP_COUNT = 8
+
+
 class IntegParams(C.Structure):
     _fields_ = ([("variant", C.c_int32)] +
                 [(f, C.c_double) for f in ("rhoIce", "g", "lift", "waterLevel", "airDrag", "waterDrag",
@@ -63,7 +72,8 @@ class IntegParams(C.Structure):
                 [("ext", C.c_double * 6), ("maxVel", C.c_double), ("maxRot", C.c_double),
                  ("velocityLimit", C.c_int32), ("bound", C.c_int32 * 6),
                  ("lower", C.c_double * 3), ("upper", C.c_double * 3),
-                 ("blockId", C.POINTER(C.c_int32)), ("blockWater", C.POINTER(C.c_double))])
+                 ("blockId", C.POINTER(C.c_int32)), ("blockWater", C.POINTER(C.c_double)),
+                 ("maps", C.POINTER(Map2d) * MAP_COUNT)])
 
 
 class BreakParams(C.Structure):
@@ -115,6 +125,12 @@ def lib(fast=False):
         L.hdmo_skin_heuristic.restype = C.c_double
         L.hdmo_skin_exact.argtypes = [C.POINTER(Particles), _dp, _dp, _dp]
         L.hdmo_skin_exact.restype = C.c_double
+        L.hdmo_map_init.argtypes = [C.POINTER(Map2d)]
+        L.hdmo_map_init.restype = None
+        L.hdmo_map_indexes.argtypes = [C.POINTER(Map2d), C.c_float, C.c_float, C.POINTER(C.c_int32)]
+        L.hdmo_map_indexes.restype = None
+        L.hdmo_map_interp.argtypes = [C.POINTER(Map2d), C.c_float, C.c_float]
+        L.hdmo_map_interp.restype = C.c_double
         L.hdmo_sim_init.argtypes = [C.POINTER(Sim)]
         L.hdmo_sim_run.argtypes = [C.POINTER(Sim), C.c_int64, _u8p, _i64p, C.c_int64]
         L.hdmo_sim_run.restype = C.c_int64
@@ -162,15 +178,41 @@ def force_params(fp):
     return ForceParams(*[getattr(fp, f) for f, _ in ForceParams._fields_])
 
 
-def integ_params(ip, block_id=None, block_water=None):
+def map_struct(values, x_min, y_min, x_max, y_max, x_offset=0.0, y_offset=0.0):
+    """Map2d over values[iy, ix] (float64; kept alive by the returned pair),
+    derived fields from hdmo_map_init."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    m = Map2d(v.shape[1], v.shape[0], x_min, y_min, x_max, y_max, 0, 0, 0, 0, x_offset, y_offset,
+              v.ctypes.data_as(_dp))
+    lib().hdmo_map_init(C.byref(m))
+    return m, v
+
+
+def map_interp(m, x, y):
+    """Map2d::interp of one point (x, y rounded to float as the callers do)."""
+    return lib().hdmo_map_interp(C.byref(m), C.c_float(x), C.c_float(y))
+
+
+def map_indexes(m, x, y):
+    out = (C.c_int32 * 4)()
+    lib().hdmo_map_indexes(C.byref(m), C.c_float(x), C.c_float(y), out)
+    return tuple(out)
+
+
+def integ_params(ip, block_id=None, block_water=None, maps=None):
     """Copies an hdm_integ_params; optional per-particle block ids and
-    per-block waterlines (numpy arrays kept alive by the caller)."""
+    per-block waterlines (numpy arrays kept alive by the caller) and the
+    GlacierDem map slots (a list of 8 Map2d structs or None, kept alive by
+    the caller)."""
     o = IntegParams()
     if block_id is not None:
         o.blockId = block_id.ctypes.data_as(C.POINTER(C.c_int32))
         o.blockWater = block_water.ctypes.data_as(C.POINTER(C.c_double))
+    for k, m in enumerate(maps or []):
+        if m is not None:
+            o.maps[k] = C.pointer(m)
     for f, _ in IntegParams._fields_:
-        if f in ("blockId", "blockWater"):
+        if f in ("blockId", "blockWater", "maps"):
             continue
         v = getattr(ip, f)
         if f in ("ext", "bound", "lower", "upper"):
@@ -245,9 +287,12 @@ def link_lists(n_particles, l):
 class OracleSim:
     """Owns copies of the particle/link arrays and steps them on the CPU."""
 
-    def __init__(self, p, l, setup, fast=False, block_id=None, block_water=None):
+    def __init__(self, p, l, setup, fast=False, block_id=None, block_water=None, maps=None):
+        """maps: GlacierDem map slots, 8 entries of (values, x_min, y_min,
+        x_max, y_max) or None (bed, friction, external force x..w)."""
         self.L = lib(fast)
         self._blocks = (block_id, block_water)
+        self._maps = [map_struct(*m) if m is not None else None for m in (maps or [])]
         self.p = p.copy()
         self.l = l.copy()
         self._ps = particles_struct(self.p)
@@ -256,7 +301,7 @@ class OracleSim:
         s.p = C.pointer(self._ps)
         s.l = C.pointer(self._ls)
         s.fp = force_params(setup.force)
-        s.ip = integ_params(setup.integ, block_id, block_water)
+        s.ip = integ_params(setup.integ, block_id, block_water, [m[0] if m else None for m in self._maps])
         s.bp = break_params(setup.brk)
         s.searchDistance = setup.search_distance
         s.interactDist = setup.interact_dist
