@@ -91,9 +91,11 @@ enum { HDM_INTEG_ICEBLOCK = 0, HDM_INTEG_GLACIER = 1 };
 
 /* applyForce integrator parameters.  HDM_INTEG_ICEBLOCK: MainIceBlock.cpp
  * applyForce (:299-400) plus the clamp (HDM_TYPE_FIXED).  HDM_INTEG_GLACIER:
- * MainDem.cpp applyForce (:145-259) with an analytic bed: height bedZ for
+ * MainDem.cpp applyForce (:145-259), HiDEMDefVelocityLimit and
+ * HiDEMDefBoundaryCondition, with the bed, friction and external force from
+ * the maps of hdm_set_maps where bound, else analytic: height bedZ for
  * x < bedXEnd else deepZ, friction 1 for x < frictionXEnd else 0, a constant
- * external force ext[6], HiDEMDefVelocityLimit and HiDEMDefBoundaryCondition. */
+ * external force ext[6]. */
 typedef struct hdm_integ_params {
     int32_t variant;
     double rhoIce, g, lift, waterLevel, airDrag, waterDrag;
