@@ -259,6 +259,7 @@ def _declare(L):
         "hdm_sched_stats": ([P, _i64p, _i64p, _i64p], C.c_int),
         "hdm_set_exec": ([P, C.POINTER(HdmExecParams)], C.c_int),
         "hdm_set_maps": ([P, C.POINTER(C.POINTER(HdmMap2d))], C.c_int),
+        "hdm_reorder": ([P, C.c_double, _i64p, _i64p], C.c_int),
         "hdm_map2d_load": ([C.c_char_p, C.c_int32, C.POINTER(HdmMap2d)], C.c_int),
         "hdm_map2d_free": ([C.POINTER(HdmMap2d)], None),
         "hdm_sched_label_begin": ([P, C.c_int32, C.POINTER(HdmRunParams)], C.c_int),
@@ -609,6 +610,15 @@ class Context:
         self._c(lib().hdm_sched_label_end(self._h, slot, capacity, _ptr(out, C.c_int32), C.byref(nb)),
                 "hdm_sched_label_end")
         return out[:nb.value]
+
+    def reorder(self, search_distance):
+        """hdm_reorder: returns (order, link_order), the old index of each new
+        particle / link."""
+        order = np.zeros(self.n, np.int64)
+        lorder = np.zeros(max(self.L, 1), np.int64)
+        self._c(lib().hdm_reorder(self._h, search_distance, _ptr(order, C.c_int64), _ptr(lorder, C.c_int64)),
+                "hdm_reorder")
+        return order, lorder[:self.L]
 
     def set_maps(self, bed: "Map2d | None" = None, friction: "Map2d | None" = None, ext=None):
         """GlacierDem map inputs (hdm_set_maps); ext: six maps or None entries

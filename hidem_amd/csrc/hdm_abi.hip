@@ -668,6 +668,37 @@ int hdm_set_blocks(hdm_ctx *ctx, const int32_t *blockId, int32_t nBlocks, const 
     return HDM_OK;
 }
 
+int hdm_reorder(hdm_ctx *ctx, double searchDistance, int64_t *order, int64_t *linkOrder) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!(searchDistance > 0)) return fail(ctx, HDM_E_INVALID, "hdm_reorder: searchDistance must be > 0");
+    if (c.n <= 1) return fail(ctx, HDM_E_STATE, "hdm_reorder: fewer than two particles");
+    // link endpoints are checked by the link build; the reorder indexes with them
+    if (c.L > 0 && !c.linksBuilt) {
+        int r = hdm_links_build(ctx);
+        if (r != HDM_OK) return r;
+    }
+    const bool relink = c.linksBuilt;
+    // CT back to checkpoint order first; the reorder permutes that copy
+    if (c.linksBuilt) HIP_CHECK(ctx, hdm::launch_ct_export(c));
+    hipError_t e = hdm::launch_reorder(c, searchDistance, order, linkOrder);
+    if (e == hipErrorInvalidValue) return fail(ctx, HDM_E_CAPACITY, "hdm_reorder: grid extent not finite or too large");
+    HIP_CHECK(ctx, e);
+    c.linksBuilt = false;
+    c.grid.built = false;
+    c.neighBuilt = false;
+    c.snapshotValid = false;
+    c.skinFused = false;
+    c.skinFusedDisp = false;
+    c.recCurrent = false;
+    c.soaStale = false;
+    c.altHoldsPrev = false;
+    c.schedAssigned = false;
+    for (auto &lab : c.labels) lab.active = false;
+    if (relink) return hdm_links_build(ctx);
+    return HDM_OK;
+}
+
 int hdm_set_maps(hdm_ctx *ctx, const hdm_map2d *const *maps) {
     if (!valid(ctx)) return HDM_E_INVALID;
     Context &c = ctx->c;

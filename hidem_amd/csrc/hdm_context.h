@@ -28,7 +28,6 @@
 
 namespace hdm {
 
-// Raw device allocation that grows on demand (contents are not preserved).
 // Device descriptor of an hdm_map2d (Map2d's fields, with xRes, yRes and
 // the inverse extents computed in float as its constructor does).
 struct MapDev {
@@ -37,6 +36,7 @@ struct MapDev {
     const double *data;
 };
 
+// Raw device allocation that grows on demand (contents are not preserved).
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
@@ -257,6 +257,10 @@ hipError_t launch_side_thresholds(Context &c, uint64_t *l0MaxBits);
 // a side to itself (after link_fill).
 hipError_t launch_partner_check(Context &c, uint64_t *flag);
 hipError_t launch_pack_records(Context &c, double tame);  // SoA state -> rec / drec
+// Redistribution order (hdm_reorder): permutes the SoA particle state, the
+// links and the CT stage (checkpoint order) on the device; order / linkOrder
+// (host, may be null) receive the old index of each new particle / link.
+hipError_t launch_reorder(Context &c, double searchDistance, int64_t *order, int64_t *linkOrder);
 // Brings x, y, z and diff[] up to date with the records after fused steps
 // (no-op otherwise).  Every launcher that reads them calls it first.
 hipError_t sync_state(Context &c);

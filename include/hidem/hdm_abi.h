@@ -389,6 +389,19 @@ int  hdm_gen_batch_thresholds(const hdm_batch_spec *spec, const hdm_batch_block 
  * (replaces hdm_integ_params.waterLevel per particle); nBlocks = 0 clears. */
 int  hdm_set_blocks(hdm_ctx *ctx, const int32_t *blockId, int32_t nBlocks, const double *waterLevel);
 
+/* Redistribution order on one rank (HiDEMDistribution.h:150-181): particles
+ * sorted by their cell colours on grids of spacing s, 2s, 4s, 8s (s =
+ * searchDistance; NeighbourGrid's bounding box and colours), coarsest first
+ * (ParallelSort::sortIndirectNLevel), ties by old index; links sorted by
+ * their old (A, B) (LinkData::getNewOrder, LinkData.h:166-181), ties by link
+ * index, endpoints renamed (reorderLinkTargets, :116-137), CT sides moved
+ * with their link.  Every particle and link array is permuted in place, so
+ * later downloads return the new order.  order[i] / linkOrder[j] (host, n /
+ * L entries, may be NULL) receive the old index of new particle i / link j.
+ * The grid and neighbour list must be built again (hdm_run does so on its
+ * first step); link lists built before are rebuilt by the call. */
+int  hdm_reorder(hdm_ctx *ctx, double searchDistance, int64_t *order, int64_t *linkOrder);
+
 /* ---- GlacierDem map inputs ----------------------------------------------------
  * A 2-D field on a regular xPoints x yPoints grid over [xMin, xMax] x
  * [yMin, yMax] (Map2d, Map2d.h:22-61), data[ix + iy * xPoints] (host

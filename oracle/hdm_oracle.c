@@ -118,6 +118,134 @@ int hdmo_grid_build(const hdmo_particles *p, double searchDistance, hdmo_grid *g
     return 0;
 }
 
+/* ---- redistribution order ---------------------------------------------------- */
+typedef struct { int32_t c[4]; int64_t i; } reorder_key;
+
+static int cmp_reorder_key(const void *pa, const void *pb)
+{
+    const reorder_key *a = (const reorder_key *)pa, *b = (const reorder_key *)pb;
+    for (int k = 3; k >= 0; k--)   /* spacing 8s first (grids[3]) */
+        if (a->c[k] != b->c[k]) return a->c[k] < b->c[k] ? -1 : 1;
+    return (a->i < b->i) ? -1 : (a->i > b->i);
+}
+
+typedef struct { int32_t a, b; int64_t i; } link_key;
+
+static int cmp_link_key(const void *pa, const void *pb)
+{
+    const link_key *x = (const link_key *)pa, *y = (const link_key *)pb;
+    if (x->a != y->a) return x->a < y->a ? -1 : 1;
+    if (x->b != y->b) return x->b < y->b ? -1 : 1;
+    return (x->i < y->i) ? -1 : (x->i > y->i);
+}
+
+static void permute_d(double *v, const int64_t *order, int64_t n, double *tmp)
+{
+    for (int64_t i = 0; i < n; i++) tmp[i] = v[order[i]];
+    memcpy(v, tmp, sizeof(double) * (size_t)n);
+}
+
+int hdmo_reorder(hdmo_particles *p, hdmo_links *l, double searchDistance, int64_t *newOrder, int64_t *linkOrder)
+{
+    const int64_t n = p->n;
+    if (n <= 1) return -1;
+    double xMin = p->x[0], xMax = p->x[0], yMin = p->y[0], yMax = p->y[0], zMin = p->z[0], zMax = p->z[0];
+    for (int64_t i = 1; i < n; i++) {
+        if (p->x[i] < xMin) xMin = p->x[i];
+        if (p->x[i] > xMax) xMax = p->x[i];
+        if (p->y[i] < yMin) yMin = p->y[i];
+        if (p->y[i] > yMax) yMax = p->y[i];
+        if (p->z[i] < zMin) zMin = p->z[i];
+        if (p->z[i] > zMax) zMax = p->z[i];
+    }
+    reorder_key *keys = (reorder_key *)malloc(sizeof(reorder_key) * (size_t)n);
+    const int64_t L = l ? l->n : 0;
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)((2 * L > n ? 2 * L : n) + 1));
+    int64_t *order = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t *inv = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    if (!keys || !tmp || !order || !inv) {
+        free(keys); free(tmp); free(order); free(inv);
+        return -2;
+    }
+    for (int k = 0; k < 4; k++) {
+        const double s = searchDistance * (double)(1 << k);
+        hdmo_grid g;
+        memset(&g, 0, sizeof(g));
+        g.X = 1 + (int)floor((xMax - xMin) / s);
+        g.Y = 1 + (int)floor((yMax - yMin) / s);
+        for (int64_t i = 0; i < n; i++) {
+            int ix = (int)floor((p->x[i] - xMin) / s);
+            int iy = (int)floor((p->y[i] - yMin) / s);
+            int iz = (int)floor((p->z[i] - zMin) / s);
+            keys[i].c[k] = (int32_t)grid_color(&g, ix, iy, iz);
+            keys[i].i = i;
+        }
+    }
+    qsort(keys, (size_t)n, sizeof(reorder_key), cmp_reorder_key);
+    for (int64_t i = 0; i < n; i++) {
+        order[i] = keys[i].i;
+        inv[keys[i].i] = i;
+    }
+    free(keys);
+    double *fields[] = {p->x, p->y, p->z, p->tx, p->ty, p->tz, p->dx, p->dy, p->dz, p->du, p->dv, p->dw,
+                        p->radius, p->fx, p->fy, p->fz, p->fu, p->fv, p->fw};
+    for (size_t f = 0; f < sizeof(fields) / sizeof(fields[0]); f++) permute_d(fields[f], order, n, tmp);
+    if (p->type) {
+        uint8_t *t = (uint8_t *)tmp;
+        for (int64_t i = 0; i < n; i++) t[i] = p->type[order[i]];
+        memcpy(p->type, t, (size_t)n);
+    }
+    if (L > 0) {
+        link_key *lk = (link_key *)malloc(sizeof(link_key) * (size_t)L);
+        int64_t *lo = (int64_t *)malloc(sizeof(int64_t) * (size_t)L);
+        int64_t *so = (int64_t *)malloc(sizeof(int64_t) * (size_t)(2 * L));
+        if (!lk || !lo || !so) {
+            free(lk); free(lo); free(so); free(tmp); free(order); free(inv);
+            return -2;
+        }
+        for (int64_t j = 0; j < L; j++) {
+            lk[j].a = l->a[j];
+            lk[j].b = l->b[j];
+            lk[j].i = j;
+        }
+        qsort(lk, (size_t)L, sizeof(link_key), cmp_link_key);
+        for (int64_t j = 0; j < L; j++) {
+            lo[j] = lk[j].i;
+            so[2 * j] = 2 * lk[j].i;
+            so[2 * j + 1] = 2 * lk[j].i + 1;
+        }
+        free(lk);
+        int32_t *ia = (int32_t *)tmp;
+        for (int64_t j = 0; j < L; j++) ia[j] = (int32_t)inv[l->a[lo[j]]];
+        int32_t *ib = (int32_t *)malloc(sizeof(int32_t) * (size_t)L);
+        uint8_t *st = (uint8_t *)malloc((size_t)L);
+        if (!ib || !st) {
+            free(ib); free(st); free(lo); free(so); free(tmp); free(order); free(inv);
+            return -2;
+        }
+        for (int64_t j = 0; j < L; j++) {
+            ib[j] = (int32_t)inv[l->b[lo[j]]];
+            st[j] = l->state[lo[j]];
+        }
+        memcpy(l->a, ia, sizeof(int32_t) * (size_t)L);
+        memcpy(l->b, ib, sizeof(int32_t) * (size_t)L);
+        memcpy(l->state, st, (size_t)L);
+        free(ib);
+        free(st);
+        permute_d(l->initialLen, lo, L, tmp);
+        if (l->maxLoad) permute_d(l->maxLoad, lo, L, tmp);
+        for (int c = 0; c < 6; c++) permute_d(l->ct[c], so, 2 * L, tmp);
+        if (linkOrder) memcpy(linkOrder, lo, sizeof(int64_t) * (size_t)L);
+        free(lo);
+        free(so);
+    }
+    if (newOrder) memcpy(newOrder, order, sizeof(int64_t) * (size_t)n);
+    free(tmp);
+    free(order);
+    free(inv);
+    return 0;
+}
+
 void hdmo_grid_free(hdmo_grid *g)
 {
     free(g->colours); free(g->order); free(g->occColour); free(g->bucketStart); free(g->bucketSize);

@@ -154,6 +154,21 @@ typedef struct hdmo_break_params {
 int  hdmo_grid_build(const hdmo_particles *p, double searchDistance, hdmo_grid *g);
 void hdmo_grid_free(hdmo_grid *g);
 
+/* ---- redistribution order (single rank) ----
+ * HiDEMDistribution's sort (HiDEMDistribution.h:150-181): particles ordered by
+ * their cell colours on four grids of spacing s, 2s, 4s, 8s (bounding box and
+ * colours as NeighbourGrid::generateNeighbourGrid / colourParticles,
+ * NeighbourGrid.cpp:72-161), coarsest first (ParallelSort::sortIndirectNLevel,
+ * ParallelSort.h:84-89), ties by index (the reference's sort leaves them
+ * unspecified); ParticleDataHandler::reorder (ParticleDataHandler.h:36-69).
+ * Links ordered by their old (A, B) (LinkData::getNewOrder, LinkData.h:166-
+ * 181), ties by link index, then renamed to the new particle indices
+ * (reorderLinkTargets, :116-137); each link's two CT sides move with it.
+ * newOrder[i] = old index of the new particle i, linkOrder[j] = old index of
+ * the new link j (either may be NULL).  Returns 0, -1 for fewer than two
+ * particles, -2 when out of memory. */
+int  hdmo_reorder(hdmo_particles *p, hdmo_links *l, double searchDistance, int64_t *newOrder, int64_t *linkOrder);
+
 /* ---- neighbour list (NeighAccel.h:25-137) ---- */
 int  hdmo_neigh_build(const hdmo_particles *p, const hdmo_grid *g, double interactDist, hdmo_neigh *nl);
 void hdmo_neigh_free(hdmo_neigh *nl);

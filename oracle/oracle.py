@@ -125,6 +125,8 @@ def lib(fast=False):
         L.hdmo_skin_heuristic.restype = C.c_double
         L.hdmo_skin_exact.argtypes = [C.POINTER(Particles), _dp, _dp, _dp]
         L.hdmo_skin_exact.restype = C.c_double
+        L.hdmo_reorder.argtypes = [C.POINTER(Particles), C.POINTER(Links), C.c_double, _i64p, _i64p]
+        L.hdmo_reorder.restype = C.c_int
         L.hdmo_map_init.argtypes = [C.POINTER(Map2d)]
         L.hdmo_map_init.restype = None
         L.hdmo_map_indexes.argtypes = [C.POINTER(Map2d), C.c_float, C.c_float, C.POINTER(C.c_int32)]
@@ -172,6 +174,20 @@ def links_struct(l):
     for k in range(6):
         s.ct[k] = _p(l.ct[k], C.c_double)
     return s
+
+
+def reorder(p, l, search_distance):
+    """hdmo_reorder on copies: (particles, links, new_order, link_order) with
+    new_order[i] = old index of new particle i, link_order[j] = old index of
+    new link j."""
+    q, m = p.copy(), l.copy()
+    ps, ls = particles_struct(q), links_struct(m)
+    order = np.zeros(p.n, np.int64)
+    lorder = np.zeros(max(l.n, 1), np.int64)
+    rc = lib().hdmo_reorder(C.byref(ps), C.byref(ls), search_distance, _p(order, C.c_int64), _p(lorder, C.c_int64))
+    if rc != 0:
+        raise RuntimeError(f"hdmo_reorder failed: {rc}")
+    return q, m, order, lorder[:l.n]
 
 
 def force_params(fp):

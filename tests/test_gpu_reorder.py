@@ -1,0 +1,98 @@
+"""hdm_reorder on the GPU: the redistribution order of the device (colour
+keys, hipCUB radix passes, gathers) equal to the CPU oracle's, array for
+array, and runs after a reorder -- from the start and in the middle of a
+run, with CT already in the side store -- bit-identical to the oracle doing
+the same."""
+import numpy as np
+import pytest
+
+from hidem_amd import _abi as A
+from hidem_amd import params
+from tests.test_reorder import shuffled
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+def system(cells=(10, 8, 6), seed=4):
+    s, p, l = params.build_inputs(params.load_config("iceblock_16k"), list(cells))
+    r = np.random.default_rng(seed)
+    for c in range(6):
+        l.ct[c][:] = r.normal(size=2 * l.n)
+    q, m = shuffled(p, l, seed)
+    return s, q, m
+
+
+def test_device_order_equals_oracle(orc):
+    s, p, l = system()
+    say("reorder vs oracle", p.n, l.n)
+    q, m, order, lorder = orc.reorder(p, l, s.search_distance)
+    with A.Context(A.HDM_PREC_FP64) as ctx:
+        ctx.upload(p, l)
+        go, glo = ctx.reorder(s.search_distance)
+        gp = ctx.download_particles()
+        gl = ctx.download_links()
+    assert np.array_equal(go, order)
+    assert np.array_equal(glo, lorder)
+    for f in A.PARTICLE_FIELDS:
+        assert bits_equal(gp[f], q[f]), f
+    assert np.array_equal(gl.a, m.a) and np.array_equal(gl.b, m.b) and np.array_equal(gl.state, m.state)
+    assert bits_equal(gl.initialLen, m.initialLen)
+    for c in range(6):
+        assert bits_equal(gl.ct[c], m.ct[c]), c
+    # consecutive particles are spatial neighbours again (the input was shuffled)
+    def step(d):
+        xyz = np.stack([np.asarray(d[f]) for f in ("xCord", "yCord", "zCord")], 1)
+        return np.mean(np.linalg.norm(np.diff(xyz, axis=0), axis=1))
+    assert step(gp) < 0.2 * step(p)
+
+
+@pytest.mark.parametrize("prec", [A.HDM_PREC_FP64, A.HDM_PREC_MIXED])
+def test_reorder_mid_run(prec, orc):
+    """50 steps, reorder (CT in the side store goes through the checkpoint
+    order and back), 50 more: FP64 bit-identical to the oracle doing the
+    same; the FP32 form bit-identical to its own run with the reorder done
+    through a download / oracle reorder / upload."""
+    s, p, l = system(seed=6)
+    say("reorder mid run", prec)
+    with A.Context(prec) as ctx:
+        ctx.upload(p, l)
+        ctx.links_build()
+        ctx.run(s.run, 50)
+        ctx.reorder(s.search_distance)
+        ctx.run(s.run, 50)
+        gp = ctx.download_particles()
+        gl = ctx.download_links()
+    if prec == A.HDM_PREC_FP64:
+        sim = orc.OracleSim(p, l, s)
+        sim.run(50)
+        q, m, _, _ = orc.reorder(sim.p, sim.l, s.search_distance)
+        sim2 = orc.OracleSim(q, m, s)
+        sim2.s.ts = 50
+        sim2.run(50)
+        want_p, want_l = sim2.p, sim2.l
+    else:
+        with A.Context(prec) as ctx:
+            ctx.upload(p, l)
+            ctx.links_build()
+            ctx.run(s.run, 50)
+            hp, hl = ctx.download_particles(), ctx.download_links()
+        q, m, _, _ = orc.reorder(hp, hl, s.search_distance)
+        with A.Context(prec) as ctx:
+            ctx.upload(q, m)
+            ctx.links_build()
+            ctx.set_timestep(50)
+            ctx.run(s.run, 50)
+            want_p, want_l = ctx.download_particles(), ctx.download_links()
+    for f in ("xCord", "yCord", "zCord", "xTheta", "zTheta", "xDiff", "wDiff"):
+        assert bits_equal(gp[f], want_p[f]), f
+    assert np.array_equal(gl.state, want_l.state)
+    for c in range(6):
+        assert bits_equal(gl.ct[c], want_l.ct[c]), c
