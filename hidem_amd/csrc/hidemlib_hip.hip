@@ -423,6 +423,17 @@ int hdm_sim_create(int32_t precision, int32_t argc, const char *const *argv, hdm
         s->integ = si.info.run.integ;
         s->lib.maxMinutes = s->args.getArg<unsigned int>("iteration.maxMinutes").value_or((unsigned int)-1);
         s->lib.setup(si.info);
+        // hdm.redistribute: the redistribution sort the reference's first
+        // gridService does (HiDEMLib.h:569-587), once, before the first step;
+        // off by default (the generator already emits a spatial order)
+        if (s->args.getArg<bool>("hdm.redistribute").value_or(false)) {
+            int rc = hdm_reorder(s->lib.accel.ctx, s->lib.searchDistance, nullptr, nullptr);
+            if (rc != HDM_OK) {
+                std::string msg = std::string("hdm_reorder: ") + hdm_last_error(s->lib.accel.ctx);
+                delete s;
+                throw hdm::TomlError(msg);
+            }
+        }
         if (s->integ.variant == HDM_INTEG_GLACIER) {
             std::vector<hdm::MapFile> files = hdm::map_inputs(s->args);
             if (!files.empty()) {

@@ -96,3 +96,27 @@ def test_reorder_mid_run(prec, orc):
     assert np.array_equal(gl.state, want_l.state)
     for c in range(6):
         assert bits_equal(gl.ct[c], want_l.ct[c]), c
+
+
+def test_driver_redistribute_matches_python():
+    """hdm.redistribute = true in the TOML driver: the reorder before the
+    first step, then the same run as hdm_reorder + hdm_run from Python."""
+    import os
+    cfg = os.path.join(params.CONFIG_DIR, "iceblock_16k.toml")
+    say("driver redistribute")
+    with A.Sim(cfg, ["--hdm.redistribute=true"], precision=A.HDM_PREC_FP64) as sim:
+        sim.steps(60)
+        q = sim.ctx.download_particles()
+        ql = sim.ctx.download_links(with_max_load=False)
+    s, p, l = params.build_inputs(params.load_config("iceblock_16k"))
+    with A.Context(A.HDM_PREC_FP64) as ctx:
+        ctx.upload(p, l)
+        ctx.links_build()
+        order, _ = ctx.reorder(s.search_distance)
+        ctx.run(s.run, 60)
+        g = ctx.download_particles()
+        gl = ctx.download_links(with_max_load=False)
+    assert not np.array_equal(order, np.arange(p.n))   # the reorder did something
+    for f in ("xCord", "yCord", "zCord", "xTheta", "xDiff"):
+        assert bits_equal(q[f], g[f]), f
+    assert np.array_equal(ql.state, gl.state) and np.array_equal(ql.a, gl.a)
