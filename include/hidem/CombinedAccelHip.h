@@ -213,6 +213,19 @@ public:
     void applyForce(const hdm_integ_params &ip) { hdm_check(ctx, hdm_integrate(ctx, &ip), "hdm_integrate"); }
 
     void sync() { hdm_check(ctx, hdm_sync(ctx), "hdm_sync"); }
+
+    // The redistribution sort (HiDEMDistribution.h:150-181) on this rank's
+    // data; order / linkOrder (may be null) receive the old index of each new
+    // particle / link, in place of the reference's global-key remapping.
+    void redistribute(double searchDistance, int64_t *order = nullptr, int64_t *linkOrder = nullptr) {
+        hdm_check(ctx, hdm_reorder(ctx, searchDistance, order, linkOrder), "hdm_reorder");
+    }
+
+    // GlacierDem map inputs (InputBed's bed and friction, ExternalForces):
+    // HDM_MAP_COUNT slots, null for the analytic input of hdm_integ_params.
+    void setMaps(const hdm_map2d *const maps[HDM_MAP_COUNT]) {
+        hdm_check(ctx, hdm_set_maps(ctx, maps), "hdm_set_maps");
+    }
 };
 
 }  // namespace hidem

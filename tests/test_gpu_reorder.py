@@ -120,3 +120,29 @@ def test_driver_redistribute_matches_python():
     for f in ("xCord", "yCord", "zCord", "xTheta", "xDiff"):
         assert bits_equal(q[f], g[f]), f
     assert np.array_equal(ql.state, gl.state) and np.array_equal(ql.a, gl.a)
+
+
+def test_reorder_batched_blocks(orc):
+    """Batched form: the per-particle block ids move with their particles, so
+    each particle keeps its block's waterline; FP64 run after the reorder
+    bit-identical to the oracle run on the reordered state."""
+    b = A.gen_batch(n_blocks=4, min_particles=800, max_particles=2500, seed=5)
+    s = params.derive(params.load_config("iceblock_16k"))
+    p, l = shuffled(b.particles, b.links, 8)
+    r = np.random.default_rng(8)
+    perm = r.permutation(b.particles.n)   # the same permutation shuffled() drew
+    bid = b.block_id[perm]
+    say("reorder batched", p.n)
+    with A.Context(A.HDM_PREC_FP64) as ctx:
+        ctx.upload(p, l)
+        A.set_blocks(ctx, bid, b.water)
+        ctx.links_build()
+        order, _ = ctx.reorder(s.search_distance)
+        ctx.run(s.run, 80)
+        gp = ctx.download_particles()
+    q, m, o2, _ = orc.reorder(p, l, s.search_distance)
+    assert np.array_equal(order, o2)
+    sim = orc.OracleSim(q, m, s, block_id=bid[order].astype(np.int32), block_water=b.water)
+    sim.run(80)
+    for f in ("xCord", "yCord", "zCord", "xTheta", "zDiff"):
+        assert bits_equal(gp[f], sim.p[f]), f
