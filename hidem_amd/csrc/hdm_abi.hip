@@ -505,17 +505,15 @@ int hdm_links_build(hdm_ctx *ctx) {
     HIP_CHECK(ctx, c.slotOther.ensure(sizeof(int32_t) * slots));
     HIP_CHECK(ctx, c.sidePos.ensure(sizeof(int32_t) * (size_t)(2 * c.L > 0 ? 2 * c.L : 1)));
     HIP_CHECK(ctx, c.ct.ensure(6 * c.ctBytes() * slots));
-    HIP_CHECK(ctx, c.slotThr.ensure(sizeof(float) * slots));
+    HIP_CHECK(ctx, c.slotThr.ensure(sizeof(uint16_t) * slots));
     HIP_CHECK(ctx, hdm::launch_link_fill(c));
     HIP_CHECK(ctx, hdm::launch_ct_import(c));
-    uint64_t *words = c.red.as<uint64_t>() + 48;  // [0] partner check flag, [1] largest L0 (float bits)
+    uint64_t *words = c.red.as<uint64_t>() + 48;  // [0] partner check flag, [1..2] largest L0 and threshold
     HIP_CHECK(ctx, hdm::launch_partner_check(c, words));
-    HIP_CHECK(ctx, hdm::launch_side_thresholds(c, words + 1));
-    uint64_t w[2] = {0, 0};
+    HIP_CHECK(ctx, hdm::launch_side_thresholds(c, words + 1, &c.l0Max, &c.thrScale));
+    uint64_t w[1] = {0};
     HIP_CHECK(ctx, d2h(w, words, sizeof(w), c.stream));
     c.partnersUnique = w[0] == 0;
-    const uint32_t l0Bits = (uint32_t)w[1];
-    std::memcpy(&c.l0Max, &l0Bits, sizeof(float));
     c.linksBuilt = true;
     if (c.nbrExcludesLinked) c.neighBuilt = false;  // the partner set may have changed
     return HDM_OK;
@@ -995,6 +993,8 @@ std::vector<unsigned char> graph_key(const Context &c, const hdm_run_params *rp,
     put(&c.stepVariant, sizeof(c.stepVariant));
     put(&c.recTame, sizeof(c.recTame));
     put(&c.hasMaxLoad, sizeof(c.hasMaxLoad));
+    put(&c.l0Max, sizeof(c.l0Max));
+    put(&c.thrScale, sizeof(c.thrScale));
     put(&skin, sizeof(skin));
     put(&rp->force, sizeof(rp->force));
     put(&rp->integ, sizeof(rp->integ));

@@ -151,7 +151,8 @@ struct Context {
     DevBuf linkCount, linkCursor, slotLink, slotOther;
     DevBuf ct;             // CT0..5 of every slot, 6 x 64 values per (wave tile, side) block (ct_offset)
     DevBuf sidePos;        // 2L: tiled slot of side (l, end)
-    DevBuf slotThr;        // float per slot: break prefilter threshold of the side's link (k_side_thresholds)
+    DevBuf slotThr;        // u16 per slot: break prefilter threshold code of the side's link (k_side_codes)
+    float thrScale = 0.0f; // threshold = code * thrScale (at most the link's L0 (1 + maxLoad), or L0)
     float l0Max = 0.0f;    // largest initialLen (rounded up to float)
     // neighbour list (wave tiles)
     int32_t maxNeigh = 0;
@@ -250,9 +251,10 @@ hipError_t launch_link_count(Context &c);
 hipError_t launch_link_fill(Context &c);
 hipError_t launch_ct_import(Context &c);   // ctStage (checkpoint order) -> ct (tiled)
 hipError_t launch_ct_export(Context &c);   // ct (tiled) -> ctStage (checkpoint order)
-// initialLen / maxLoad -> slotThr, and the largest initialLen as float bits
-// into *l0MaxBits (after link_fill)
-hipError_t launch_side_thresholds(Context &c, uint64_t *l0MaxBits);
+// initialLen / maxLoad -> slotThr codes (after link_fill); maxBits: 2 device
+// words of scratch; returns the largest initialLen (rounded up) and the code
+// scale
+hipError_t launch_side_thresholds(Context &c, uint64_t *maxBits, float *l0Max, float *scale);
 // *flag != 0 after completion when a particle has two sides to one partner or
 // a side to itself (after link_fill).
 hipError_t launch_partner_check(Context &c, uint64_t *flag);

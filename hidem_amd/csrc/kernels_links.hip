@@ -15,6 +15,8 @@
 //                is what a 64-lane wavefront reads in one access).  sidePos
 //                plays the role of sideIndEndA/B (LinkData::reindexSideData,
 //                LinkData.h:183-208).
+#include <cstring>
+
 #include "hdm_context.h"
 #include "kernel_common.h"
 
@@ -124,40 +126,70 @@ __global__ void __launch_bounds__(TPB) k_ct_export(const int32_t *__restrict__ s
     s5[i] = (double)ct[ct_offset<CT>(d, 5)];
 }
 
-// Per-side float threshold of the break prefilter, in the side's wave-tile
-// slot (one coalesced 4 B load per side; the exact test still reads the FP64
-// values): L0 (1 + maxLoad) with per-link maxLoad, else L0; NaN when L0 is
-// below 1e-30 or not finite.  Also the largest L0 (as float bits, rounded up;
-// +inf when some L0 is not finite), which bounds the bend term.
-__global__ void __launch_bounds__(TPB) k_side_thresholds(const int32_t *__restrict__ sidePos, int64_t nSides,
-                                                          const double *__restrict__ initialLen,
-                                                          const double *__restrict__ maxLoad,
-                                                          float *__restrict__ slotThr, uint64_t *l0MaxBits) {
+// Break prefilter thresholds, in two passes after the link build.
+// Pass 1 (per link): the largest L0 (as float bits, rounded up; +inf when
+// some L0 is not finite), which bounds the bend term, and the largest
+// threshold T = L0 (1 + maxLoad) with per-link maxLoad, else L0 (float bits,
+// rounded up), which sets the scale of the codes.
+__device__ __forceinline__ float float_up(double v) {
+    const float f = (float)v;
+    return (double)f < v ? __uint_as_float(__float_as_uint(f) + 1u) : f;  // next float up
+}
+
+__device__ __forceinline__ void block_max_bits(float v, uint64_t *out) {
     __shared__ uint32_t part[TPB / 64];
-    float big = 0.0f;
-    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < nSides; i += (int64_t)gridDim.x * TPB) {
-        const int64_t l = i >> 1;
-        const double L0 = initialLen[l];
-        const bool ok = L0 >= 1e-30 && L0 <= 1e300;
-        const double T = maxLoad ? L0 * (1.0 + maxLoad[l]) : L0;
-        slotThr[sidePos[i]] = ok ? (float)T : __builtin_nanf("");
-        float b = 0.0f;
-        if (ok) {
-            const float f = (float)L0;
-            b = (double)f < L0 ? __uint_as_float(__float_as_uint(f) + 1u) : f;  // next float up
-        } else if (!(L0 < 1e-30)) {
-            b = __builtin_inff();  // NaN or huge: no useful bound
-        }
-        big = b > big ? b : big;
-    }
     // non-negative floats order as their bit patterns
-    const uint32_t bits = wave_max_u32(__float_as_uint(big));
+    const uint32_t bits = wave_max_u32(__float_as_uint(v));
+    __syncthreads();
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = bits;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t m = part[0];
         for (int w = 1; w < TPB / 64; w++) m = part[w] > m ? part[w] : m;
-        if (m) atomicMax((unsigned long long *)l0MaxBits, (unsigned long long)m);
+        if (m) atomicMax((unsigned long long *)out, (unsigned long long)m);
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_threshold_max(int64_t L, const double *__restrict__ initialLen,
+                                                        const double *__restrict__ maxLoad, uint64_t *maxBits) {
+    float bigL0 = 0.0f, bigT = 0.0f;
+    for (int64_t l = (int64_t)blockIdx.x * TPB + threadIdx.x; l < L; l += (int64_t)gridDim.x * TPB) {
+        const double L0 = initialLen[l];
+        if (L0 >= 1e-30 && L0 <= 1e300) {
+            const double T = maxLoad ? L0 * (1.0 + maxLoad[l]) : L0;
+            const float b = float_up(L0);
+            bigL0 = b > bigL0 ? b : bigL0;
+            if (T > 0.0 && T <= 1e300) {
+                const float t = float_up(T);
+                bigT = t > bigT ? t : bigT;
+            }
+        } else if (!(L0 < 1e-30)) {
+            bigL0 = __builtin_inff();  // NaN or huge: no useful bound
+        }
+    }
+    block_max_bits(bigL0, maxBits);
+    block_max_bits(bigT, maxBits + 1);
+}
+
+// Pass 2 (per side, into its wave-tile slot): a 16-bit code with
+// code * scale <= T, code = floor(T / scale) held to 65535, where scale is
+// the largest T / 65535 (so codes resolve T to 1.5e-5 of the largest); 0
+// (never passes) when L0 is below 1e-30, not finite, or no scale exists.
+// The prefilter reads 2 B per side; the exact test still reads the FP64
+// initialLen and maxLoad.
+__global__ void __launch_bounds__(TPB) k_side_codes(const int32_t *__restrict__ sidePos, int64_t nSides,
+                                                     const double *__restrict__ initialLen,
+                                                     const double *__restrict__ maxLoad, double scale,
+                                                     uint16_t *__restrict__ slotThr) {
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < nSides; i += (int64_t)gridDim.x * TPB) {
+        const int64_t l = i >> 1;
+        const double L0 = initialLen[l];
+        const bool ok = L0 >= 1e-30 && L0 <= 1e300 && scale > 0.0;
+        const double T = maxLoad ? L0 * (1.0 + maxLoad[l]) : L0;
+        double q = ok ? floor(T / scale) : 0.0;
+        if (!(q >= 0.0)) q = 0.0;  // also NaN
+        if (q > 65535.0) q = 65535.0;
+        slotThr[sidePos[i]] = (uint16_t)q;
     }
 }
 
@@ -241,15 +273,32 @@ hipError_t launch_ct_export(Context &c) {
     return hipGetLastError();
 }
 
-hipError_t launch_side_thresholds(Context &c, uint64_t *l0MaxBits) {
-    hipError_t e = launch_fill_u64(c.stream, l0MaxBits, 0ull, 1);
-    int64_t nSides = 2 * c.L;
+hipError_t launch_side_thresholds(Context &c, uint64_t *maxBits, float *l0Max, float *scale) {
+    hipError_t e = launch_fill_u64(c.stream, maxBits, 0ull, 2);
+    *l0Max = 0.0f;
+    *scale = 0.0f;
+    const int64_t nSides = 2 * c.L;
     if (e != hipSuccess || nSides == 0) return e;
-    const int64_t nb = blocks_for(nSides);
     const int cap = c.numCU > 0 ? c.numCU * 8 : 2048;
-    hipLaunchKernelGGL(k_side_thresholds, dim3((unsigned)(nb < cap ? nb : cap)), dim3(TPB), 0, c.stream, c.sidePos.as<int32_t>(),
-                       nSides, c.initialLen.as<double>(), c.hasMaxLoad ? c.maxLoad.as<double>() : nullptr,
-                       c.slotThr.as<float>(), l0MaxBits);
+    const double *ml = c.hasMaxLoad ? c.maxLoad.as<double>() : nullptr;
+    int64_t nb = blocks_for(c.L);
+    hipLaunchKernelGGL(k_threshold_max, dim3((unsigned)(nb < cap ? nb : cap)), dim3(TPB), 0, c.stream, c.L,
+                       c.initialLen.as<double>(), ml, maxBits);
+    uint64_t w[2] = {0, 0};
+    e = hipMemcpyAsync(w, maxBits, sizeof(w), hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) return e;
+    const uint32_t b0 = (uint32_t)w[0], b1 = (uint32_t)w[1];
+    float tMax;
+    std::memcpy(l0Max, &b0, sizeof(float));
+    std::memcpy(&tMax, &b1, sizeof(float));
+    float s = tMax / 65535.0f;
+    if (!(s > 0.0f && s < 3e38f)) s = 0.0f;
+    *scale = s;
+    nb = blocks_for(nSides);
+    hipLaunchKernelGGL(k_side_codes, dim3((unsigned)(nb < cap ? nb : cap)), dim3(TPB), 0, c.stream,
+                       c.sidePos.as<int32_t>(), nSides, c.initialLen.as<double>(), ml, (double)s,
+                       c.slotThr.as<uint16_t>());
     return hipGetLastError();
 }
 

@@ -695,10 +695,11 @@ struct StepRecs {
     double *drecOut;
     MixedTail *tailOut;
     double tame;               // Diff bound behind the radius flag of rec
-    // Break prefilter inputs: per side a float threshold, L0 (1 + maxLoad)
-    // with per-link maxLoad (perLinkMaxLoad) else L0, NaN when L0 < 1e-30 or
-    // not finite; the largest L0 of the state (rounded up)
-    const float *slotThr;
+    // Break prefilter inputs: per side a 16-bit code, code * thrScale at most
+    // L0 (1 + maxLoad) with per-link maxLoad (perLinkMaxLoad) else L0, 0 when
+    // L0 < 1e-30 or not finite; the largest L0 of the state (rounded up)
+    const uint16_t *slotThr;
+    float thrScale;
     int32_t prefilter;         // 0: every intact side takes the exact break test
     int32_t perLinkMaxLoad;
     float scalarMaxLoad, bendBreak, l0Max;
@@ -789,12 +790,13 @@ __device__ __forceinline__ void partner_diff(const StepRecs<R> &r, int32_t q, co
 // (s - L0) / L0 + bendBreak * maxBend >= maxLoad, with L0 > 0, is
 // s + bendBreak * maxBend * L0 >= L0 (1 + maxLoad) =: T.  In float, with the
 // rotation bound (at least maxBend) and the largest L0 standing in for the
-// link's own, the left side can only grow; the link passes when that estimate
-// is below T by more than 1e-5 of the magnitudes.  Float rounding of the
-// inputs and operations is below 1e-6 of those magnitudes and bendBreak >= 0
-// (the host disables the prefilter otherwise), so a link the exact FP64 test
-// would break never passes.  NaN/inf inputs (a NaN threshold for L0 < 1e-30)
-// fail the comparison and go to the exact test.
+// link's own, the left side can only grow, and T comes from a 16-bit code
+// that rounds it down (k_side_codes); the link passes when that estimate is
+// below T by more than 1e-5 of the magnitudes.  Float rounding of the inputs
+// and operations is below 1e-6 of those magnitudes and bendBreak >= 0 (the
+// host disables the prefilter otherwise), so a link the exact FP64 test would
+// break never passes.  NaN/inf inputs and code 0 (L0 < 1e-30) fail the
+// comparison and go to the exact test.
 __device__ __forceinline__ bool certainly_intact(float T, float bendBreak, float l0Max, double dx, double dy,
                                                  double dz, float rotBound) {
     const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
@@ -944,7 +946,7 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                 if (ow < 0) {
                     mask = 0;  // broken link (state == 0)
                 } else if (f.breakCheck) {
-                    const float thr = ld_stream(r.slotThr + sidx);
+                    const float thr = (float)ld_stream(r.slotThr + sidx) * r.thrScale;
                     const float T = r.perLinkMaxLoad ? thr : thr * (1.0f + r.scalarMaxLoad);
                     const bool skip = r.prefilter &&
                                       certainly_intact(T, r.bendBreak, r.l0Max, dx, dy, dz, rotSelf + rotOther);
@@ -1306,7 +1308,8 @@ StepRecs<R> step_recs(Context &c, double tame, const hdm_break_params *brk) {
     r.drecOut = c.drecAlt.as<double>();
     r.tailOut = c.dtailAlt.as<MixedTail>();
     r.tame = tame;
-    r.slotThr = c.slotThr.as<float>();
+    r.slotThr = c.slotThr.as<uint16_t>();
+    r.thrScale = c.thrScale;
     r.l0Max = c.l0Max;
     r.perLinkMaxLoad = c.hasMaxLoad ? 1 : 0;
     r.scalarMaxLoad = brk ? (float)brk->maxLoad : 0.0f;
