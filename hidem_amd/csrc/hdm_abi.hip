@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <new>
 #include <string>
 #include <vector>
@@ -735,7 +736,7 @@ int hdm_set_maps(hdm_ctx *ctx, const hdm_map2d *const *maps) {
 int hdm_map2d_load(const char *path, int32_t fieldID, hdm_map2d *map) {
     if (!path || !map || fieldID < 0) return HDM_E_INVALID;
     *map = hdm_map2d{};
-    FILE *f = std::fopen(path, "r");
+    std::ifstream f(path);
     if (!f) return HDM_E_INVALID;
     // Map2dLoader::readInput (Map2dLoader.cpp:8-86): whitespace-separated
     // floats, every row with the first row's column count; the extents from
@@ -743,18 +744,22 @@ int hdm_map2d_load(const char *path, int32_t fieldID, hdm_map2d *map) {
     std::vector<std::vector<float>> cols;
     std::vector<float> row;
     std::vector<std::pair<float, int64_t>> rowCounts;  // distinct y values in first-seen order
-    char line[1 << 14];
+    std::string line;
     bool ok = true;
-    while (ok && std::fgets(line, sizeof(line), f)) {
+    while (ok && std::getline(f, line)) {
         row.clear();
-        for (char *t = std::strtok(line, " \t\r\n"); t; t = std::strtok(nullptr, " \t\r\n")) {
+        const char *q = line.c_str();
+        while (*q) {
+            while (*q == ' ' || *q == '\t' || *q == '\r') q++;
+            if (!*q) break;
             char *end = nullptr;
-            const float v = std::strtof(t, &end);
-            if (end == t) {
+            const float v = std::strtof(q, &end);
+            if (end == q || (*end && *end != ' ' && *end != '\t' && *end != '\r')) {
                 ok = false;
                 break;
             }
             row.push_back(v);
+            q = end;
         }
         if (!ok) break;
         if (row.empty()) continue;
@@ -772,7 +777,6 @@ int hdm_map2d_load(const char *path, int32_t fieldID, hdm_map2d *map) {
             }
         if (!seen) rowCounts.emplace_back(row[1], 1);
     }
-    std::fclose(f);
     if (!ok || cols.empty() || (size_t)fieldID >= cols.size()) return HDM_E_INVALID;
     const int64_t xPoints = rowCounts.front().second, yPoints = (int64_t)rowCounts.size();
     for (auto &rc : rowCounts)

@@ -122,6 +122,9 @@ def test_loader_rejects_malformed(tmp_path):
     f.write_text("0 0 1\n1 0 1\n0 1 1\n")      # rows of unequal length
     with pytest.raises(A.HdmError):
         A.load_map(str(f), 2)
+    f.write_text("0 0 1\n1 0 2x\n")            # not a number
+    with pytest.raises(A.HdmError):
+        A.load_map(str(f), 2)
     with pytest.raises(A.HdmError):
         A.load_map(str(tmp_path / "missing.csv"), 2)
 
