@@ -282,7 +282,7 @@ def declared_symbols(header: str = HEADER_PATH):
     """Function names declared in hdm_abi.h."""
     import re
     text = open(header).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void)\s+\**(hdm_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char)\s*\**\s*(hdm_\w+)\s*\(", text, re.M)))
 
 
 def _ptr(a, ct):

@@ -110,3 +110,15 @@ def test_timer_names_match_header_enum():
     names = ["HDM_T_FORCE"] + [s.strip() for s in body.split(",") if s.strip()]
     assert len(names) == len(A.TIMER_NAMES)
     assert [n[6:].lower() for n in names] == list(A.TIMER_NAMES)
+
+
+def test_io_library_exports_every_declared_symbol():
+    from hidem_amd import h5io
+    h5io.io_lib()
+    declared = A.declared_symbols(h5io.IO_HEADER_PATH)
+    assert len(declared) >= 6
+    out = subprocess.run(["nm", "-D", "--defined-only", h5io.IO_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [s for s in declared if s not in exported]
+    assert not missing, f"declared in hdm_io.h but not exported: {missing}"
