@@ -62,6 +62,12 @@ void set_map_inputs(hdm_ctx *ctx, const std::vector<MapFile> &files);
 struct SetupInfo {
     hdm_setup_info info;
     GeneralParams gp;
+    // [lattice] type = "File": particle input file and its dataset names
+    // (HiDEMGeometry.cpp:111-121); [restart] fromRestart: a checkpoint to
+    // start from.  Both read through libhidem_io.so.
+    std::string particleFile;
+    std::string fileNames[4];
+    std::string restartFile;
 };
 SetupInfo configure(ArgParser &parser);
 
@@ -75,6 +81,11 @@ public:
     // HiDEMLib::setup (HiDEMLib.h:707-822)
     void setup(ArgParser &parser, double scale);
     void setup(const hdm_setup_info &info);
+    // the generator-free part of setup: uploads a given state (host arrays
+    // are copied into the h_* members first) at the given step
+    void setupState(const hdm_setup_info &info, const hdm_particles_host &p, const hdm_links_host &l, int timestep);
+    // setup from a SetupInfo that may name a particle file or a checkpoint
+    void setup(const SetupInfo &s);
 
     bool run() const;                 // HiDEMLib.h:236-240
     void gridService();               // HiDEMLib.h:533-628 (rebuild part)

@@ -2,6 +2,8 @@
 // reference-shaped timestep loop).
 #include "hidemlib_hip.h"
 
+#include <dlfcn.h>
+
 #include <cmath>
 #include <cstdio>
 #include <limits>
@@ -187,15 +189,25 @@ SetupInfo configure(ArgParser &parser) {
     int regen = arg_or<int>(parser, "grid.gridRegenInterval", -1);
 
     std::string type = arg_or<std::string>(parser, "lattice.type", "");
-    if (type != "FCC") throw TomlError("lattice.type: only \"FCC\" is generated on this path");
+    if (type != "FCC" && type != "File")
+        throw TomlError("lattice.type: \"FCC\" (generated) or \"File\" (particle file) on this path");
     auto need = [&](const char *k) {
         auto v = parser.getArg<double>(k);
         if (!v) throw TomlError(std::string("missing ") + k);
         return *v;
     };
-    double spacing = need("lattice.fccSpacing") * scale;
+    // the largest radius sets the search distance (HiDEMGeometry.cpp:86-121)
     double radius = need("lattice.particleRadius") * scale;
     o.linkGenScale = need("lattice.linkGenScale");
+    o.fcc.radius = radius;
+    if (type == "File") {
+        s.particleFile = arg_or<std::string>(parser, "lattice.path", "");
+        if (s.particleFile.empty()) throw TomlError("lattice.path: the particle file of type \"File\"");
+        const char *keys[4] = {"lattice.xCords", "lattice.yCords", "lattice.zCords", "lattice.radius"};
+        const char *dflt[4] = {"xCord", "yCord", "zCord", "radius"};
+        for (int k = 0; k < 4; k++) s.fileNames[k] = arg_or<std::string>(parser, keys[k], dflt[k]);
+    } else {
+    double spacing = need("lattice.fccSpacing") * scale;
     auto cells = parser.getArg<std::vector<int>>("hdm.cells");
     if (!cells || cells->size() != 3) throw TomlError("hdm.cells needs 3 values");
     std::vector<double> origin = arg_or<std::vector<double>>(parser, "hdm.origin", {0.0, 0.0, 0.0});
@@ -205,7 +217,8 @@ SetupInfo configure(ArgParser &parser) {
     o.fcc.nz = (*cells)[2];
     o.fcc.a = spacing;
     for (int k = 0; k < 3; k++) o.fcc.origin[k] = origin[(size_t)k];
-    o.fcc.radius = radius;
+    }
+    s.restartFile = arg_or<std::string>(parser, "restart.fromRestart", "");
     o.fcc.jitter = arg_or<double>(parser, "hdm.jitter", 0.0);
     o.fcc.seed = (uint64_t)arg_or<int64_t>(parser, "lattice.seed", 1);
     o.fcc.clampXBelow = arg_or<double>(parser, "hdm.clampXBelow", -std::numeric_limits<double>::infinity());
@@ -247,7 +260,124 @@ void HiDEMLibHip::setup(ArgParser &parser, double scale) {
     if (scale != s.gp.scale) throw TomlError("setup: scale differs from general.scale");
     maxMinutes = arg_or<unsigned int>(parser, "iteration.maxMinutes", (unsigned int)-1);
     fusedSteps = arg_or<bool>(parser, "hdm.fusedSteps", true);
-    setup(s.info);
+    setup(s);
+}
+
+namespace {
+
+// libhidem_io.so, loaded on first use from the directory of this library
+// (the compute library itself does not link HDF5).
+struct IoLib {
+    void *h = nullptr;
+    const char *(*lastError)(void) = nullptr;
+    int (*ckptInfo)(const char *, int64_t *, int64_t *, int32_t *) = nullptr;
+    int (*ckptRead)(const char *, hdm_particles_host *, hdm_links_host *) = nullptr;
+    int (*fileCount)(const char *, const char *, int64_t *) = nullptr;
+    int (*fileRead)(const char *, const char *, const char *, const char *, const char *, hdm_particles_host *) = nullptr;
+
+    static IoLib &get() {
+        static IoLib io;
+        if (!io.h) {
+            Dl_info info{};
+            std::string dir = ".";
+            if (dladdr(reinterpret_cast<void *>(&hdm_sim_create), &info) && info.dli_fname) {
+                dir = info.dli_fname;
+                const size_t slash = dir.rfind('/');
+                dir = slash == std::string::npos ? "." : dir.substr(0, slash);
+            }
+            const std::string path = dir + "/libhidem_io.so";
+            io.h = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+            if (!io.h) throw TomlError("cannot load " + path + " (HDF5 IO): " + dlerror());
+            io.lastError = reinterpret_cast<const char *(*)(void)>(dlsym(io.h, "hdm_io_last_error"));
+            io.ckptInfo = reinterpret_cast<decltype(io.ckptInfo)>(dlsym(io.h, "hdm_checkpoint_info"));
+            io.ckptRead = reinterpret_cast<decltype(io.ckptRead)>(dlsym(io.h, "hdm_checkpoint_read"));
+            io.fileCount = reinterpret_cast<decltype(io.fileCount)>(dlsym(io.h, "hdm_particle_file_count"));
+            io.fileRead = reinterpret_cast<decltype(io.fileRead)>(dlsym(io.h, "hdm_particle_file_read"));
+            if (!io.lastError || !io.ckptInfo || !io.ckptRead || !io.fileCount || !io.fileRead)
+                throw TomlError(path + " lacks the HDF5 IO entry points");
+        }
+        return io;
+    }
+    void check(int rc, const std::string &what) {
+        if (rc != HDM_OK) throw TomlError(what + ": " + lastError());
+    }
+};
+
+// Host storage behind hdm_particles_host / hdm_links_host views.
+struct HostState {
+    std::vector<double> pf[19];
+    std::vector<uint8_t> type;
+    std::vector<int32_t> a, b;
+    std::vector<uint8_t> state;
+    std::vector<double> len, maxLoad, ct[6];
+    hdm_particles_host p{};
+    hdm_links_host l{};
+    void particles(int64_t n) {
+        for (auto &v : pf) v.assign((size_t)n, 0.0);
+        type.assign((size_t)n, 0);
+        double **f[19] = {&p.xCord, &p.yCord, &p.zCord, &p.xTheta, &p.yTheta, &p.zTheta, &p.xDiff,
+                          &p.yDiff, &p.zDiff, &p.uDiff, &p.vDiff, &p.wDiff, &p.radius, &p.xForce,
+                          &p.yForce, &p.zForce, &p.uForce, &p.vForce, &p.wForce};
+        for (int k = 0; k < 19; k++) *f[k] = pf[k].data();
+        p.type = type.data();
+        p.n = n;
+    }
+    void links(int64_t L, bool withCT) {
+        a.assign((size_t)L, 0);
+        b.assign((size_t)L, 0);
+        state.assign((size_t)L, 1);
+        len.assign((size_t)L, 0.0);
+        maxLoad.assign((size_t)L, 0.0);
+        for (auto &c : ct) c.assign(withCT ? (size_t)(2 * L) : 0, 0.0);
+        l.n = L;
+        l.localPartIndA = a.data();
+        l.localPartIndB = b.data();
+        l.state = state.data();
+        l.initialLen = len.data();
+        l.maxLoad = maxLoad.data();
+        for (int c = 0; c < 6; c++) l.CT[c] = withCT ? ct[c].data() : nullptr;
+    }
+};
+
+}  // namespace
+
+void HiDEMLibHip::setup(const SetupInfo &s) {
+    if (s.restartFile.empty() && s.particleFile.empty()) {
+        setup(s.info);
+        return;
+    }
+    IoLib &io = IoLib::get();
+    HostState st;
+    int timestep = 0;
+    if (!s.restartFile.empty()) {
+        // HiDEMLib::setup from a checkpoint (ParticleDataHandler::fromCheckpoint,
+        // LinkDataHandler::fromCheckpoint); per-link thresholds from the file
+        // when it has them, else the scalar maxLoad
+        int64_t n = 0, L = 0;
+        int32_t ts = 0;
+        io.check(io.ckptInfo(s.restartFile.c_str(), &n, &L, &ts), "hdm_checkpoint_info");
+        st.particles(n);
+        st.links(L, true);
+        if (io.ckptRead(s.restartFile.c_str(), &st.p, &st.l) != HDM_OK) {
+            st.l.maxLoad = nullptr;
+            io.check(io.ckptRead(s.restartFile.c_str(), &st.p, &st.l), "hdm_checkpoint_read");
+        }
+        timestep = ts;
+    } else {
+        int64_t n = 0;
+        io.check(io.fileCount(s.particleFile.c_str(), s.fileNames[0].c_str(), &n), "hdm_particle_file_count");
+        st.particles(n);
+        io.check(io.fileRead(s.particleFile.c_str(), s.fileNames[0].c_str(), s.fileNames[1].c_str(),
+                             s.fileNames[2].c_str(), s.fileNames[3].c_str(), &st.p),
+                 "hdm_particle_file_read");
+        int64_t L = 0;
+        hidem::hdm_check(nullptr, hdm_gen_links_count(&st.p, s.info.linkGenScale, &L), "hdm_gen_links_count");
+        st.links(L, false);
+        hidem::hdm_check(nullptr, hdm_gen_links_fill(&st.p, s.info.linkGenScale, s.info.threshold, s.info.spread,
+                                                     s.info.linkSeed, &st.l),
+                         "hdm_gen_links_fill");
+    }
+    setupState(s.info, st.p, st.l, timestep);
 }
 
 void HiDEMLibHip::setup(const hdm_setup_info &info) {
@@ -259,7 +389,6 @@ void HiDEMLibHip::setup(const hdm_setup_info &info) {
     policy = info.run.policy;
     gridRegenInterval = info.run.regenInterval;
     hdm_fcc_spec spec = info.fcc;
-    double radius = spec.radius;
 
     hidem::hdm_check(nullptr, hdm_gen_fcc_count(&spec, &nParticles), "hdm_gen_fcc_count");
     size_t n = (size_t)nParticles;
@@ -302,6 +431,34 @@ void HiDEMLibHip::setup(const hdm_setup_info &info) {
     l.maxLoad = h_maxLoad.data();
     hidem::hdm_check(nullptr, hdm_gen_links_fill(&p, info.linkGenScale, info.threshold, info.spread, info.linkSeed, &l),
                      "hdm_gen_links_fill");
+    setupState(info, p, l, 0);
+}
+
+void HiDEMLibHip::setupState(const hdm_setup_info &info, const hdm_particles_host &p, const hdm_links_host &l,
+                             int timestep) {
+    runStart = std::chrono::steady_clock::now();
+    endTimestep = (int)info.endTimestep;
+    searchDistance = info.run.searchDistance;
+    interactDist = info.run.interactDist;
+    attractDistance = info.run.force.attractDistance;
+    policy = info.run.policy;
+    gridRegenInterval = info.run.regenInterval;
+    const double radius = info.fcc.radius;
+    nParticles = p.n;
+    nLinks = l.n;
+    const size_t n = (size_t)p.n;
+    if (h_cords[0].data() != p.xCord) {
+        const double *src[3] = {p.xCord, p.yCord, p.zCord};
+        for (int k = 0; k < 3; k++) h_cords[k].assign(src[k], src[k] + n);
+        h_radius.assign(p.radius, p.radius + n);
+        h_type.assign(n, 0);
+        if (p.type) h_type.assign(p.type, p.type + n);
+        h_a.assign(l.localPartIndA, l.localPartIndA + l.n);
+        h_b.assign(l.localPartIndB, l.localPartIndB + l.n);
+        h_len.assign(l.initialLen, l.initialLen + l.n);
+        h_maxLoad.clear();
+        if (l.maxLoad) h_maxLoad.assign(l.maxLoad, l.maxLoad + l.n);
+    }
 
     accel.uploadParticles(p);
     accel.uploadLinks(l);
@@ -319,8 +476,9 @@ void HiDEMLibHip::setup(const hdm_setup_info &info) {
     d.lower = d.lower - radius;
     d.setHalo(searchDistance);
     world.domains = {d};
-    currentTimestep = startTimestep = 0;
+    currentTimestep = startTimestep = timestep;
     built = false;
+    if (timestep != 0) hidem::hdm_check(accel.ctx, hdm_set_timestep(accel.ctx, timestep), "hdm_set_timestep");
 }
 
 bool HiDEMLibHip::run() const {
@@ -422,7 +580,7 @@ int hdm_sim_create(int32_t precision, int32_t argc, const char *const *argv, hdm
         s->gp = si.gp;
         s->integ = si.info.run.integ;
         s->lib.maxMinutes = s->args.getArg<unsigned int>("iteration.maxMinutes").value_or((unsigned int)-1);
-        s->lib.setup(si.info);
+        s->lib.setup(si);
         // hdm.redistribute: the redistribution sort the reference's first
         // gridService does (HiDEMLib.h:569-587), once, before the first step;
         // off by default (the generator already emits a spatial order)

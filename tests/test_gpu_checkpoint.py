@@ -46,3 +46,83 @@ def test_restart_from_checkpoint(prec, tmp_path):
     for c in range(6):
         assert bits_equal(fl.ct[c], ml.ct[c]), c
     assert (fl.state == 0).sum() > 0 or (mem_l.state == 0).sum() == 0
+
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)), "golden")
+
+
+def test_driver_file_lattice(tmp_path):
+    """[lattice] type = "File" in the TOML driver: the reference's 160k
+    benchmark particle file read through libhidem_io.so, links generated,
+    20 steps -- bit-identical to the same inputs assembled from Python."""
+    import os
+    path = os.path.join(GOLDEN, "benchmark_160k_particles.h5")
+    cfg = tmp_path / "file_lattice.toml"
+    cfg.write_text(f"""
+[general]
+scale = 12.0
+E = 2e9
+dt = 0.002
+waterLevel = 300.0
+maxLoad = 5e-4
+bendBreak = 0.01
+damping = 10000.0
+dampingRotation = 10000.0
+breakTime = 0.0
+[iteration]
+timeSteps = 1000
+[grid]
+searchDistanceScale = 1.4
+[lattice]
+type = "File"
+path = "{path}"
+particleRadius = 0.5
+linkGenScale = 1.2
+[hdm]
+integrator = "iceblock"
+rebuildPolicy = "exact"
+maxLoadSpread = 0.1
+linkSeed = 7
+""")
+    print("driver file lattice", flush=True)
+    with A.Sim(str(cfg), [], precision=A.HDM_PREC_FP64) as sim:
+        rp = sim.run_params()
+        sim.steps(20)
+        q = sim.ctx.download_particles()
+        n, L = sim.n, sim.L
+    p = h5io.read_particle_file(path)
+    l = A.gen_links(p, 1.2, 5e-4, 0.1, 7)
+    assert (p.n, l.n) == (n, L) and l.n > 5 * p.n
+    with A.Context(A.HDM_PREC_FP64) as ctx:
+        ctx.upload(p, l)
+        ctx.links_build()
+        ctx.run(rp, 20)
+        g = ctx.download_particles()
+    for f in ("xCord", "yCord", "zCord", "xTheta", "zDiff"):
+        assert bits_equal(q[f], g[f]), f
+    assert not bits_equal(q["zCord"], p["zCord"])  # the block moved
+
+
+def test_driver_restart(tmp_path):
+    """[restart] fromRestart in the TOML driver continues a checkpoint at its
+    step, bit-identical to h5io.load + hdm_run."""
+    import os
+    s, p, l = params.build_inputs(params.load_config("iceblock_16k"))
+    path = str(tmp_path / "checkpoint_30.h5")
+    with A.Context(A.HDM_PREC_FP64) as ctx:
+        ctx.upload(p, l)
+        ctx.links_build()
+        ctx.run(s.run, 30)
+        h5io.save(ctx, path, 30)
+    cfg = os.path.join(params.CONFIG_DIR, "iceblock_16k.toml")
+    print("driver restart", flush=True)
+    with A.Sim(cfg, [f"--restart.fromRestart={path}"], precision=A.HDM_PREC_FP64) as sim:
+        st = sim.steps(40)
+        q = sim.ctx.download_particles()
+    assert st.timestep == 70
+    with A.Context(A.HDM_PREC_FP64) as ctx:
+        assert h5io.load(ctx, path) == 30
+        ctx.run(s.run, 40)
+        g = ctx.download_particles()
+    for f in ("xCord", "yCord", "zCord", "xTheta", "zDiff"):
+        assert bits_equal(q[f], g[f]), f
