@@ -247,6 +247,7 @@ def _declare(L):
         "hdm_sim_info": ([P, _i64p, _i64p, _i64p, _i64p], C.c_int),
         "hdm_sim_steps": ([P, C.c_int64, _u8p, C.POINTER(HdmRunStats)], C.c_int),
         "hdm_sim_config_dump": ([P], C.c_char_p),
+        "hdm_sim_checkpoint": ([P, C.c_char_p], C.c_int),
         "hdm_sched_configure": ([P, C.POINTER(HdmSchedParams)], C.c_int),
         "hdm_sched_set_model": ([P, C.POINTER(HdmMlpWeights)], C.c_int),
         "hdm_sched_features": ([P, C.POINTER(HdmRunParams), C.c_int64, C.c_int32, _i32p, _f32p, _f32p, _u8p],
@@ -715,6 +716,12 @@ class Sim:
         log = np.zeros(max(n, 1), np.uint8) if log_rebuilds else None
         check(lib().hdm_sim_steps(self._h, n, _ptr(log, C.c_uint8), C.byref(st)), what="hdm_sim_steps")
         return (st, log[:n]) if log_rebuilds else st
+
+    def checkpoint(self, path):
+        """hdm_sim_checkpoint: HDF5 checkpoint of the current state and step."""
+        rc = lib().hdm_sim_checkpoint(self._h, os.fsencode(path))
+        if rc != HDM_OK:
+            raise HdmError(f"hdm_sim_checkpoint failed ({rc}): {lib().hdm_sim_last_error().decode()}")
 
     def config_dump(self):
         return lib().hdm_sim_config_dump(self._h).decode()

@@ -86,6 +86,8 @@ public:
     void setupState(const hdm_setup_info &info, const hdm_particles_host &p, const hdm_links_host &l, int timestep);
     // setup from a SetupInfo that may name a particle file or a checkpoint
     void setup(const SetupInfo &s);
+    // checkpoint of the current state and step (hdm_sim_checkpoint)
+    void writeCheckpoint(const std::string &path);
 
     bool run() const;                 // HiDEMLib.h:236-240
     void gridService();               // HiDEMLib.h:533-628 (rebuild part)

@@ -271,6 +271,7 @@ struct IoLib {
     void *h = nullptr;
     const char *(*lastError)(void) = nullptr;
     int (*ckptInfo)(const char *, int64_t *, int64_t *, int32_t *) = nullptr;
+    int (*ckptWrite)(const char *, int32_t, const hdm_particles_host *, const hdm_links_host *) = nullptr;
     int (*ckptRead)(const char *, hdm_particles_host *, hdm_links_host *) = nullptr;
     int (*fileCount)(const char *, const char *, int64_t *) = nullptr;
     int (*fileRead)(const char *, const char *, const char *, const char *, const char *, hdm_particles_host *) = nullptr;
@@ -291,9 +292,10 @@ struct IoLib {
             io.lastError = reinterpret_cast<const char *(*)(void)>(dlsym(io.h, "hdm_io_last_error"));
             io.ckptInfo = reinterpret_cast<decltype(io.ckptInfo)>(dlsym(io.h, "hdm_checkpoint_info"));
             io.ckptRead = reinterpret_cast<decltype(io.ckptRead)>(dlsym(io.h, "hdm_checkpoint_read"));
+            io.ckptWrite = reinterpret_cast<decltype(io.ckptWrite)>(dlsym(io.h, "hdm_checkpoint_write"));
             io.fileCount = reinterpret_cast<decltype(io.fileCount)>(dlsym(io.h, "hdm_particle_file_count"));
             io.fileRead = reinterpret_cast<decltype(io.fileRead)>(dlsym(io.h, "hdm_particle_file_read"));
-            if (!io.lastError || !io.ckptInfo || !io.ckptRead || !io.fileCount || !io.fileRead)
+            if (!io.lastError || !io.ckptInfo || !io.ckptRead || !io.ckptWrite || !io.fileCount || !io.fileRead)
                 throw TomlError(path + " lacks the HDF5 IO entry points");
         }
         return io;
@@ -340,6 +342,18 @@ struct HostState {
 };
 
 }  // namespace
+
+void HiDEMLibHip::writeCheckpoint(const std::string &path) {
+    IoLib &io = IoLib::get();
+    HostState st;
+    st.particles(nParticles);
+    st.links(nLinks, true);
+    accel.downloadParticles(st.p);
+    hdm_links_host l = st.l;
+    if (h_maxLoad.empty()) l.maxLoad = nullptr;  // no per-link thresholds bound
+    accel.downloadLinks(l);
+    io.check(io.ckptWrite(path.c_str(), (int32_t)currentTimestep, &st.p, &l), "hdm_checkpoint_write");
+}
 
 void HiDEMLibHip::setup(const SetupInfo &s) {
     if (s.restartFile.empty() && s.particleFile.empty()) {
@@ -634,6 +648,17 @@ int hdm_sim_run_params(hdm_sim *s, hdm_run_params *rp) {
     rp->policy = s->lib.policy;
     rp->regenInterval = s->lib.gridRegenInterval;
     return HDM_OK;
+}
+
+int hdm_sim_checkpoint(hdm_sim *s, const char *path) {
+    if (!s || !path) return HDM_E_INVALID;
+    try {
+        s->lib.writeCheckpoint(path);
+        return HDM_OK;
+    } catch (const std::exception &e) {
+        g_sim_error = e.what();
+        return HDM_E_INVALID;
+    }
 }
 
 int hdm_sim_info(hdm_sim *s, int64_t *nParticles, int64_t *nLinks, int64_t *timestep, int64_t *endTimestep) {

@@ -471,6 +471,10 @@ int  hdm_sim_run_params(hdm_sim *sim, hdm_run_params *rp);
 int  hdm_sim_info(hdm_sim *sim, int64_t *nParticles, int64_t *nLinks, int64_t *timestep, int64_t *endTimestep);
 int  hdm_sim_steps(hdm_sim *sim, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats *stats);
 const char *hdm_sim_config_dump(hdm_sim *sim);
+/* HiDEMH5Checkpoint::writeCheckpoint (HiDEMH5Checkpoint.h:23-49) of the
+ * sim's current state and step, in the reference's layout (hdm_io.h);
+ * loads libhidem_io.so from this library's directory. */
+int  hdm_sim_checkpoint(hdm_sim *sim, const char *path);
 
 #ifdef __cplusplus
 }

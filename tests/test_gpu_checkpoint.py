@@ -126,3 +126,30 @@ def test_driver_restart(tmp_path):
         g = ctx.download_particles()
     for f in ("xCord", "yCord", "zCord", "xTheta", "zDiff"):
         assert bits_equal(q[f], g[f]), f
+
+
+def test_driver_writes_checkpoint(tmp_path):
+    """hdm_sim_checkpoint after 25 driver steps holds the same state as a
+    checkpoint written from Python after the same 25 steps."""
+    import os
+    cfg = os.path.join(params.CONFIG_DIR, "iceblock_16k.toml")
+    a, b = str(tmp_path / "driver.h5"), str(tmp_path / "python.h5")
+    print("driver checkpoint", flush=True)
+    with A.Sim(cfg, [], precision=A.HDM_PREC_FP64) as sim:
+        sim.steps(25)
+        sim.checkpoint(a)
+    s, p, l = params.build_inputs(params.load_config("iceblock_16k"))
+    with A.Context(A.HDM_PREC_FP64) as ctx:
+        ctx.upload(p, l)
+        ctx.links_build()
+        ctx.run(s.run, 25)
+        h5io.save(ctx, b, 25)
+    ta, pa, la = h5io.read_checkpoint(a, with_max_load=True)
+    tb, pb, lb = h5io.read_checkpoint(b, with_max_load=True)
+    assert ta == tb == 25
+    for f in ("xCord", "yCord", "zCord", "xTheta", "zDiff", "radius"):
+        assert bits_equal(pa[f], pb[f]), f
+    assert np.array_equal(la.a, lb.a) and np.array_equal(la.state, lb.state)
+    assert bits_equal(la.maxLoad, lb.maxLoad)
+    for c in range(6):
+        assert bits_equal(la.ct[c], lb.ct[c]), c
