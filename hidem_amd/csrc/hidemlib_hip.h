@@ -88,6 +88,16 @@ public:
     void setup(const SetupInfo &s);
     // checkpoint of the current state and step (hdm_sim_checkpoint)
     void writeCheckpoint(const std::string &path);
+    // [restart] keys (HiDEMLib.h:729-733) and the writes they ask for:
+    // periodic from finalizeTs (HiDEMLib.h:536-541), at the end from
+    // finalize (:476-484), to outWorkingPath/checkpointDirectory/
+    // checkpoint[_<step>].h5 (HiDEMH5Checkpoint.h:23-49)
+    void readCheckpointConfig(ArgParser &parser);
+    void periodicCheckpoint();
+    void finalize();
+    bool writeEndCheckpoint = false, timestepCheckpoint = false, keepOldCheckpoint = false, finalized = false;
+    int checkpointInterval = 10000;
+    std::string checkpointDirectory = "checkpoints", outWorkingPath;
 
     bool run() const;                 // HiDEMLib.h:236-240
     void gridService();               // HiDEMLib.h:533-628 (rebuild part)

@@ -5,6 +5,7 @@
 #include <dlfcn.h>
 
 #include <cmath>
+#include <filesystem>
 #include <cstdio>
 #include <limits>
 #include <new>
@@ -259,6 +260,7 @@ void HiDEMLibHip::setup(ArgParser &parser, double scale) {
     SetupInfo s = configure(parser);
     if (scale != s.gp.scale) throw TomlError("setup: scale differs from general.scale");
     maxMinutes = arg_or<unsigned int>(parser, "iteration.maxMinutes", (unsigned int)-1);
+    readCheckpointConfig(parser);
     fusedSteps = arg_or<bool>(parser, "hdm.fusedSteps", true);
     setup(s);
 }
@@ -539,7 +541,44 @@ void HiDEMLibHip::applyForce(const hdm_integ_params &ip) { accel.applyForce(ip);
 
 void HiDEMLibHip::linkUpdate(const hidem::HiDEMDefLinkBreakFunc &b) { accel.iterateLinks(b); }
 
-void HiDEMLibHip::finalizeTs() { currentTimestep++; }
+void HiDEMLibHip::finalizeTs() {
+    currentTimestep++;
+    periodicCheckpoint();
+}
+
+void HiDEMLibHip::readCheckpointConfig(ArgParser &parser) {
+    writeEndCheckpoint = arg_or<bool>(parser, "restart.writeEndCheckpoint", false);
+    timestepCheckpoint = arg_or<bool>(parser, "restart.timestepCheckpoint", false);
+    keepOldCheckpoint = arg_or<bool>(parser, "restart.keepOld", false);
+    checkpointInterval = arg_or<int>(parser, "restart.checkpointInterval", 10000);
+    checkpointDirectory = arg_or<std::string>(parser, "restart.checkpointDirectory", "checkpoints");
+    outWorkingPath = arg_or<std::string>(parser, "output.outWorkingPath", "");
+}
+
+static void write_named_checkpoint(HiDEMLibHip &h) {
+    // an empty working path means the current directory (the reference
+    // would write under the filesystem root)
+    const std::string dir = (h.outWorkingPath.empty() ? std::string(".") : h.outWorkingPath) + "/" +
+                            h.checkpointDirectory + "/";
+    const std::string suffix = h.timestepCheckpoint ? "_" + std::to_string(h.getCurrentTimestep()) : "";
+    const std::string target = dir + "checkpoint" + suffix + ".h5";
+    std::error_code ec;
+    std::filesystem::create_directories(dir, ec);
+    if (h.keepOldCheckpoint && std::filesystem::exists(target))
+        std::filesystem::rename(target, dir + "checkpoint" + suffix + "_old.h5", ec);
+    h.writeCheckpoint(target);
+}
+
+void HiDEMLibHip::periodicCheckpoint() {
+    if (checkpointInterval != -1 && checkpointInterval > 0 && currentTimestep != 0 &&
+        currentTimestep % checkpointInterval == 0)
+        write_named_checkpoint(*this);
+}
+
+void HiDEMLibHip::finalize() {
+    if (writeEndCheckpoint && !finalized) write_named_checkpoint(*this);
+    finalized = true;
+}
 
 }  // namespace hdm
 
@@ -595,6 +634,7 @@ int hdm_sim_create(int32_t precision, int32_t argc, const char *const *argv, hdm
         s->integ = si.info.run.integ;
         s->lib.maxMinutes = s->args.getArg<unsigned int>("iteration.maxMinutes").value_or((unsigned int)-1);
         s->lib.setup(si);
+        s->lib.readCheckpointConfig(s->args);
         // hdm.redistribute: the redistribution sort the reference's first
         // gridService does (HiDEMLib.h:569-587), once, before the first step;
         // off by default (the generator already emits a spatial order)
@@ -688,25 +728,56 @@ int hdm_sim_steps(hdm_sim *s, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats
         hdm_run_params rp;
         int rc = hdm_sim_run_params(s, &rp);
         if (rc != HDM_OK) return rc;
-        hdm_run_stats st{};
-        rc = hdm_run(h.accel.ctx, &rp, n, rebuildLog, &st);
-        if (rc != HDM_OK) {
-            g_sim_error = hdm_last_error(h.accel.ctx);
-            return rc;
+        hdm_run_stats total{};
+        total.timestep = h.getCurrentTimestep();
+        try {
+            // runs end at the periodic checkpoint steps, where the state is written
+            while (done < n) {
+                int64_t k = n - done;
+                const int64_t iv = h.checkpointInterval;
+                if (iv > 0) {
+                    const int64_t next = ((int64_t)h.getCurrentTimestep() / iv + 1) * iv;
+                    k = std::min<int64_t>(k, next - h.getCurrentTimestep());
+                }
+                hdm_run_stats st{};
+                rc = hdm_run(h.accel.ctx, &rp, k, rebuildLog ? rebuildLog + done : nullptr, &st);
+                if (rc != HDM_OK) {
+                    g_sim_error = hdm_last_error(h.accel.ctx);
+                    return rc;
+                }
+                h.syncFromRun(st);
+                h.periodicCheckpoint();
+                done += st.steps;
+                total.steps += st.steps;
+                total.rebuilds += st.rebuilds;
+                total.seconds += st.seconds;
+                total.timestep = st.timestep;
+                total.brokenTotal = st.brokenTotal;
+                if (st.steps < k) break;
+            }
+            if (!h.run()) h.finalize();
+        } catch (const std::exception &e) {
+            g_sim_error = e.what();
+            return HDM_E_INVALID;
         }
-        h.syncFromRun(st);
-        if (stats) *stats = st;
+        if (stats) *stats = total;
         return HDM_OK;
     }
-    for (; done < nSteps && h.run(); done++) {
-        h.gridService();
-        if (rebuildLog) rebuildLog[done] = h.lastStepRebuilt() ? 1 : 0;
-        h.computePartForce(nF, lF);
-        h.applyForce(s->integ);
-        if (h.getCurrentTimestep() * s->gp.dt > s->gp.breakTime) h.linkUpdate(bF);
-        h.finalizeTs();
+    try {
+        for (; done < nSteps && h.run(); done++) {
+            h.gridService();
+            if (rebuildLog) rebuildLog[done] = h.lastStepRebuilt() ? 1 : 0;
+            h.computePartForce(nF, lF);
+            h.applyForce(s->integ);
+            if (h.getCurrentTimestep() * s->gp.dt > s->gp.breakTime) h.linkUpdate(bF);
+            h.finalizeTs();
+        }
+        h.accel.sync();
+        if (!h.run()) h.finalize();
+    } catch (const std::exception &e) {
+        g_sim_error = e.what();
+        return HDM_E_INVALID;
     }
-    h.accel.sync();
     auto t1 = std::chrono::steady_clock::now();
     if (stats) {
         stats->timestep = h.getCurrentTimestep();

@@ -42,6 +42,7 @@ int main(int argc, char *argv[]) {
             hiDem.finalizeTs();
         }
         hiDem.accel.sync();
+        hiDem.finalize();
         double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - iterStart).count();
         int64_t broken = 0;
         hidem::hdm_check(hiDem.accel.ctx, hdm_count_broken(hiDem.accel.ctx, &broken), "hdm_count_broken");

@@ -153,3 +153,31 @@ def test_driver_writes_checkpoint(tmp_path):
     assert bits_equal(la.maxLoad, lb.maxLoad)
     for c in range(6):
         assert bits_equal(la.ct[c], lb.ct[c]), c
+
+
+def test_driver_periodic_and_end_checkpoints(tmp_path):
+    """[restart] checkpointInterval / timestepCheckpoint / writeEndCheckpoint
+    in the driver (HiDEMLib.h:476-484, 536-541): files at steps 20, 40 and the
+    end (50) under outWorkingPath/checkpoints, each holding the state of that
+    step."""
+    import os
+    cfg = os.path.join(params.CONFIG_DIR, "iceblock_16k.toml")
+    over = ["--restart.checkpointInterval=20", "--restart.timestepCheckpoint=true",
+            "--restart.writeEndCheckpoint=true", f"--output.outWorkingPath={tmp_path}", "--iteration.timeSteps=50"]
+    print("driver periodic checkpoints", flush=True)
+    with A.Sim(cfg, over, precision=A.HDM_PREC_FP64) as sim:
+        st = sim.steps(80)
+    assert st.timestep == 50 and st.steps == 50
+    files = sorted(os.listdir(tmp_path / "checkpoints"))
+    assert files == ["checkpoint_20.h5", "checkpoint_40.h5", "checkpoint_50.h5"], files
+    s, p, l = params.build_inputs(params.load_config("iceblock_16k"))
+    with A.Context(A.HDM_PREC_FP64) as ctx:
+        ctx.upload(p, l)
+        ctx.links_build()
+        ctx.run(s.run, 20)
+        ctx.run(s.run, 20)
+        g = ctx.download_particles()
+    ts, q, _ = h5io.read_checkpoint(str(tmp_path / "checkpoints" / "checkpoint_40.h5"))
+    assert ts == 40
+    for f in ("xCord", "yCord", "zCord", "xTheta", "zDiff"):
+        assert bits_equal(q[f], g[f]), f
