@@ -174,7 +174,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--config", default="glacier_8m")
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
-    ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps in the CPU baseline sample (0: skip)")
     ap.add_argument("--cells", type=int, nargs=3, default=None, help="override unit cells (smaller runs)")
     ap.add_argument("--probe-traffic", action="store_true",
                     help="for rocprofv3 --pmc runs: a 512 MiB copy kernel of known traffic, then 3 force launches "
