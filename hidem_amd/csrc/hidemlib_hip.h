@@ -96,6 +96,17 @@ public:
     void periodicCheckpoint();
     void finalize();
     bool writeEndCheckpoint = false, timestepCheckpoint = false, keepOldCheckpoint = false, finalized = false;
+    // hdm.redistribute with the [distribution] keys (HiDEMLib.h:560-587,
+    // 725-727) on one rank: the redistribution order (hdm_reorder) on the
+    // first step, on firstBalanceTimestep and, unless dynamicRedistribution,
+    // every redistributionInterval steps (the dynamic bounds check concerns
+    // domain boundaries, which one rank does not have)
+    void readDistributionConfig(ArgParser &parser);
+    bool redistributionDue() const;
+    int nextRedistribution() const;   // first due step after the current one (INT_MAX: none)
+    void redistributeNow();
+    bool redistribute = false, dynamicRedistribution = true;
+    int redistributionInterval = 100, firstBalanceTimestep = 2000;
     int checkpointInterval = 10000;
     std::string checkpointDirectory = "checkpoints", outWorkingPath;
 

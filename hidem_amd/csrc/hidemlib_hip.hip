@@ -261,6 +261,7 @@ void HiDEMLibHip::setup(ArgParser &parser, double scale) {
     if (scale != s.gp.scale) throw TomlError("setup: scale differs from general.scale");
     maxMinutes = arg_or<unsigned int>(parser, "iteration.maxMinutes", (unsigned int)-1);
     readCheckpointConfig(parser);
+    readDistributionConfig(parser);
     fusedSteps = arg_or<bool>(parser, "hdm.fusedSteps", true);
     setup(s);
 }
@@ -513,6 +514,7 @@ void HiDEMLibHip::rebuild() {
 // HiDEMLib::gridService (HiDEMLib.h:544-559, 589-627) with the build's exact
 // displacement test as an alternative policy.
 void HiDEMLibHip::gridService() {
+    if (redistributionDue()) redistributeNow();
     bool needsGridRegen = currentTimestep == startTimestep || !built;
     if (policy == HDM_REBUILD_INTERVAL) {
         if (gridRegenInterval > 0 && (currentTimestep % gridRegenInterval) == 0) needsGridRegen = true;
@@ -567,6 +569,33 @@ static void write_named_checkpoint(HiDEMLibHip &h) {
     if (h.keepOldCheckpoint && std::filesystem::exists(target))
         std::filesystem::rename(target, dir + "checkpoint" + suffix + "_old.h5", ec);
     h.writeCheckpoint(target);
+}
+
+void HiDEMLibHip::readDistributionConfig(ArgParser &parser) {
+    redistribute = arg_or<bool>(parser, "hdm.redistribute", false);
+    redistributionInterval = arg_or<int>(parser, "distribution.redistributionInterval", 100);
+    firstBalanceTimestep = arg_or<int>(parser, "distribution.firstBalanceTimestep", 2000);
+    dynamicRedistribution = arg_or<bool>(parser, "distribution.dynamicRedistribution", true);
+}
+
+bool HiDEMLibHip::redistributionDue() const {
+    if (!redistribute) return false;
+    if (currentTimestep == startTimestep || currentTimestep == firstBalanceTimestep) return true;
+    return !dynamicRedistribution && redistributionInterval > 0 && currentTimestep % redistributionInterval == 0;
+}
+
+int HiDEMLibHip::nextRedistribution() const {
+    int next = std::numeric_limits<int>::max();
+    if (!redistribute) return next;
+    if (firstBalanceTimestep > currentTimestep) next = firstBalanceTimestep;
+    if (!dynamicRedistribution && redistributionInterval > 0)
+        next = std::min(next, (currentTimestep / redistributionInterval + 1) * redistributionInterval);
+    return next;
+}
+
+void HiDEMLibHip::redistributeNow() {
+    hidem::hdm_check(accel.ctx, hdm_reorder(accel.ctx, searchDistance, nullptr, nullptr), "hdm_reorder");
+    built = false;
 }
 
 void HiDEMLibHip::periodicCheckpoint() {
@@ -635,17 +664,10 @@ int hdm_sim_create(int32_t precision, int32_t argc, const char *const *argv, hdm
         s->lib.maxMinutes = s->args.getArg<unsigned int>("iteration.maxMinutes").value_or((unsigned int)-1);
         s->lib.setup(si);
         s->lib.readCheckpointConfig(s->args);
-        // hdm.redistribute: the redistribution sort the reference's first
-        // gridService does (HiDEMLib.h:569-587), once, before the first step;
-        // off by default (the generator already emits a spatial order)
-        if (s->args.getArg<bool>("hdm.redistribute").value_or(false)) {
-            int rc = hdm_reorder(s->lib.accel.ctx, s->lib.searchDistance, nullptr, nullptr);
-            if (rc != HDM_OK) {
-                std::string msg = std::string("hdm_reorder: ") + hdm_last_error(s->lib.accel.ctx);
-                delete s;
-                throw hdm::TomlError(msg);
-            }
-        }
+        // hdm.redistribute (off by default: the generator already emits a
+        // spatial order): the redistribution sort of the reference's
+        // gridService (HiDEMLib.h:560-587), from the first step on
+        s->lib.readDistributionConfig(s->args);
         if (s->integ.variant == HDM_INTEG_GLACIER) {
             std::vector<hdm::MapFile> files = hdm::map_inputs(s->args);
             if (!files.empty()) {
@@ -739,6 +761,9 @@ int hdm_sim_steps(hdm_sim *s, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats
                     const int64_t next = ((int64_t)h.getCurrentTimestep() / iv + 1) * iv;
                     k = std::min<int64_t>(k, next - h.getCurrentTimestep());
                 }
+                // runs also end where a redistribution is due
+                if (h.redistributionDue()) h.redistributeNow();
+                k = std::min<int64_t>(k, (int64_t)h.nextRedistribution() - h.getCurrentTimestep());
                 hdm_run_stats st{};
                 rc = hdm_run(h.accel.ctx, &rp, k, rebuildLog ? rebuildLog + done : nullptr, &st);
                 if (rc != HDM_OK) {

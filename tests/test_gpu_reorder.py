@@ -146,3 +146,31 @@ def test_reorder_batched_blocks(orc):
     sim.run(80)
     for f in ("xCord", "yCord", "zCord", "xTheta", "zDiff"):
         assert bits_equal(gp[f], sim.p[f]), f
+
+
+def test_driver_interval_redistribution():
+    """hdm.redistribute with dynamicRedistribution = false: the driver reorders
+    on the first step and every redistributionInterval steps (HiDEMLib.h
+    :560-587); the per-call loop and the fused runs (which end at those
+    steps) agree bit for bit, and match hdm_reorder + hdm_run from Python."""
+    import os
+    cfg = os.path.join(params.CONFIG_DIR, "iceblock_16k.toml")
+    over = ["--hdm.redistribute=true", "--distribution.dynamicRedistribution=false",
+            "--distribution.redistributionInterval=25"]
+    say("driver interval redistribution")
+    out = {}
+    for name, extra in (("per_call", ["--hdm.fusedSteps=false"]), ("fused", [])):
+        with A.Sim(cfg, over + extra, precision=A.HDM_PREC_FP64) as sim:
+            sim.steps(60)
+            out[name] = sim.ctx.download_particles()
+    s, p, l = params.build_inputs(params.load_config("iceblock_16k"))
+    with A.Context(A.HDM_PREC_FP64) as ctx:
+        ctx.upload(p, l)
+        ctx.links_build()
+        for k in (25, 25, 10):
+            ctx.reorder(s.search_distance)
+            ctx.run(s.run, k)
+        g = ctx.download_particles()
+    for f in ("xCord", "yCord", "zCord", "xTheta", "zDiff"):
+        assert bits_equal(out["per_call"][f], out["fused"][f]), f
+        assert bits_equal(out["fused"][f], g[f]), f
