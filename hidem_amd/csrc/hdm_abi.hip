@@ -85,12 +85,41 @@ int collect_timing(hdm_ctx *ctx) {
     return HDM_OK;
 }
 
+// The caller's arrays may live in host memory or in device (or managed)
+// memory of this GPU, as HiDEMLib's ManVec buffers do in its GPU builds:
+// plain copies let the runtime see which (hipMemcpyDefault), conversions
+// between FP64 and the FP32 form's fields run on the side that holds the
+// array.
+bool is_device_ptr(const void *p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable host memory is not known to the runtime
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.isManaged;
+}
+
+template <class S, class D>
+__global__ void __launch_bounds__(256) k_convert(const S *__restrict__ src, D *__restrict__ dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = (D)src[i];
+}
+
+hipError_t copy_in(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s);
+}
+
 template <class T>
 hipError_t upload_as(hdm::DevBuf &dst, const double *src, int64_t n, hipStream_t s) {
     hipError_t e = dst.ensure(sizeof(T) * (size_t)(n > 0 ? n : 1));
     if (e != hipSuccess || n == 0) return e;
     if (src == nullptr) return hipMemsetAsync(dst.ptr, 0, sizeof(T) * (size_t)n, s);
-    if (sizeof(T) == sizeof(double)) return hipMemcpyAsync(dst.ptr, src, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s);
+    if (sizeof(T) == sizeof(double)) return copy_in(dst.ptr, src, sizeof(double) * (size_t)n, s);
+    if (is_device_ptr(src)) {
+        hipLaunchKernelGGL((k_convert<double, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src,
+                           dst.as<T>(), n);
+        return hipGetLastError();
+    }
     std::vector<T> tmp((size_t)n);
     for (int64_t i = 0; i < n; i++) tmp[(size_t)i] = (T)src[i];
     e = hipMemcpyAsync(dst.ptr, tmp.data(), sizeof(T) * (size_t)n, hipMemcpyHostToDevice, s);
@@ -102,7 +131,13 @@ template <class T>
 hipError_t download_as(double *dst, const hdm::DevBuf &src, int64_t n, hipStream_t s) {
     if (dst == nullptr || n == 0) return hipSuccess;
     if (sizeof(T) == sizeof(double)) {
-        hipError_t e = hipMemcpyAsync(dst, src.ptr, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s);
+        hipError_t e = hipMemcpyAsync(dst, src.ptr, sizeof(double) * (size_t)n, hipMemcpyDefault, s);
+        return e == hipSuccess ? hipStreamSynchronize(s) : e;
+    }
+    if (is_device_ptr(dst)) {
+        hipLaunchKernelGGL((k_convert<T, double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           src.as<T>(), dst, n);
+        hipError_t e = hipGetLastError();
         return e == hipSuccess ? hipStreamSynchronize(s) : e;
     }
     std::vector<T> tmp((size_t)n);
@@ -113,8 +148,9 @@ hipError_t download_as(double *dst, const hdm::DevBuf &src, int64_t n, hipStream
     return hipSuccess;
 }
 
+// Device buffer -> the caller's array (host or device), waited for.
 hipError_t d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
-    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s);
     return e == hipSuccess ? hipStreamSynchronize(s) : e;
 }
 
@@ -265,7 +301,7 @@ int hdm_particles_upload(hdm_ctx *ctx, const hdm_particles_host *p) {
     }
     HIP_CHECK(ctx, c.type.ensure((size_t)(c.n > 0 ? c.n : 1)));
     if (c.n > 0) {
-        if (p->type) HIP_CHECK(ctx, hipMemcpyAsync(c.type.ptr, p->type, (size_t)c.n, hipMemcpyHostToDevice, s));
+        if (p->type) HIP_CHECK(ctx, copy_in(c.type.ptr, p->type, (size_t)c.n, s));
         else HIP_CHECK(ctx, hipMemsetAsync(c.type.ptr, 0, (size_t)c.n, s));
     }
     for (hdm::DevBuf *b : {&c.x0, &c.y0, &c.z0}) HIP_CHECK(ctx, b->ensure(sizeof(double) * (size_t)(c.n > 0 ? c.n : 1)));
@@ -317,9 +353,9 @@ int hdm_links_upload(hdm_ctx *ctx, const hdm_links_host *l) {
     HIP_CHECK(ctx, c.lb.ensure(sizeof(int32_t) * L));
     HIP_CHECK(ctx, c.state.ensure(L));
     if (c.L > 0) {
-        HIP_CHECK(ctx, hipMemcpyAsync(c.la.ptr, l->localPartIndA, sizeof(int32_t) * c.L, hipMemcpyHostToDevice, s));
-        HIP_CHECK(ctx, hipMemcpyAsync(c.lb.ptr, l->localPartIndB, sizeof(int32_t) * c.L, hipMemcpyHostToDevice, s));
-        HIP_CHECK(ctx, hipMemcpyAsync(c.state.ptr, l->state, (size_t)c.L, hipMemcpyHostToDevice, s));
+        HIP_CHECK(ctx, copy_in(c.la.ptr, l->localPartIndA, sizeof(int32_t) * c.L, s));
+        HIP_CHECK(ctx, copy_in(c.lb.ptr, l->localPartIndB, sizeof(int32_t) * c.L, s));
+        HIP_CHECK(ctx, copy_in(c.state.ptr, l->state, (size_t)c.L, s));
     }
     HIP_CHECK(ctx, upload_as<double>(c.initialLen, l->initialLen, c.L, s));
     c.hasMaxLoad = l->maxLoad != nullptr;

@@ -55,6 +55,9 @@ typedef struct hdm_ctx hdm_ctx;
  * (HiDEMParticleData.h:20-39, ParticleData.h:29-37).  Any pointer may be NULL
  * on download (field skipped); on upload only the force fields may be NULL
  * (zero-filled).  Arrays hold n entries. */
+/* The arrays of hdm_particles_host / hdm_links_host may be host memory or
+ * device (or managed) memory of the context's GPU; uploads and downloads copy
+ * on the device for the latter. */
 typedef struct hdm_particles_host {
     int64_t n;
     double *xCord, *yCord, *zCord;
