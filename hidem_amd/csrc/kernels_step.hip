@@ -932,38 +932,51 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
             const int32_t cnt = (int32_t)ld_stream(f.linkCount + p);
             const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + lane;
             uint64_t activeSides = 0;  // linked pairs in contact (FP32, merged list; cnt <= 64)
+            // Software-pipelined: a side's partner word is loaded one
+            // iteration ahead, and its record, CT and threshold code are
+            // issued together, so a side waits for one memory latency (the
+            // slowest of them) instead of a chain of dependent ones.
+            int32_t owNext = cnt > 0 ? ld_stream(f.slotOther + base) : 0;  // partner; sign bit: broken
+            // vmcnt(0) here: nothing loaded before the loop is still pending
+            // at its header, so the header needs no wait (which, with the
+            // previous side's CT stores outstanding, would drain them too)
+            __builtin_amdgcn_s_waitcnt(0x0F70);
             for (int32_t k = 0; k < cnt; k++) {
                 const int64_t sidx = base + (int64_t)k * 64;
-                const int32_t ow = ld_stream(f.slotOther + sidx);  // partner; sign bit: broken
+                const int32_t ow = owNext;
                 const int32_t q = ow & SLOT_LINK_MASK;
                 const auto Q = r.rec[q];
+                CT cs[6];
+                ct_load(f.ct, sidx, cs);
+                // both loads unconditional (slotThr has the slot arrays'
+                // extent; the last side re-reads its own word), so no branch
+                // splits the issue from the one wait
+                const uint16_t code = ld_stream(r.slotThr + sidx);
+                owNext = ld_stream(f.slotOther + (k + 1 < cnt ? sidx + 64 : sidx));
                 R other[7], d[6];
                 float rotOther;
                 partner_diff<R>(r, q, Q, diff, other, d, rotOther);
                 other[6] = R(0);  // no radius needed
                 const double dx = xCord - Q.x, dy = yCord - Q.y, dz = zCord - Q.z;
-                R mask = 1;
-                if (ow < 0) {
-                    mask = 0;  // broken link (state == 0)
-                } else if (f.breakCheck) {
-                    const float thr = (float)ld_stream(r.slotThr + sidx) * r.thrScale;
-                    const float T = r.perLinkMaxLoad ? thr : thr * (1.0f + r.scalarMaxLoad);
-                    const bool skip = r.prefilter &&
-                                      certainly_intact(T, r.bendBreak, r.l0Max, dx, dy, dz, rotSelf + rotOther);
-                    if (!skip) {
-                        const int32_t word = f.slotLink[sidx];  // link id (intact: sign bit clear)
-                        const double theta[3] = {g.tx[p], g.ty[p], g.tz[p]};
-                        if (link_breaks(f, word, q, dx, dy, dz, theta)) {
-                            // both ends reach the same verdict; each flags its own slot
-                            mask = 0;
-                            f.slotLinkW[sidx] = word | SLOT_BROKEN;
-                            f.slotOtherW[sidx] = ow | SLOT_BROKEN;
-                            f.state[word] = 0;
-                        }
+                // The prefilter is evaluated for every side and folded into
+                // one branch condition, so the code's load cannot be sunk
+                // into the break branch (where its wait would come late).
+                const float thr = (float)code * r.thrScale;
+                const float T = r.perLinkMaxLoad ? thr : thr * (1.0f + r.scalarMaxLoad);
+                const bool skip =
+                    (r.prefilter != 0) & certainly_intact(T, r.bendBreak, r.l0Max, dx, dy, dz, rotSelf + rotOther);
+                R mask = ow < 0 ? R(0) : R(1);  // broken link (state == 0)
+                if ((ow >= 0) & (f.breakCheck != 0) & !skip) {
+                    const int32_t word = f.slotLink[sidx];  // link id (intact: sign bit clear)
+                    const double theta[3] = {g.tx[p], g.ty[p], g.tz[p]};
+                    if (link_breaks(f, word, q, dx, dy, dz, theta)) {
+                        // both ends reach the same verdict; each flags its own slot
+                        mask = 0;
+                        f.slotLinkW[sidx] = word | SLOT_BROKEN;
+                        f.slotOtherW[sidx] = ow | SLOT_BROKEN;
+                        f.state[word] = 0;
                     }
                 }
-                CT cs[6];
-                ct_load(f.ct, sidx, cs);
                 R ct[6] = {(R)cs[0], (R)cs[1], (R)cs[2], (R)cs[3], (R)cs[4], (R)cs[5]};
                 beam_law<R>(mask, (R)dx, (R)dy, (R)dz, self, other, d, f.fp, ct, F);
                 if (sizeof(R) == 4 && f.linkedContacts) {
