@@ -129,12 +129,14 @@ __device__ __forceinline__ void ct_store(CT *ct, int64_t s, const CT v[6]) {
 
 // ---- FP32-form particle state ------------------------------------------------
 // Gather record of a particle in the FP32 form (64 B: a link partner costs
-// four 16 B loads): coordinates in FP64, Diff rounded to FP32, the rotation
-// bound, the radius (NaN when not tame) and the low parts of Diff0 and Diff1.
+// four 16 B loads): coordinates in FP64, the radius (NaN when not tame), the
+// rotation bound, Diff rounded to FP32 and the low parts of Diff0 and Diff1.
+// The contact test's fields (x, y, z, radius) fill the first 32 B, so an
+// inert neighbour costs two 16 B loads.
 struct __attribute__((aligned(16))) MixedRec {
     double x, y, z;
+    float radius, rot;
     float d[6];
-    float rot, radius;
     float lo[2];
 };
 // The rest of the particle's own FP64 Diff (32 B, read only by the particle):
