@@ -939,7 +939,9 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
             int32_t owNext = cnt > 0 ? ld_stream(f.slotOther + base) : 0;  // partner; sign bit: broken
             // vmcnt(0) here: nothing loaded before the loop is still pending
             // at its header, so the header needs no wait (which, with the
-            // previous side's CT stores outstanding, would drain them too)
+            // previous side's CT stores outstanding, would drain them too).
+            // gfx9 s_waitcnt immediate: vmcnt in bits 3:0 and 15:14 (0),
+            // expcnt in 6:4 and lgkmcnt in 11:8 at their maxima (no wait).
             __builtin_amdgcn_s_waitcnt(0x0F70);
             for (int32_t k = 0; k < cnt; k++) {
                 const int64_t sidx = base + (int64_t)k * 64;
