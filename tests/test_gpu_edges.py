@@ -5,7 +5,10 @@
 - no links at all (contacts only);
 - a duplicated link (two sides of one particle to the same partner), which
   also makes the FP32 form keep link partners in its neighbour list;
-- an isolated particle (no neighbours, no links) beside a block.
+- an isolated particle (no neighbours, no links) beside a block;
+- a pair joined by a single link beside a block (one link side per end);
+- a particle with more than 64 links (wider link slots than the FP32 form's
+  merged list allows).
 """
 import numpy as np
 import pytest
@@ -34,6 +37,18 @@ def squeezed_small():
     cfg["lattice"]["linkGenScale"] = 1.05 * 0.97
     cfg["hdm"]["cells"] = [6, 5, 4]
     return params.build_inputs(cfg)
+
+
+def add_links(l, a, b, length):
+    """l with links a[i]-b[i] of rest length length[i] appended (CT zero)."""
+    m = len(a)
+    n = l.n + m
+    return A.Links(n, np.concatenate([l.a, np.asarray(a, dtype=l.a.dtype)]),
+                   np.concatenate([l.b, np.asarray(b, dtype=l.b.dtype)]),
+                   np.concatenate([l.state, np.ones(m, dtype=l.state.dtype)]),
+                   np.concatenate([l.initialLen, np.asarray(length, dtype=l.initialLen.dtype)]),
+                   None if l.maxLoad is None else np.concatenate([l.maxLoad, np.full(m, l.maxLoad[0])]),
+                   [np.zeros(2 * n) for _ in range(6)])
 
 
 def cat_links(l, extra_idx):
@@ -120,3 +135,35 @@ def test_isolated_particle(orc):
     sim, out = run_both(s, q, l, 30, orc)
     check(sim, out, q)
     assert sim.p["zCord"][p.n] != q["zCord"][p.n]  # it moved (gravity / buoyancy)
+
+
+def test_single_link_pair(squeezed_small, orc):
+    s, p, l = squeezed_small
+    q = A.Particles.zeros(p.n + 2)
+    for f in A.PARTICLE_FIELDS:
+        q.arrays[f][:p.n] = p[f]
+    q.type[:p.n] = p.type
+    x = float(np.max(p["xCord"])) + 20.0  # far beyond the search distance of the block
+    for i, y in enumerate((float(np.mean(p["yCord"])), float(np.mean(p["yCord"])) + 0.97)):
+        q.arrays["radius"][p.n + i] = p["radius"][0]
+        q.arrays["xCord"][p.n + i] = x
+        q.arrays["yCord"][p.n + i] = y
+        q.arrays["zCord"][p.n + i] = float(np.mean(p["zCord"]))
+    ql = add_links(l, [p.n], [p.n + 1], [0.97])
+    say("single link pair", q.n, ql.n)
+    sim, out = run_both(s, q, ql, 30, orc)
+    check(sim, out, q)
+    assert out[A.HDM_PREC_MIXED][2]["merged_contacts"]
+
+
+def test_particle_with_more_than_64_links(squeezed_small, orc):
+    s, p, l = squeezed_small
+    linked = set(l.b[l.a == 0].tolist()) | set(l.a[l.b == 0].tolist())
+    partners = [j for j in range(1, p.n) if j not in linked][:70]
+    xyz = np.stack([p[f] for f in XYZ])
+    length = np.linalg.norm(xyz[:, partners] - xyz[:, [0]], axis=0)  # at rest
+    ql = add_links(l, [0] * len(partners), partners, length)
+    say("particle with", len(partners) + len(linked), "links")
+    sim, out = run_both(s, p, ql, 10, orc)
+    check(sim, out, p, rel=1e-2)
+    assert not out[A.HDM_PREC_MIXED][2]["merged_contacts"]  # 64 sides at most in the merged list
