@@ -5,21 +5,40 @@ A "step" is one full timestep of the hot path -- rebuild check (and rebuild
 when due), contact + beam force gather, time integration, link break -- over
 all 8,000,000 particles / ~48M links resident in HBM.  Prints one JSON line.
 
+Each form runs on its own context from the same generated inputs:
+  1. launch-shape autotuning: whole tuning batches (every shape three times,
+     identical results by construction) until a shape is picked -- never
+     inside the warmup or the timed steps, whatever --warmup is;
+  2. W warmup steps and K timed steps with the form's own rebuild policy,
+     bracketed by device synchronisation;
+  3. the state after the window (positions, the last step's force, broken
+     links) for the error report, and a timed forced rebuild.
+Every form has run the same number of steps when its state is read, so the
+mixed forms' error is taken against the FP64 form at the same step.
+
 value      whole-job particle-steps/s of the headline form, mixed+learned
            (FP32 force arithmetic, FP64 integration and strain, learned
            rebuild scheduler with the exact skin test on the device):
-           N_particles * K / wall time of the K timed steps, bracketed by
-           device synchronisation.  The plain FP64 method (reference rebuild
-           heuristic) and the mixed form with a host-checked exact test run
-           beside it under "forms".
+           N_particles * K / wall time of the K timed steps.  The plain FP64
+           method (reference rebuild heuristic) and the mixed form with a
+           host-checked exact test run beside it under "forms".
 roofline   for the dominant kernel (k_step: link break of the previous step,
            force gather and integration in one launch): algorithmic bytes per
            launch (compulsory traffic of the arrays it must touch, see
            step_bytes) / its mean launch time from HIP events recorded on the
-           context's stream during the timed steps, against 8 TB/s.
+           context's stream during the timed steps, against 8 TB/s.  traffic
+           is the HBM bytes per launch measured with rocprofv3 counters.
+error      each mixed form against the FP64 form after the window: position
+           max|dx| / max|x|, force ||dF||/||F|| (translational, last step),
+           broken-link count difference.
+amortised  the window of an intact slab holds no rebuild; the amortised
+           step adds a forced rebuild (timed here) at the rebuild rate of the
+           policy measured over steps 5k-15k of the same slab.
 cpu_baseline  the CPU oracle (a C/OpenMP restatement of the reference's CPU
-           path, built with the reference's release flags) timed on this host
-           on a bounded sample of the same workload.
+           path) timed on this host: a bounded sample of the 8M slab built
+           with the reference's release flags (-O3 -ffast-math), the same
+           sample without -ffast-math, and the 16k block (2,000 steps) with
+           both builds.
 """
 import argparse
 import json
@@ -37,6 +56,7 @@ from hidem_amd import params  # noqa: E402
 
 METRIC = "particle-steps/s at 8M particles (mixed+learned vs plain FP64); % HBM peak"
 HBM_PEAK_GBS = 8000.0
+TUNE_BATCHES_MAX = 6
 
 
 def log(*a):
@@ -69,26 +89,59 @@ def reference_list_entries(n_list, n_links, exec_info):
     return n_list + (2 * n_links if exec_info["merged_contacts"] else 0)
 
 
-def break_bytes(n_links):
-    """Compulsory bytes of the standalone k_break: A, B, state, initialLen,
-    maxLoad per link (endpoint coordinates come through the cache)."""
-    return n_links * (4 + 4 + 1 + 8 + 8)
-
-
 # Forms: (name, precision, rebuild policy, uses the trained scheduler network)
 FORMS = {
     "fp64": (A.HDM_PREC_FP64, A.HDM_REBUILD_HEURISTIC, False),      # plain FP64 method, reference gridService
     "mixed": (A.HDM_PREC_MIXED, A.HDM_REBUILD_EXACT, False),        # precision form, exact test checked each step
     "mixed+learned": (A.HDM_PREC_MIXED, A.HDM_REBUILD_LEARNED, True),  # precision form + learned scheduler
 }
+# rebuild-rate source of each policy: the 10,000-step evaluation (steps 5k-15k) of the same slab
+POLICY_EVAL = {A.HDM_REBUILD_HEURISTIC: "heuristic", A.HDM_REBUILD_EXACT: "exact", A.HDM_REBUILD_LEARNED: "learned"}
 
 
 def sched_model(config):
     path = os.path.join(ROOT, "hidem_amd", "models", f"sched_{config}.bin")
     if not os.path.exists(path):
-        return None, None
+        return None, None, None
     from hidem_amd import sched_train
-    return sched_train.load_model(path), os.path.relpath(path, ROOT)
+    report = None
+    rj = path[:-4] + ".json"
+    if os.path.exists(rj):
+        with open(rj) as f:
+            report = json.load(f)
+    return sched_train.load_model(path), os.path.relpath(path, ROOT), report
+
+
+def tune_shape(ctx, rp):
+    """Runs whole tuning batches of the learned policy (the only one that
+    tunes) until the context has picked a launch shape; returns the steps
+    this advanced the state by."""
+    rw = A.HdmRunParams.from_buffer_copy(rp)
+    rw.policy = A.HDM_REBUILD_LEARNED
+    batch = 3 * len(ctx.exec_info()["tune_ms"])
+    steps = 0
+    while not ctx.exec_info()["tuned"]:
+        if steps >= TUNE_BATCHES_MAX * batch:
+            raise RuntimeError(f"launch-shape autotuning did not settle in {steps} steps")
+        ctx.run(rw, batch)
+        steps += batch
+    return steps
+
+
+def timed_rebuild(ctx, rp, repeats=3):
+    """Wall time of a forced rebuild (grid, neighbour list, rebuild snapshot),
+    device-synchronised; best of `repeats`."""
+    best = None
+    for _ in range(repeats):
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.grid_build(rp.searchDistance)
+        ctx.neighbours_build(rp.interactDist)
+        ctx.skin_snapshot()
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return best
 
 
 def run_gpu(setup, p, l, prec, policy, model, steps, warmup):
@@ -100,13 +153,11 @@ def run_gpu(setup, p, l, prec, policy, model, steps, warmup):
         ctx.sched_configure(block_cells=4, min_batch=8, max_batch=256, safety=0.5)
         if model is not None:
             ctx.sched_set_model(model)
-        # every form autotunes the step's launch shape during warmup (all shapes give
-        # identical results); the timed steps then use the form's own rebuild policy
+        # every form autotunes its launch shape first (all shapes give identical results)
         ctx.set_exec(autotune=True, graphs=policy == A.HDM_REBUILD_LEARNED)
+        tune_steps = tune_shape(ctx, rp)
         if warmup > 0:
-            rw = A.HdmRunParams.from_buffer_copy(rp)
-            rw.policy = A.HDM_REBUILD_LEARNED
-            ctx.run(rw, warmup)
+            ctx.run(rp, warmup)
         ctx.sync()
         s0 = ctx.sched_stats()
         ctx.set_timing(True)
@@ -123,12 +174,33 @@ def run_gpu(setup, p, l, prec, policy, model, steps, warmup):
         n_neigh = reference_list_entries(int(counts.sum()), l.n, exec_info)
         broken = ctx.count_broken()
         dev_bytes = ctx.device_bytes()
+        q = ctx.download_particles()
+        state = {"x": np.stack([q["xCord"], q["yCord"], q["zCord"]], 1),
+                 "f": np.stack([q["xForce"], q["yForce"], q["zForce"]], 1).astype(np.float64),
+                 "broken": broken, "step": st.timestep}
+        del q
+        rebuild_s = timed_rebuild(ctx, rp)
     sched = {k: s1[k] - s0[k] for k in s1}
     return dict(wall=t1 - t0, stats=st, kernels=kt, n_neigh=n_neigh, broken=broken, dev_bytes=dev_bytes,
-                sched=sched, exec=exec_info)
+                sched=sched, exec=exec_info, state=state, tune_steps=tune_steps, rebuild_s=rebuild_s)
 
 
-def cpu_baseline(setup, p, l, steps, fast=True):
+def error_vs_fp64(s64, s):
+    """Error of a form's state against the FP64 form's state at the same step."""
+    if s["step"] != s64["step"]:
+        return {"comparable": False, "step": s["step"], "step_fp64": s64["step"]}
+    err = np.linalg.norm(s["x"] - s64["x"], axis=1)
+    fnorm = float(np.linalg.norm(s64["f"]))
+    b64, b = s64["broken"], s["broken"]
+    return {"step": int(s["step"]),
+            "pos_max_rel": float(err.max() / np.abs(s64["x"]).max()),
+            "pos_max_abs_m": float(err.max()),
+            "force_rel_l2": float(np.linalg.norm(s["f"] - s64["f"]) / fnorm) if fnorm > 0 else None,
+            "broken_fp64": int(b64), "broken": int(b),
+            "broken_rel_diff": float(abs(b - b64) / b64) if b64 else float(b != 0)}
+
+
+def cpu_sample(setup, p, l, steps, fast):
     from oracle import oracle as orc  # checker / baseline only
     t0 = time.perf_counter()
     sim = orc.OracleSim(p, l, setup, fast=fast)
@@ -138,7 +210,37 @@ def cpu_baseline(setup, p, l, steps, fast=True):
     t2 = time.perf_counter()
     threads = orc.num_threads(fast)
     sim.close()
-    return dict(value=p.n * steps / (t2 - t1), seconds=t2 - t1, setup_seconds=t1 - t0, threads=threads)
+    return {"value": p.n * steps / (t2 - t1), "steps": steps, "seconds": t2 - t1, "setup_seconds": t1 - t0,
+            "threads": threads}
+
+
+def cpu_baseline(setup, p, l, steps, exact_steps, block_steps):
+    """The oracle on this host's cores: the slab with both builds, the 16k block with both."""
+    import platform
+    out = {}
+    log(f"cpu baseline: oracle -ffast-math, {steps} steps of the slab")
+    out["slab_fast_math"] = cpu_sample(setup, p, l, steps, True)
+    if exact_steps > 0:
+        log(f"cpu baseline: oracle without -ffast-math, {exact_steps} steps of the slab")
+        out["slab_ieee"] = cpu_sample(setup, p, l, exact_steps, False)
+    if block_steps > 0:
+        cfg0 = params.load_config("iceblock_16k")
+        s0, p0, l0 = params.build_inputs(cfg0)
+        for key, fast in (("block16k_fast_math", True), ("block16k_ieee", False)):
+            log(f"cpu baseline: {key}, {block_steps} steps")
+            out[key] = cpu_sample(s0, p0, l0, block_steps, fast)
+    cpu_model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    out["cpu_model"] = cpu_model
+    out["host_cpus"] = os.cpu_count()
+    return out
 
 
 def probe_traffic(setup, p, l, forms):
@@ -150,9 +252,7 @@ def probe_traffic(setup, p, l, forms):
             ctx.links_build()
             # the same autotuned launch shape the timed bench steps use
             ctx.set_exec(autotune=True)
-            rw = A.HdmRunParams.from_buffer_copy(setup.run)
-            rw.policy = A.HDM_REBUILD_LEARNED
-            ctx.run(rw, 30)
+            tune_shape(ctx, setup.run)
             if prec == forms[0]:
                 g = ctx.bw_probe(A.HDM_PROBE_COPY, 1 << 30, 1)  # k_probe_copy: 512 MiB read + 512 MiB written
                 log(f"copy probe 512 MiB read + 512 MiB written: {g:.0f} GB/s")
@@ -167,6 +267,10 @@ def probe_traffic(setup, p, l, forms):
                   flush=True)
 
 
+def ratio(a, b):
+    return a / b if b else None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,7 +278,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--config", default="glacier_8m")
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
-    ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=30,
+                    help="oracle steps of the slab in the CPU baseline, -ffast-math build (0: no CPU baseline)")
+    ap.add_argument("--cpu-ieee-steps", type=int, default=10,
+                    help="oracle steps of the slab, build without -ffast-math (0: skip)")
+    ap.add_argument("--cpu-block-steps", type=int, default=2000,
+                    help="oracle steps of the 16k block, both builds (0: skip)")
     ap.add_argument("--cells", type=int, nargs=3, default=None, help="override unit cells (smaller runs)")
     ap.add_argument("--probe-traffic", action="store_true",
                     help="for rocprofv3 --pmc runs: a 512 MiB copy kernel of known traffic, then 3 force launches "
@@ -182,8 +291,11 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "this build measures one GPU; runs on more than one GPU are not part of it",
                           "n_gpus": args.gpus}))
+        sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        print(json.dumps({"error": "--steps must be >= 1 and --warmup >= 0"}))
         sys.exit(2)
 
     cfg = params.load_config(args.config)
@@ -196,21 +308,34 @@ def main():
     if args.probe_traffic:
         probe_traffic(setup, p, l, sorted({FORMS[n][0] for n in names}))
         return
-    model, model_path = sched_model(args.config)
+    model, model_path, model_report = sched_model(args.config)
     results = {}
     for name in names:
         prec, policy, learned = FORMS[name]
-        log(f"running {name}: warmup {args.warmup}, timed {args.steps}")
+        log(f"running {name}: autotune, warmup {args.warmup}, timed {args.steps}")
         r = run_gpu(setup, p, l, prec, policy, model if learned else None, args.steps, args.warmup)
         r["value"] = p.n * args.steps / r["wall"]
         sms, scnt = r["kernels"]["step"]
         executed = r["sched"]["steps_executed"] if policy == A.HDM_REBUILD_LEARNED else scnt
-        r["step_avg_ms"] = sms / max(executed, 1)     # skipped (guarded) launches cost ~µs and are not steps
+        if sms <= 0 or executed <= 0:
+            raise RuntimeError(f"{name}: no timed step launches recorded ({sms} ms over {executed} steps)")
+        r["step_avg_ms"] = sms / executed     # skipped (guarded) launches cost ~µs and are not steps
         r["step_bytes"] = step_bytes(p.n, r["n_neigh"], l.n, prec)
+        r["policy"] = policy
         results[name] = r
         log(f"  {name}: {r['value']:.4e} p-steps/s, {1e3 * r['wall'] / args.steps:.3f} ms/step, "
             f"step kernel {r['step_avg_ms']:.3f} ms ({r['step_bytes'] / r['step_avg_ms'] / 1e6:.0f} GB/s), "
-            f"rebuilds {r['stats'].rebuilds}, broken {r['broken']}, sched {r['sched']}")
+            f"shape {r['exec']['name']} (tuned in {r['tune_steps']} steps), rebuilds {r['stats'].rebuilds}, "
+            f"broken {r['broken']}, forced rebuild {1e3 * r['rebuild_s']:.1f} ms, sched {r['sched']}")
+
+    errors = {}
+    if "fp64" in results:
+        for name in ("mixed", "mixed+learned"):
+            if name in results:
+                errors[name] = error_vs_fp64(results["fp64"]["state"], results[name]["state"])
+                log(f"  {name} vs fp64: {errors[name]}")
+    for r in results.values():
+        del r["state"]
 
     head = next(n for n in ("mixed+learned", "mixed", "fp64") if n in results)
     h = results[head]
@@ -221,13 +346,26 @@ def main():
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
+    def amortised(r):
+        ev = (model_report or {}).get("evaluation", {}).get(POLICY_EVAL[r["policy"]])
+        if not ev or args.cells is not None:
+            return None
+        rate = ev["rebuilds"] / model_report["eval_steps"]
+        ms = 1e3 * r["wall"] / args.steps + 1e3 * r["rebuild_s"] * rate
+        return {"ms_per_step": ms, "value": p.n / (ms * 1e-3), "rebuild_ms": 1e3 * r["rebuild_s"],
+                "rebuilds_per_step": rate,
+                "rate_source": f"{model_report['eval_steps']}-step evaluation of the "
+                               f"{POLICY_EVAL[r['policy']]} policy, steps 5k-15k ({model_path[:-4]}.json)"}
+
     cpu = None
     if args.cpu_steps > 0:
-        log(f"cpu baseline: oracle, {args.cpu_steps} steps")
-        c = cpu_baseline(setup, p, l, args.cpu_steps)
-        cpu = {"value": c["value"], "unit": "particle-steps/s", "cores": c["threads"], "kind": "port",
-               "sample": f"{args.cpu_steps} timesteps of the full {p.n}-particle workload (after 1 setup step), "
-                         f"C/OpenMP oracle built -O3 -ffast-math, {c['seconds']:.1f} s"}
+        c = cpu_baseline(setup, p, l, args.cpu_steps, args.cpu_ieee_steps, args.cpu_block_steps)
+        s = c["slab_fast_math"]
+        cpu = {"value": s["value"], "unit": "particle-steps/s", "cores": s["threads"], "kind": "port",
+               "sample": f"{s['steps']} timesteps of the full {p.n}-particle workload (after 1 setup step), "
+                         f"C/OpenMP oracle built -O3 -ffast-math (the reference's release flags), "
+                         f"{s['seconds']:.1f} s on {s['threads']} threads of {c['cpu_model']}",
+               "runs": c}
 
     kern = {}
     for name, r in results.items():
@@ -252,25 +390,38 @@ def main():
                            "mixed+learned": "FP32 force / FP64 integration, learned rebuild scheduler "
                                             "with the exact skin test on the device"}[head],
             "particles": p.n, "links": l.n, "form": head,
+            "lattice": "a = 1.0 m, nearest-neighbour distance 0.70711 m, particle diameter 0.66711 m: "
+                       "a 4 cm gap, so the intact slab starts free of contact pre-stress and its "
+                       "neighbour pairs are all beyond contact range in the timed window",
+            "steps_before_window": int(h["tune_steps"] + args.warmup),
             "rebuilds_in_timed_steps": int(h["stats"].rebuilds),
             "scheduler_model": model_path if head == "mixed+learned" else None,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_over_algorithmic": ratio(traffic, h["step_bytes"]) if traffic else None,
+                     "traffic_GBps": traffic / (h["step_avg_ms"] * 1e-3) / 1e9 if traffic else None,
                      "kernel": "k_step (link break + force gather + integration)",
                      "bytes_per_launch": h["step_bytes"], "avg_launch_ms": h["step_avg_ms"]},
+        "error_vs_fp64": errors.get(head) if head != "fp64" else None,
+        "amortised_with_rebuilds": amortised(h),
         "cpu_baseline": cpu,
         "forms": {name: {"value": r["value"], "ms_per_step": 1e3 * r["wall"] / args.steps,
                          "step_avg_ms": r["step_avg_ms"],
                          "step_GBps": r["step_bytes"] / (r["step_avg_ms"] * 1e-3) / 1e9,
                          "rebuilds": int(r["stats"].rebuilds), "broken_links": r["broken"],
                          "device_bytes": r["dev_bytes"], "kernels": kern[name], "scheduler": r["sched"],
-                         "launch_shape": r["exec"]}
+                         "launch_shape": r["exec"], "tune_steps": r["tune_steps"],
+                         "forced_rebuild_ms": 1e3 * r["rebuild_s"],
+                         "error_vs_fp64": errors.get(name), "amortised_with_rebuilds": amortised(r)}
                   for name, r in results.items()},
     }
     for name in ("mixed", "mixed+learned"):
         if "fp64" in results and name in results:
-            out[f"{name}_over_fp64"] = results[name]["value"] / results["fp64"]["value"]
+            out[f"{name}_over_fp64"] = ratio(results[name]["value"], results["fp64"]["value"])
+            a, b = out["forms"][name]["amortised_with_rebuilds"], out["forms"]["fp64"]["amortised_with_rebuilds"]
+            if a and b:
+                out[f"{name}_over_fp64_amortised"] = ratio(a["value"], b["value"])
     print(json.dumps(out), flush=True)
 
 

@@ -1146,6 +1146,8 @@ int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t 
                 hipEvent_t a = nullptr, b = nullptr;
                 HIP_CHECK(ctx, hipEventCreate(&a));
                 HIP_CHECK(ctx, hipEventCreate(&b));
+                // a tuning launch is a step like any other for the step timer
+                Timer t(c, HDM_T_STEP);
                 HIP_CHECK(ctx, hipEventRecord(a, c.stream));
                 HIP_CHECK(ctx, hdm::launch_step(c, rp->force, rp->integ, skin, true, breakPending ? &rp->brk : nullptr,
                                                 s == nSteps - 1, &guard));
