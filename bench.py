@@ -278,9 +278,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--config", default="glacier_8m")
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
-    ap.add_argument("--cpu-steps", type=int, default=30,
+    ap.add_argument("--cpu-steps", type=int, default=100,
                     help="oracle steps of the slab in the CPU baseline, -ffast-math build (0: no CPU baseline)")
-    ap.add_argument("--cpu-ieee-steps", type=int, default=10,
+    ap.add_argument("--cpu-ieee-steps", type=int, default=20,
                     help="oracle steps of the slab, build without -ffast-math (0: skip)")
     ap.add_argument("--cpu-block-steps", type=int, default=2000,
                     help="oracle steps of the 16k block, both builds (0: skip)")

@@ -238,6 +238,7 @@ def _declare(L):
         "hdm_bw_probe": ([P, C.c_int32, C.c_int64, C.c_int32, _dp], C.c_int),
         "hdm_set_timestep": ([P, C.c_int64], C.c_int),
         "hdm_count_broken": ([P, _i64p], C.c_int),
+        "hdm_particle_stats": ([P, _i32p, _i32p, _i64p, _i64p], C.c_int),
         "hdm_config_derive": ([C.c_int32, C.POINTER(C.c_char_p), C.POINTER(HdmSetupInfo)], C.c_int),
         "hdm_sim_create": ([C.c_int32, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(P)], C.c_int),
         "hdm_sim_destroy": ([P], C.c_int),
@@ -502,6 +503,16 @@ class Context:
         nb = C.c_int64()
         self._c(lib().hdm_break(self._h, C.byref(bp), C.byref(nb)), "hdm_break")
         return nb.value
+
+    def particle_stats(self):
+        """finalizeTs statistics: per-particle active links and reference
+        neighbour-list entries, and their sums (hdm_particle_stats)."""
+        act = np.zeros(self.n, np.int32)
+        nbr = np.zeros(self.n, np.int32)
+        sa, sn = C.c_int64(), C.c_int64()
+        self._c(lib().hdm_particle_stats(self._h, _ptr(act, C.c_int32), _ptr(nbr, C.c_int32), C.byref(sa),
+                                         C.byref(sn)), "hdm_particle_stats")
+        return act, nbr, sa.value, sn.value
 
     def skin_heuristic(self, step_size=10):
         d = C.c_double()

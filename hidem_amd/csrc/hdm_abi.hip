@@ -260,7 +260,7 @@ int hdm_device_bytes(hdm_ctx *ctx, int64_t *bytes) {
     size_t t = 0;
     const hdm::DevBuf *bufs[] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz, &c.radius, &c.type, &c.la, &c.lb,
                                  &c.state, &c.initialLen, &c.maxLoad, &c.linkCount, &c.linkCursor, &c.slotLink,
-                                 &c.slotOther, &c.sidePos, &c.nbrCount, &c.nbr, &c.colour, &c.order,
+                                 &c.slotOther, &c.sidePos, &c.nbrCount, &c.nbrLinked, &c.nbr, &c.colour, &c.order,
                                  &c.cellStart, &c.cellCursor, &c.scanPartials, &c.red, &c.x0, &c.y0, &c.z0,
                                  &c.skinPartials, &c.blockId, &c.blockWater, &c.probe, &c.rec, &c.recAlt,
                                  &c.drec, &c.drecAlt, &c.dtail, &c.dtailAlt, &c.slotThr};
@@ -276,6 +276,7 @@ int hdm_device_bytes(hdm_ctx *ctx, int64_t *bytes) {
 int hdm_particles_upload(hdm_ctx *ctx, const hdm_particles_host *p) {
     if (!valid(ctx) || !p || p->n < 0) return HDM_E_INVALID;
     Context &c = ctx->c;
+    c.stateEpoch++;
     if (p->n > 0 && (!p->xCord || !p->yCord || !p->zCord || !p->radius || !p->xTheta || !p->yTheta ||
                      !p->zTheta || !p->xDiff || !p->yDiff || !p->zDiff || !p->uDiff || !p->vDiff || !p->wDiff))
         return fail(ctx, HDM_E_INVALID, "hdm_particles_upload: missing required field");
@@ -343,6 +344,7 @@ int hdm_particles_download(hdm_ctx *ctx, hdm_particles_host *p) {
 int hdm_links_upload(hdm_ctx *ctx, const hdm_links_host *l) {
     if (!valid(ctx) || !l || l->n < 0) return HDM_E_INVALID;
     Context &c = ctx->c;
+    c.stateEpoch++;
     if (l->n > 0 && (!l->localPartIndA || !l->localPartIndB || !l->state || !l->initialLen))
         return fail(ctx, HDM_E_INVALID, "hdm_links_upload: missing required field");
     if (2 * l->n >= (int64_t)INT32_MAX) return fail(ctx, HDM_E_CAPACITY, "more than int32 link sides");
@@ -467,14 +469,24 @@ int hdm_neighbours_build(hdm_ctx *ctx, double interactDist) {
     if (!c.grid.built) return fail(ctx, HDM_E_STATE, "grid not built");
     Timer t(c, HDM_T_NEIGH);
     HIP_CHECK(ctx, c.nbrCount.ensure(sizeof(int32_t) * (size_t)c.n));
-    HIP_CHECK(ctx, hdm::launch_neigh_count(c, interactDist));
+    HIP_CHECK(ctx, c.nbrLinked.ensure(sizeof(int32_t) * (size_t)c.n));
+    // the cell records must hold the current positions
+    if (c.grid.recEpoch != c.stateEpoch || c.grid.recTs != c.ts) HIP_CHECK(ctx, hdm::launch_cell_records(c));
     uint64_t *mx = c.red.as<uint64_t>() + 12;
+    // One fill pass into the previous build's width; a count pass first
+    // only when there is none, and a second fill when some count exceeds it.
+    const bool fillFirst = c.maxNeigh > 0 && c.nbr.bytes >= sizeof(int32_t) * (size_t)c.tiles() * 64 * (size_t)c.maxNeigh;
+    if (fillFirst) HIP_CHECK(ctx, hdm::launch_neigh_fill(c, interactDist));
+    else HIP_CHECK(ctx, hdm::launch_neigh_count(c, interactDist));
     HIP_CHECK(ctx, hdm::launch_max_i32(c, c.nbrCount.as<int32_t>(), c.n, mx));
     uint64_t m = 0;
     HIP_CHECK(ctx, d2h(&m, mx, sizeof(m), c.stream));
-    c.maxNeigh = (int32_t)(m > 0 ? m : 1);
-    HIP_CHECK(ctx, c.nbr.ensure(sizeof(int32_t) * (size_t)c.tiles() * 64 * (size_t)c.maxNeigh));
-    HIP_CHECK(ctx, hdm::launch_neigh_fill(c, interactDist));
+    c.maxNeighCount = (int32_t)m;
+    if (!fillFirst || (int64_t)m > c.maxNeigh) {
+        c.maxNeigh = (int32_t)(m > 0 ? m : 1);
+        HIP_CHECK(ctx, c.nbr.ensure(sizeof(int32_t) * (size_t)c.tiles() * 64 * (size_t)c.maxNeigh));
+        HIP_CHECK(ctx, hdm::launch_neigh_fill(c, interactDist));
+    }
     c.neighBuilt = true;
     c.nbrExcludesLinked = hdm::merge_linked_contacts(c);
     return HDM_OK;
@@ -489,7 +501,7 @@ int hdm_neighbours_total(hdm_ctx *ctx, int64_t *total, int32_t *maxPerParticle) 
     int64_t t = 0;
     for (auto v : cnt) t += v;
     if (total) *total = t;
-    if (maxPerParticle) *maxPerParticle = c.maxNeigh;
+    if (maxPerParticle) *maxPerParticle = c.maxNeighCount;
     return HDM_OK;
 }
 
@@ -599,6 +611,7 @@ int hdm_linklists_download(hdm_ctx *ctx, int32_t *counts, int64_t *starts, int32
 int hdm_force(hdm_ctx *ctx, const hdm_force_params *fp) {
     if (!valid(ctx) || !fp) return HDM_E_INVALID;
     Context &c = ctx->c;
+    c.stateEpoch++;
     if (!c.neighBuilt || !c.linksBuilt) return fail(ctx, HDM_E_STATE, "neighbour list / link lists not built");
     Timer t(c, HDM_T_FORCE);
     HIP_CHECK(ctx, hdm::launch_force(c, *fp));
@@ -609,6 +622,7 @@ int hdm_integrate(hdm_ctx *ctx, const hdm_integ_params *ip) {
     if (!valid(ctx) || !ip) return HDM_E_INVALID;
     if (ip->variant != HDM_INTEG_ICEBLOCK && ip->variant != HDM_INTEG_GLACIER)
         return fail(ctx, HDM_E_INVALID, "unknown integrator variant");
+    ctx->c.stateEpoch++;
     Timer t(ctx->c, HDM_T_INTEGRATE);
     HIP_CHECK(ctx, hdm::launch_integrate(ctx->c, *ip));
     ctx->c.skinFused = false;
@@ -620,6 +634,7 @@ int hdm_integrate(hdm_ctx *ctx, const hdm_integ_params *ip) {
 int hdm_break(hdm_ctx *ctx, const hdm_break_params *bp, int64_t *nBroken) {
     if (!valid(ctx) || !bp) return HDM_E_INVALID;
     Context &c = ctx->c;
+    c.stateEpoch++;
     if (!c.linksBuilt) return fail(ctx, HDM_E_STATE, "link lists not built (link end indices unchecked)");
     uint64_t *counter = c.red.as<uint64_t>() + 16;
     if (nBroken) HIP_CHECK(ctx, hdm::launch_fill_u64(c.stream, counter, 0ull, 1));
@@ -684,6 +699,26 @@ int hdm_count_broken(hdm_ctx *ctx, int64_t *broken) {
     return HDM_OK;
 }
 
+int hdm_particle_stats(hdm_ctx *ctx, int32_t *activeLinks, int32_t *iteratedNeighbours, int64_t *activeLinksSum,
+                       int64_t *iteratedNeighboursSum) {
+    if (!valid(ctx)) return HDM_E_INVALID;
+    Context &c = ctx->c;
+    if (!c.neighBuilt || !c.linksBuilt) return fail(ctx, HDM_E_STATE, "neighbour list / link lists not built");
+    const bool perParticle = activeLinks || iteratedNeighbours;
+    if (perParticle) HIP_CHECK(ctx, c.statsBuf.ensure(sizeof(int32_t) * 2 * (size_t)(c.n > 0 ? c.n : 1)));
+    int32_t *dAct = perParticle ? c.statsBuf.as<int32_t>() : nullptr;
+    int32_t *dNbr = perParticle ? c.statsBuf.as<int32_t>() + c.n : nullptr;
+    uint64_t *sums = c.red.as<uint64_t>() + 56;
+    HIP_CHECK(ctx, hdm::launch_particle_stats(c, dAct, dNbr, sums));
+    uint64_t v[2] = {0, 0};
+    HIP_CHECK(ctx, d2h(v, sums, sizeof(v), c.stream));
+    if (activeLinks) HIP_CHECK(ctx, d2h(activeLinks, dAct, sizeof(int32_t) * (size_t)c.n, c.stream));
+    if (iteratedNeighbours) HIP_CHECK(ctx, d2h(iteratedNeighbours, dNbr, sizeof(int32_t) * (size_t)c.n, c.stream));
+    if (activeLinksSum) *activeLinksSum = (int64_t)v[0];
+    if (iteratedNeighboursSum) *iteratedNeighboursSum = (int64_t)v[1];
+    return HDM_OK;
+}
+
 int hdm_set_blocks(hdm_ctx *ctx, const int32_t *blockId, int32_t nBlocks, const double *waterLevel) {
     if (!valid(ctx) || nBlocks < 0) return HDM_E_INVALID;
     Context &c = ctx->c;
@@ -706,6 +741,7 @@ int hdm_set_blocks(hdm_ctx *ctx, const int32_t *blockId, int32_t nBlocks, const 
 int hdm_reorder(hdm_ctx *ctx, double searchDistance, int64_t *order, int64_t *linkOrder) {
     if (!valid(ctx)) return HDM_E_INVALID;
     Context &c = ctx->c;
+    c.stateEpoch++;
     if (!(searchDistance > 0)) return fail(ctx, HDM_E_INVALID, "hdm_reorder: searchDistance must be > 0");
     if (c.n <= 1) return fail(ctx, HDM_E_STATE, "hdm_reorder: fewer than two particles");
     // link endpoints are checked by the link build; the reorder indexes with them
@@ -887,6 +923,7 @@ int hdm_bw_probe(hdm_ctx *ctx, int32_t kind, int64_t bytes, int32_t iters, doubl
 
 int hdm_set_timestep(hdm_ctx *ctx, int64_t ts) {
     if (!valid(ctx) || ts < 0) return HDM_E_INVALID;
+    ctx->c.stateEpoch++;
     ctx->c.ts = ts;
     ctx->c.accumulatedDist = 0;
     return HDM_OK;
@@ -905,6 +942,7 @@ namespace {
 // for the exact test; the scheduler's blocks follow the new grid.
 int rebuild(hdm_ctx *ctx, const hdm_run_params *rp) {
     Context &c = ctx->c;
+    c.stateEpoch++;
     c.accumulatedDist = 0;
     int r = hdm_grid_build(ctx, rp->searchDistance);
     if (r == HDM_OK) r = hdm_neighbours_build(ctx, rp->interactDist);
@@ -1080,6 +1118,9 @@ int graph_pair(hdm_ctx *ctx, const hdm_run_params *rp, uint64_t *skin, const hdm
     Timer t(c, HDM_T_STEP);
     HIP_CHECK(ctx, hipGraphLaunch(c.graphExec, c.stream));
     c.graphReplays++;
+    // the replayed steps advanced the records, as launch_step notes for its
+    // own launches: the SoA coordinates and Diff now lag them
+    c.soaStale = true;
     return HDM_OK;
 }
 
@@ -1089,7 +1130,7 @@ int graph_pair(hdm_ctx *ctx, const hdm_run_params *rp, uint64_t *skin, const hdm
 // nothing, so the state advances exactly as under HDM_REBUILD_EXACT.
 int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats *stats) {
     Context &c = ctx->c;
-    uint64_t *skin = c.red.as<uint64_t>() + 32;
+    uint64_t *skin = c.red.as<uint64_t>() + hdm::SKIN_KEYS;
     const double thr = rp->interactDist - rp->force.attractDistance;
     hdm::StepGuard guard{thr};
     auto t0 = std::chrono::steady_clock::now();
@@ -1119,9 +1160,15 @@ int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t 
             if (rebuildLog) rebuildLog[step] = 1;
         }
         int64_t H = 0;
-        int r = sched_horizon(ctx, rp, &H);
-        if (r != HDM_OK) return r;
+        if (c.schedCarry > 0 && c.schedCarryEpoch == c.stateEpoch && c.schedCarryTs == c.ts) {
+            H = c.schedCarry;   // the rest of the batch an earlier run scheduled
+        } else {
+            int r = sched_horizon(ctx, rp, &H);
+            if (r != HDM_OK) return r;
+        }
+        c.schedCarry = 0;
         if (c.autotune && !c.tuned) H = 3 * hdm::STEP_VARIANTS;   // a tuning batch: every shape three times
+        const int64_t Hplanned = H;
         if (H > nSteps - step) H = nSteps - step;
         // guard flag and executed-step count start at 0 (a fill kernel, capture-safe)
         HIP_CHECK(ctx, hdm::launch_guard_reset(c));
@@ -1204,10 +1251,15 @@ int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t 
         c.skinFused = true;
         c.skinFusedDisp = true;
         if (k == 0 && !regen) return fail(ctx, HDM_E_STATE, "guarded batch executed no step");
+        if (!regen && k == H && H < Hplanned) {   // the run ended inside the batch
+            c.schedCarry = Hplanned - H;
+            c.schedCarryTs = c.ts;
+            c.schedCarryEpoch = c.stateEpoch;
+        }
     }
     if (breakPending) {
-        int r = hdm_break(ctx, &rp->brk, nullptr);
-        if (r != HDM_OK) return r;
+        Timer t(c, HDM_T_BREAK);
+        HIP_CHECK(ctx, hdm::launch_break_records(c, rp->brk, nullptr));
     }
     HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
     auto t1 = std::chrono::steady_clock::now();
@@ -1240,7 +1292,8 @@ int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *reb
         if (r != HDM_OK) return r;
     }
     if (rp->policy == HDM_REBUILD_LEARNED) return run_learned(ctx, rp, nSteps, rebuildLog, stats);
-    uint64_t *skin = c.red.as<uint64_t>() + 32;
+    c.stateEpoch++;
+    uint64_t *skin = c.red.as<uint64_t>() + hdm::SKIN_KEYS;
     const bool wantSkin = rp->policy != HDM_REBUILD_INTERVAL;
     const bool wantDisp = rp->policy == HDM_REBUILD_EXACT;
     auto t0 = std::chrono::steady_clock::now();
@@ -1299,8 +1352,8 @@ int hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *reb
         c.stepsSinceRebuild++;
     }
     if (breakPending) {
-        int r = hdm_break(ctx, &rp->brk, nullptr);
-        if (r != HDM_OK) return r;
+        Timer t(c, HDM_T_BREAK);
+        HIP_CHECK(ctx, hdm::launch_break_records(c, rp->brk, nullptr));
     }
     HIP_CHECK(ctx, hipStreamSynchronize(c.stream));
     auto t1 = std::chrono::steady_clock::now();
@@ -1324,6 +1377,7 @@ int hdm_sched_configure(hdm_ctx *ctx, const hdm_sched_params *sp) {
     if (sp->blockCells < 1 || sp->minBatch < 1 || sp->maxBatch < sp->minBatch || !(sp->safety > 0.0f))
         return fail(ctx, HDM_E_INVALID, "hdm_sched_params: need blockCells >= 1, 1 <= minBatch <= maxBatch, safety > 0");
     Context &c = ctx->c;
+    c.stateEpoch++;
     if (sp->blockCells != c.sched.blockCells) c.schedAssigned = false;
     c.sched = *sp;
     return HDM_OK;
@@ -1332,6 +1386,7 @@ int hdm_sched_configure(hdm_ctx *ctx, const hdm_sched_params *sp) {
 int hdm_sched_set_model(hdm_ctx *ctx, const hdm_mlp_weights *w) {
     if (!valid(ctx)) return HDM_E_INVALID;
     Context &c = ctx->c;
+    c.stateEpoch++;
     if (!w) {
         c.haveModel = false;
         return HDM_OK;

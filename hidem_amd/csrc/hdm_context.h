@@ -79,6 +79,9 @@ struct GridState {
     int64_t occupied = 0;
     int32_t maxPerCell = 0;
     bool built = false;
+    // the cell records hold the positions of this state (Context::stateEpoch, ts)
+    uint64_t recEpoch = 0;
+    int64_t recTs = -1;
 };
 
 // One training-label reference of the learned scheduler (hdm_sched_label_*).
@@ -155,12 +158,18 @@ struct Context {
     float thrScale = 0.0f; // threshold = code * thrScale (at most the link's L0 (1 + maxLoad), or L0)
     float l0Max = 0.0f;    // largest initialLen (rounded up to float)
     // neighbour list (wave tiles)
-    int32_t maxNeigh = 0;
+    int32_t maxNeigh = 0;      // ELL width of the list (at least the largest count; kept across builds)
+    int32_t maxNeighCount = 0; // largest count of the current list
     bool neighBuilt = false;
     DevBuf nbrCount, nbr;
+    DevBuf statsBuf;       // i32 [2][n]: hdm_particle_stats output staging
+    DevBuf nbrLinked;      // i32 per particle: link partners within the skin that the list leaves out
+                           // (merge_linked_contacts), so the reference list holds nbrCount + nbrLinked
     // grid
     GridState grid;
     DevBuf colour, order, cellStart, cellCursor, scanPartials;
+    DevBuf cellRec;        // double4 [n]: {x, y, z, radius} of order[i] (the list build's candidate records)
+    DevBuf cellPos;        // i32 [n]: position of each particle in order
     // reductions / scalars
     DevBuf red;            // u64 scratch for bbox / max reductions and counters
     DevBuf skinPartials;   // f64 [workgroups][4]: rebuild-test maxima per integrating workgroup
@@ -187,6 +196,12 @@ struct Context {
     bool schedAssigned = false;  // block ids match the last grid build
     int64_t stepsSinceRebuild = 0, schedLastEval = 0;
     int64_t schedChecks = 0, schedLaunched = 0, schedExecuted = 0;
+    // Horizon left over when a learned run ended inside a scheduled batch:
+    // the next run continues that batch instead of evaluating the network
+    // again, as one longer run would.  Valid while the state is the one the
+    // run left (stateEpoch unchanged, same timestep).
+    int64_t schedCarry = 0, schedCarryTs = -1;
+    uint64_t stateEpoch = 0, schedCarryEpoch = 0;
     DevBuf schedBlockOf, schedAcc, schedFeat, schedDisp, schedValid, schedPred, schedBrokenPrev;
     int32_t schedNb = 0;         // blocks of the last feature evaluation
     hdm_mlp_weights schedModel;  // the scheduler's network (host copy)
@@ -243,7 +258,9 @@ hipError_t launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t v, int64_t n);
 hipError_t launch_bbox(Context &c);
 hipError_t launch_colour_count(Context &c);
 hipError_t launch_exclusive_scan_i32(Context &c, const int32_t *in, int32_t *out, int64_t count);
-hipError_t launch_scatter_sort(Context &c);
+hipError_t launch_scatter_sort(Context &c);   // ends with launch_cell_records
+// cellRec / cellPos from the current positions in the grid's order.
+hipError_t launch_cell_records(Context &c);
 hipError_t launch_neigh_count(Context &c, double interactDist);
 hipError_t launch_neigh_fill(Context &c, double interactDist);
 hipError_t launch_max_i32(Context &c, const int32_t *v, int64_t n, uint64_t *out);
@@ -282,6 +299,10 @@ int pick_step_variant(const double ms[STEP_VARIANTS]);
 
 // Guard words in Context::guardState.
 enum { GUARD_TICKET = 0, GUARD_FLAG = 1, GUARD_STEPS = 2, GUARD_WORDS = 4 };
+// u64 slot of Context::red where the skin keys of the last integrating
+// launch start (max+ Diff per axis at SKIN_KEYS + 0..2, max |x - x0|^2 at
+// SKIN_KEYS + 4).
+enum { SKIN_KEYS = 32 };
 
 // Exact rebuild test evaluated on the device by launch_step: the step's last
 // workgroup sets the guard flag when 2 max |x - x0| >= threshold; steps
@@ -302,10 +323,18 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
 // Clears the guard flag and the executed-step counter (GUARD_FLAG, GUARD_STEPS).
 hipError_t launch_guard_reset(Context &c);
 hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter);
+// The same break per particle from the current records (hdm_run's last step;
+// launch_break when the records are not current).  counter: links broken.
+hipError_t launch_break_records(Context &c, const hdm_break_params &bp, uint64_t *counter);
 hipError_t launch_skin_heuristic(Context &c, uint64_t *out3);
 hipError_t launch_skin_exact(Context &c, uint64_t *out1);
 hipError_t launch_snapshot(Context &c);
 hipError_t launch_count_broken(Context &c, uint64_t *counter);
+// computeActiveLinks / computeIteratedNeighbours (HiDEMLib.h:197-232):
+// per particle its intact links and its reference neighbour-list entries
+// (activeLinks / iteratedNeighbours, device arrays of n, may be nullptr) and
+// their sums into sums[0], sums[1].
+hipError_t launch_particle_stats(Context &c, int32_t *activeLinks, int32_t *iteratedNeighbours, uint64_t *sums);
 // learned scheduler (kernels_sched.hip)
 int64_t mlp_param_count();
 hipError_t sched_assign_blocks(Context &c, const SchedGeometry &g);

@@ -116,6 +116,17 @@ public:
     void applyForce(const hdm_integ_params &ip);
     void linkUpdate(const hidem::HiDEMDefLinkBreakFunc &b);
     void finalizeTs();                // HiDEMLib.h:495-530
+    // finalizeTs statistics (HiDEMLib.h:507-529): after the steps named in
+    // [output] statIntervals and every multiple of the last one, the sums of
+    // computeActiveLinks and computeIteratedNeighbours (hdm_particle_stats),
+    // printed as the reference logs them and kept in the members below
+    std::vector<int> statIntervals{100, 200, 1000, 5000, 10000, 20000, 50000};
+    bool printStats = true;
+    bool statsDue() const;
+    int nextStats() const;            // first due step after the current one (INT_MAX: none)
+    void writeStats();
+    int statsTimestep = -1;
+    int64_t linksIterated = 0, neighboursIterated = 0;
     int getCurrentTimestep() const { return currentTimestep; }
 
     int64_t numParticles() const { return nParticles; }

@@ -545,7 +545,34 @@ void HiDEMLibHip::linkUpdate(const hidem::HiDEMDefLinkBreakFunc &b) { accel.iter
 
 void HiDEMLibHip::finalizeTs() {
     currentTimestep++;
+    if (statsDue()) writeStats();
     periodicCheckpoint();
+}
+
+bool HiDEMLibHip::statsDue() const {
+    if (statIntervals.empty()) return false;
+    for (int i : statIntervals)
+        if (i == currentTimestep) return true;
+    return statIntervals.back() > 0 && currentTimestep % statIntervals.back() == 0;
+}
+
+int HiDEMLibHip::nextStats() const {
+    int next = std::numeric_limits<int>::max();
+    if (statIntervals.empty()) return next;
+    for (int i : statIntervals)
+        if (i > currentTimestep) next = std::min(next, i);
+    const int last = statIntervals.back();
+    if (last > 0) next = std::min(next, (currentTimestep / last + 1) * last);
+    return next;
+}
+
+void HiDEMLibHip::writeStats() {
+    hidem::hdm_check(accel.ctx, hdm_particle_stats(accel.ctx, nullptr, nullptr, &linksIterated, &neighboursIterated),
+                     "hdm_particle_stats");
+    statsTimestep = currentTimestep;
+    if (printStats)
+        std::printf("timestep done %d\niteratedLinks %lld links\niteratedNegibours %lld neighbours\n", currentTimestep,
+                    (long long)linksIterated, (long long)neighboursIterated);
 }
 
 void HiDEMLibHip::readCheckpointConfig(ArgParser &parser) {
@@ -555,6 +582,8 @@ void HiDEMLibHip::readCheckpointConfig(ArgParser &parser) {
     checkpointInterval = arg_or<int>(parser, "restart.checkpointInterval", 10000);
     checkpointDirectory = arg_or<std::string>(parser, "restart.checkpointDirectory", "checkpoints");
     outWorkingPath = arg_or<std::string>(parser, "output.outWorkingPath", "");
+    statIntervals = arg_or<std::vector<int>>(parser, "output.statIntervals", statIntervals);
+    printStats = arg_or<bool>(parser, "hdm.printStats", true);
 }
 
 static void write_named_checkpoint(HiDEMLibHip &h) {
@@ -761,9 +790,10 @@ int hdm_sim_steps(hdm_sim *s, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats
                     const int64_t next = ((int64_t)h.getCurrentTimestep() / iv + 1) * iv;
                     k = std::min<int64_t>(k, next - h.getCurrentTimestep());
                 }
-                // runs also end where a redistribution is due
+                // runs also end where a redistribution is due and after steps that print statistics
                 if (h.redistributionDue()) h.redistributeNow();
                 k = std::min<int64_t>(k, (int64_t)h.nextRedistribution() - h.getCurrentTimestep());
+                k = std::min<int64_t>(k, (int64_t)h.nextStats() - h.getCurrentTimestep());
                 hdm_run_stats st{};
                 rc = hdm_run(h.accel.ctx, &rp, k, rebuildLog ? rebuildLog + done : nullptr, &st);
                 if (rc != HDM_OK) {
@@ -771,6 +801,7 @@ int hdm_sim_steps(hdm_sim *s, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats
                     return rc;
                 }
                 h.syncFromRun(st);
+                if (h.statsDue()) h.writeStats();
                 h.periodicCheckpoint();
                 done += st.steps;
                 total.steps += st.steps;

@@ -12,13 +12,17 @@
 //   scatter_sort   NeighbourGrid::generateIterationList (:163-184): particles
 //                  grouped by cell; each cell's members then sorted by pInd so
 //                  iterationOrder is exactly "sorted by (colour, pInd)".
+//   cell_records   {x, y, z, radius} of the particles in iteration order
+//                  and each particle's position in it.
 //   neigh_count /  NeighAccel::generateNumNeigh / iterateBlocks
 //   neigh_fill     (NeighAccel.h:85-137) over NeighbourGrid::
 //                  iteratePossibleNeighbours (NeighbourGrid.h:85-105): 27
 //                  linear colour offsets visited in z,y,x order, members in
 //                  iterationOrder, self skipped, kept if
-//                  sqrt(d.d) - (r_i + r_j) < interactDist (FP64).  Output is
-//                  a wave-tiled ELL list (see hdm_context.h).
+//                  sqrt(d.d) - (r_i + r_j) < interactDist (FP64), read as 9
+//                  contiguous runs of cell records.  Output is a wave-tiled
+//                  ELL list (see hdm_context.h); the fill pass alone suffices
+//                  when the previous build's width holds every count.
 #include "hdm_context.h"
 #include "kernel_common.h"
 
@@ -198,16 +202,42 @@ __global__ void __launch_bounds__(TPB) k_cell_sort(const int32_t *__restrict__ c
             }
         }
     }
+    // one pair of atomics per workgroup: same-address atomics of every wave
+    // serialise in L2
+    __shared__ uint32_t wocc[TPB / 64], wmx[TPB / 64];
     occ = wave_sum_u32(occ);
     mx = wave_max_u32(mx);
     if ((threadIdx.x & 63) == 0) {
+        wocc[threadIdx.x >> 6] = occ;
+        wmx[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < TPB / 64; w++) {
+            occ += wocc[w];
+            mx = wmx[w] > mx ? wmx[w] : mx;
+        }
         if (occ) atomicAdd((unsigned long long *)&stats[0], (unsigned long long)occ);
         if (mx) atomicMax((unsigned long long *)&stats[1], (unsigned long long)mx);
     }
 }
 
+// Candidate records in iteration order: rec[i] = {x, y, z, radius} of
+// particle order[i], and pos[order[i]] = i, so the list build scans each
+// stencil row's members with one contiguous 32 B load per candidate.
+__global__ void __launch_bounds__(TPB) k_cell_records(const double *__restrict__ x, const double *__restrict__ y,
+                                                       const double *__restrict__ z, const double *__restrict__ radius,
+                                                       const int32_t *__restrict__ order, int64_t n,
+                                                       double4 *__restrict__ rec, int32_t *__restrict__ pos) {
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    const int32_t q = order[i];
+    rec[i] = make_double4(x[q], y[q], z[q], radius[q]);
+    pos[q] = (int32_t)i;
+}
+
 struct NeighArgs {
-    int64_t dirOffsets[27];
+    int64_t rowOffsets[9];  // yD*X + zD*X*Y for (zD, yD) in z, y order
     int64_t totalCells;
     double interactDist;
     int32_t width;  // ELL width (fill pass)
@@ -223,13 +253,23 @@ struct NeighArgs {
 // Largest link-slot width whose partner lists are staged in LDS (32 KiB).
 constexpr int32_t NEIGH_LDS_WIDTH = 32;
 
+// One lane per particle.  NeighbourGrid::iteratePossibleNeighbours visits
+// the 27 cells col + xD + yD*X + zD*X*Y in z, y, x order, skipping targets
+// outside [0, totalCells) and wrapping across rows as the linear offsets do;
+// for each (zD, yD) the three cells xD = -1, 0, 1 are consecutive colours,
+// so their members are one contiguous run of the iteration order
+// [cellStart[lo], cellStart[hi + 1]) with lo..hi the row's targets clipped to
+// the grid: the same candidates in the same order, read as cell records.
+// The distance test is NeighAccel's (FP64, same expressions).  FILL writes
+// the first `width` entries to the wave-tiled list; every pass writes the
+// count and the skipped link partners.
 template <bool FILL>
-__global__ void __launch_bounds__(TPB) k_neigh(const double *__restrict__ x, const double *__restrict__ y,
-                                                const double *__restrict__ z, const double *__restrict__ radius,
+__global__ void __launch_bounds__(TPB) k_neigh(const double4 *__restrict__ crec, const int32_t *__restrict__ cpos,
                                                 int64_t n, const int32_t *__restrict__ colour,
                                                 const int32_t *__restrict__ cellStart,
                                                 const int32_t *__restrict__ order, NeighArgs a,
-                                                int32_t *__restrict__ count, int32_t *__restrict__ list) {
+                                                int32_t *__restrict__ count, int32_t *__restrict__ linked,
+                                                int32_t *__restrict__ list) {
     extern __shared__ int32_t ldsPartners[];  // [linkWidth][TPB]
     int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
     if (p >= n) return;
@@ -250,30 +290,38 @@ __global__ void __launch_bounds__(TPB) k_neigh(const double *__restrict__ x, con
         }
         return false;
     };
-    const double xCord = x[p], yCord = y[p], zCord = z[p], rad = radius[p];
-    const int col = colour[p];
-    int32_t neighbours = 0;
+    const int32_t self = cpos[p];
+    const double4 P = crec[self];
+    const double xCord = P.x, yCord = P.y, zCord = P.z, rad = P.w;
+    const int64_t col = colour[p];
+    int32_t neighbours = 0, skipped = 0;
     int32_t *out = nullptr;
     if (FILL) out = list + (p >> 6) * (int64_t)64 * a.width + (p & 63);
-    for (int nn = 0; nn < 27; ++nn) {
-        int target = (int)((int64_t)col + a.dirOffsets[nn]);
-        if (target < 0 || (int64_t)target >= a.totalCells) continue;
-        int32_t s = cellStart[target], e = cellStart[target + 1];
+    for (int row = 0; row < 9; ++row) {
+        int64_t lo = col + a.rowOffsets[row] - 1, hi = col + a.rowOffsets[row] + 1;
+        lo = lo < 0 ? 0 : lo;
+        hi = hi >= a.totalCells ? a.totalCells - 1 : hi;
+        if (lo > hi) continue;
+        const int32_t s = cellStart[lo], e = cellStart[hi + 1];
         for (int32_t i = s; i < e; ++i) {
-            int32_t nInd = order[i];
-            if (nInd == p) continue;
-            double xDist = xCord - x[nInd];
+            if (i == self) continue;
+            const double4 Q = crec[i];
+            double xDist = xCord - Q.x;
             double xDist2 = xDist * xDist;
-            double yDist = yCord - y[nInd];
+            double yDist = yCord - Q.y;
             double yDist2 = yDist * yDist;
-            double zDist = zCord - z[nInd];
+            double zDist = zCord - Q.z;
             double zDist2 = zDist * zDist;
             double particleDistance2 = xDist2 + yDist2 + zDist2;
             double particleDistance = sqrt(particleDistance2);
-            double radSum = rad + radius[nInd];
+            double radSum = rad + Q.w;
             double dist = particleDistance - radSum;
             if (dist < a.interactDist) {
-                if (pcnt && is_partner(nInd)) continue;
+                const int32_t nInd = order[i];
+                if (pcnt && is_partner(nInd)) {
+                    skipped++;
+                    continue;
+                }
                 if (FILL) {
                     if (neighbours < a.width) out[(int64_t)neighbours * 64] = nInd;
                 }
@@ -281,7 +329,8 @@ __global__ void __launch_bounds__(TPB) k_neigh(const double *__restrict__ x, con
             }
         }
     }
-    if (!FILL) count[p] = neighbours;
+    count[p] = neighbours;
+    linked[p] = skipped;
 }
 
 __global__ void __launch_bounds__(TPB) k_max_i32(const int32_t *__restrict__ v, int64_t n, uint64_t *out) {
@@ -357,6 +406,20 @@ hipError_t launch_scatter_sort(Context &c) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(TPB), 0, c.stream, c.red.as<uint64_t>() + 9, 0ull, (int64_t)2);
     hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(c.grid.totalCells)), dim3(TPB), 0, c.stream,
                        c.cellStart.as<int32_t>(), c.grid.totalCells, c.order.as<int32_t>(), c.red.as<uint64_t>() + 9);
+    e = hipGetLastError();
+    return e == hipSuccess ? launch_cell_records(c) : e;
+}
+
+hipError_t launch_cell_records(Context &c) {
+    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
+    hipError_t e = c.cellRec.ensure(sizeof(double4) * (size_t)c.n);
+    if (e == hipSuccess) e = c.cellPos.ensure(sizeof(int32_t) * (size_t)c.n);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cell_records, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
+                       c.y.as<double>(), c.z.as<double>(), c.radius.as<double>(), c.order.as<int32_t>(), c.n,
+                       c.cellRec.as<double4>(), c.cellPos.as<int32_t>());
+    c.grid.recEpoch = c.stateEpoch;
+    c.grid.recTs = c.ts;
     return hipGetLastError();
 }
 
@@ -365,8 +428,7 @@ static NeighArgs neigh_args(Context &c, double interactDist) {
     int i = 0;
     for (int zD = -1; zD < 2; zD++)
         for (int yD = -1; yD < 2; yD++)
-            for (int xD = -1; xD < 2; xD++)
-                a.dirOffsets[i++] = (int64_t)xD + (int64_t)yD * c.grid.X + (int64_t)zD * c.grid.X * (int64_t)c.grid.Y;
+            a.rowOffsets[i++] = (int64_t)yD * c.grid.X + (int64_t)zD * c.grid.X * (int64_t)c.grid.Y;
     a.totalCells = c.grid.totalCells;
     a.interactDist = interactDist;
     a.width = c.maxNeigh;
@@ -383,22 +445,20 @@ static size_t neigh_lds(const NeighArgs &a) {
 }
 
 hipError_t launch_neigh_count(Context &c, double interactDist) {
-    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     NeighArgs a = neigh_args(c, interactDist);
-    hipLaunchKernelGGL(k_neigh<false>, dim3(blocks_for(c.n)), dim3(TPB), neigh_lds(a), c.stream, c.x.as<double>(),
-                       c.y.as<double>(), c.z.as<double>(), c.radius.as<double>(), c.n, c.colour.as<int32_t>(),
+    hipLaunchKernelGGL(k_neigh<false>, dim3(blocks_for(c.n)), dim3(TPB), neigh_lds(a), c.stream,
+                       c.cellRec.as<double4>(), c.cellPos.as<int32_t>(), c.n, c.colour.as<int32_t>(),
                        c.cellStart.as<int32_t>(), c.order.as<int32_t>(), a, c.nbrCount.as<int32_t>(),
-                       (int32_t *)nullptr);
+                       c.nbrLinked.as<int32_t>(), (int32_t *)nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_neigh_fill(Context &c, double interactDist) {
-    if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     NeighArgs a = neigh_args(c, interactDist);
-    hipLaunchKernelGGL(k_neigh<true>, dim3(blocks_for(c.n)), dim3(TPB), neigh_lds(a), c.stream, c.x.as<double>(),
-                       c.y.as<double>(), c.z.as<double>(), c.radius.as<double>(), c.n, c.colour.as<int32_t>(),
-                       c.cellStart.as<int32_t>(), c.order.as<int32_t>(), a, (int32_t *)nullptr,
-                       c.nbr.as<int32_t>());
+    hipLaunchKernelGGL(k_neigh<true>, dim3(blocks_for(c.n)), dim3(TPB), neigh_lds(a), c.stream,
+                       c.cellRec.as<double4>(), c.cellPos.as<int32_t>(), c.n, c.colour.as<int32_t>(),
+                       c.cellStart.as<int32_t>(), c.order.as<int32_t>(), a, c.nbrCount.as<int32_t>(),
+                       c.nbrLinked.as<int32_t>(), c.nbr.as<int32_t>());
     return hipGetLastError();
 }
 

@@ -218,9 +218,49 @@ __global__ void __launch_bounds__(TPB) k_count_broken(const uint8_t *__restrict_
     if ((threadIdx.x & 63) == 0 && c) atomicAdd((unsigned long long *)out, (unsigned long long)c);
 }
 
+// computeActiveLinks and computeIteratedNeighbours (HiDEMLib.h:197-232): a
+// particle's links with state != 0 (iterateParticleLinksCT over its slots),
+// and the entries of the reference's neighbour list (the list as built plus
+// the link partners it leaves out).  Sums over all particles (all are owned
+// on one GPU) as finalizeTs reduces them (HiDEMLib.h:518-523).
+__global__ void __launch_bounds__(TPB) k_particle_stats(const uint32_t *__restrict__ linkCount,
+                                                         const int32_t *__restrict__ slotLink, int32_t width,
+                                                         const uint8_t *__restrict__ state,
+                                                         const int32_t *__restrict__ nbrCount,
+                                                         const int32_t *__restrict__ nbrLinked, int64_t n,
+                                                         int32_t *__restrict__ activeOut,
+                                                         int32_t *__restrict__ nbrOut, uint64_t *sums) {
+    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    uint32_t act = 0, nb = 0;
+    if (p < n) {
+        const uint32_t cnt = linkCount[p];
+        const int64_t base = (p >> 6) * (int64_t)64 * width + (p & 63);
+        for (uint32_t k = 0; k < cnt; k++) act += state[slotLink[base + (int64_t)k * 64] & SLOT_LINK_MASK] ? 1u : 0u;
+        nb = (uint32_t)nbrCount[p] + (nbrLinked ? (uint32_t)nbrLinked[p] : 0u);
+        if (activeOut) activeOut[p] = (int32_t)act;
+        if (nbrOut) nbrOut[p] = (int32_t)nb;
+    }
+    act = wave_sum_u32(act);
+    nb = wave_sum_u32(nb);
+    if ((threadIdx.x & 63) == 0) {
+        if (act) atomicAdd((unsigned long long *)&sums[0], (unsigned long long)act);
+        if (nb) atomicAdd((unsigned long long *)&sums[1], (unsigned long long)nb);
+    }
+}
+
 inline int blocks_for(int64_t n) { return (int)((n + TPB - 1) / TPB); }
 
 }  // namespace
+
+hipError_t launch_particle_stats(Context &c, int32_t *activeLinks, int32_t *iteratedNeighbours, uint64_t *sums) {
+    hipError_t e = launch_fill_u64(c.stream, sums, 0ull, 2);
+    if (e != hipSuccess || c.n == 0) return e;
+    hipLaunchKernelGGL(k_particle_stats, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.linkCount.as<uint32_t>(),
+                       c.slotLink.as<int32_t>(), c.maxLinks, c.state.as<uint8_t>(), c.nbrCount.as<int32_t>(),
+                       c.nbrExcludesLinked ? c.nbrLinked.as<int32_t>() : (const int32_t *)nullptr, c.n, activeLinks,
+                       iteratedNeighbours, sums);
+    return hipGetLastError();
+}
 
 hipError_t launch_link_count(Context &c) {
     hipError_t e = launch_fill_u32(c.stream, c.linkCount.as<uint32_t>(), 0u, c.n);

@@ -1021,6 +1021,54 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
     if (g.partials) skin_partials<BLOCK>(g, active, p, cordN, diffN);
 }
 
+// The link break that ends a run: linkUpdate of its last step
+// (HiDEMLib.h:256-269, HiDEMDefLinkBreakFunc) applied per particle from the
+// records, as the next fused step would apply it, so the SoA state need not
+// be unpacked first.  Every intact side of p takes the float prefilter and,
+// when that cannot prove the link intact, the exact FP64 test; both ends
+// reach the same verdict (the pair differences are antisymmetric) and flag
+// their own slot, and the end with the smaller index counts the link.  The
+// links broken are those k_break would break.
+template <class R>
+__global__ void __launch_bounds__(TPB) k_break_recs(ForceArgs<R, R, R> f, StepRecs<R> r, uint64_t *counter) {
+    const int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    uint32_t broke = 0;
+    if (p < f.n) {
+        const auto P = r.rec[p];
+        float rotSelf;
+        if constexpr (sizeof(R) == 4) rotSelf = P.rot;
+        else rotSelf = (float)r.drec[p * 8 + 6];
+        const int32_t cnt = (int32_t)f.linkCount[p];
+        const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + (p & 63);
+        for (int32_t k = 0; k < cnt; k++) {
+            const int64_t sidx = base + (int64_t)k * 64;
+            const int32_t ow = ld_stream(f.slotOther + sidx);
+            if (ow < 0) continue;  // broken already
+            const int32_t q = ow & SLOT_LINK_MASK;
+            const auto Q = r.rec[q];
+            float rotOther;
+            if constexpr (sizeof(R) == 4) rotOther = Q.rot;
+            else rotOther = (float)r.drec[(int64_t)q * 8 + 6];
+            const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+            const float thr = (float)ld_stream(r.slotThr + sidx) * r.thrScale;
+            const float T = r.perLinkMaxLoad ? thr : thr * (1.0f + r.scalarMaxLoad);
+            if (r.prefilter != 0 && certainly_intact(T, r.bendBreak, r.l0Max, dx, dy, dz, rotSelf + rotOther))
+                continue;
+            const int32_t word = f.slotLink[sidx];
+            const double theta[3] = {f.tx[p], f.ty[p], f.tz[p]};
+            if (link_breaks(f, word, q, dx, dy, dz, theta)) {
+                f.slotLinkW[sidx] = word | SLOT_BROKEN;
+                f.slotOtherW[sidx] = ow | SLOT_BROKEN;
+                f.state[word] = 0;
+                broke += p < q ? 1u : 0u;
+            }
+        }
+    }
+    broke = wave_sum_u32(broke);
+    if ((threadIdx.x & 63) == 0 && broke && counter)
+        atomicAdd((unsigned long long *)counter, (unsigned long long)broke);
+}
+
 // SoA coordinates and Diff from the records (sync_state).
 template <class Rec>
 __global__ void __launch_bounds__(TPB) k_unpack_records(const Rec *__restrict__ rec,
@@ -1470,6 +1518,27 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
     c.soaStale = true;
     if (!skin) return hipSuccess;
     return skin_finalize(c, skin, guard, shape.block);
+}
+
+hipError_t launch_break_records(Context &c, const hdm_break_params &bp, uint64_t *counter) {
+    if (counter) {
+        hipError_t e = launch_fill_u64(c.stream, counter, 0ull, 1);
+        if (e != hipSuccess) return e;
+    }
+    if (c.L == 0 || c.n == 0) return hipSuccess;
+    if (!c.recCurrent) return launch_break(c, bp, counter);
+    hdm_force_params fp{};
+    auto go = [&](auto f, auto r) {
+        using A = decltype(f);
+        using R = typename std::conditional<std::is_same<A, ForceArgs<double, double, double>>::value, double,
+                                            float>::type;
+        f.bendBreak = bp.bendBreak;
+        f.scalarMaxLoad = bp.maxLoad;
+        hipLaunchKernelGGL((k_break_recs<R>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, f, r, counter);
+    };
+    if (c.prec == HDM_PREC_FP64) go(force_args<double, double, double>(c, fp), step_recs<double>(c, c.recTame, &bp));
+    else go(force_args<float, float, float>(c, fp), step_recs<float>(c, c.recTame, &bp));
+    return hipGetLastError();
 }
 
 const char *step_variant_name(int v) { return (v >= 0 && v < STEP_VARIANTS) ? kShapes[v].name : "?"; }

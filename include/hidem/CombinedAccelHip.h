@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "hdm_abi.h"
@@ -152,7 +153,45 @@ struct NeighbourGrid {
         iterationOrder.resize((size_t)nParticles);
         hdm_check(ctx, hdm_grid_download(ctx, colours.data(), iterationOrder.data()), "hdm_grid_download");
     }
+    // NeighbourGrid::toVTKHDF (a debugging dump of the helper structures,
+    // HiDEMLib.h:605-607 with output.writeHelperStructures) is not part of
+    // this accelerator: asking for it ends the run the reference's way.
+    void toVTKHDF(const std::string &path) const {
+        std::fprintf(stderr, "NeighbourGrid::toVTKHDF(%s): helper-structure output is not supported on this "
+                             "accelerator (unset output.writeHelperStructures)\n", path.c_str());
+        std::exit(1);
+    }
 };
+
+// ---- the link accelerator's members HiDEMLib touches ---------------------------
+// LinkAccelSide (PLiB/Accel/LinkAccelSide.h): keepInactive (HiDEMLib.h:814)
+// and sortSideData (HiDEMLib.h:621).  Broken links always stay in the lists
+// here (their CT keeps integrating, as the reference's does), so
+// keepInactive changes nothing.  The link set does not change between
+// uploads on this path, so the lists and the side permutation built once
+// stay valid across neighbour rebuilds: `current` says they are, and the
+// reference-shaped calls that the reference makes at every rebuild
+// (generateLinks + sortSideData, HiDEMLib.h:620-621) build only when not.
+struct LinkAccelHip {
+    hdm_ctx *ctx = nullptr;
+    bool keepInactive = false;
+    bool current = false;
+    // LinkAccelSide::sortSideData (LinkAccelSide.h:159-417): the CT side
+    // store is permuted when the link lists are built (hdm_links_build).
+    void sortSideData() {
+        hdm_check(ctx, hdm_links_build(ctx), "hdm_links_build");
+        current = true;
+    }
+    template <class PData, class LData>
+    void sortSideData(PData &, LData &) {
+        if (!current) sortSideData();
+    }
+};
+
+namespace detail {
+template <class F, class Want>
+constexpr bool is_default_functor = std::is_same<typename std::decay<F>::type, Want>::value;
+}
 
 // ---- the accelerator ------------------------------------------------------------
 class CombinedAccelHip {
@@ -160,6 +199,7 @@ public:
     explicit CombinedAccelHip(int32_t precision = HDM_PREC_FP64, int32_t device = 0) {
         hdm_check(nullptr, hdm_ctx_create(precision, device, &ctx), "hdm_ctx_create");
         grid.ctx = ctx;
+        lAccel.ctx = ctx;
     }
     ~CombinedAccelHip() { hdm_ctx_destroy(ctx); }
     CombinedAccelHip(const CombinedAccelHip &) = delete;
@@ -167,11 +207,14 @@ public:
 
     hdm_ctx *ctx = nullptr;
     NeighbourGrid grid;
-    bool keepInactive = false;   // LinkAccelSide::keepInactive: broken links stay in the lists (always)
+    LinkAccelHip lAccel;
 
     // Binding of the data handlers' fields (ParticleDataHandler / LinkDataHandler).
     void uploadParticles(const hdm_particles_host &p) { hdm_check(ctx, hdm_particles_upload(ctx, &p), "hdm_particles_upload"); }
-    void uploadLinks(const hdm_links_host &l) { hdm_check(ctx, hdm_links_upload(ctx, &l), "hdm_links_upload"); }
+    void uploadLinks(const hdm_links_host &l) {
+        hdm_check(ctx, hdm_links_upload(ctx, &l), "hdm_links_upload");
+        lAccel.current = false;
+    }
     void downloadParticles(hdm_particles_host &p) { hdm_check(ctx, hdm_particles_download(ctx, &p), "hdm_particles_download"); }
     void downloadLinks(hdm_links_host &l) { hdm_check(ctx, hdm_links_download(ctx, &l), "hdm_links_download"); }
 
@@ -187,8 +230,8 @@ public:
     // CombinedAccel::generateLinks (CombinedAccel.h:43-48) and
     // LinkAccelSide::sortSideData (LinkAccelSide.h:159-417): one call builds
     // the per-particle link lists and permutes the CT side store.
-    void generateLinks() { hdm_check(ctx, hdm_links_build(ctx), "hdm_links_build"); }
-    void sortSideData() { generateLinks(); }
+    void generateLinks() { lAccel.sortSideData(); }
+    void sortSideData() { lAccel.sortSideData(); }
     // CombinedAccelSmem::generateIndexes (CombinedAccelSmem.h:345-524): the
     // wave-tiled lists need no further index generation.
     void generateIndexes() {}
@@ -207,6 +250,48 @@ public:
         int64_t nb = 0;
         hdm_check(ctx, hdm_break(ctx, &b, countBroken ? &nb : nullptr), "hdm_break");
         return nb;
+    }
+
+    // ---- the reference's call expressions -----------------------------------
+    // HiDEMLib calls its accelerator with the data handlers, an execution
+    // policy and the packed ManVec argument tuples (HiDEMLib.h:265, 332,
+    // 347, 603-622, 814-818).  The context holds device copies of every
+    // field those packs name (uploadParticles / uploadLinks bind them once,
+    // downloadParticles / downloadLinks return them), so these overloads
+    // take the reference's argument lists and pass on only what the native
+    // kernels need: the distances and the functors' parameters.  Only the
+    // default functors compile; any other functor type is a compile error.
+    template <class PData, class LData>
+    void generateNeighbours(PData &, LData &, double searchDistance, double interactDist) {
+        generateNeighbours(searchDistance, interactDist);
+    }
+    template <class PData, class LData>
+    void generateLinks(PData &, LData &, double, double) {
+        if (!lAccel.current) generateLinks();
+    }
+    template <class PData, class LData>
+    void generateIndexes(PData &, LData &, double, double) {}
+
+    template <class Policy, class PData, class LData, class NFunc, class LFunc, class Pack>
+    void iterateCombinedPre(Policy, PData &, LData &, const NFunc &nFunc, const LFunc &lFunc, const Pack &) {
+        static_assert(detail::is_default_functor<NFunc, HiDEMDefNeighForceFunc>,
+                      "CombinedAccelHip runs the default neighbour force functor only (HiDEMDefNeighForceFunc)");
+        static_assert(detail::is_default_functor<LFunc, HiDEMDefLinkForceFunc>,
+                      "CombinedAccelHip runs the default link force functor only (HiDEMDefLinkForceFunc)");
+        iterateCombinedPre(nFunc, lFunc);
+    }
+    template <class Policy, class PData, class LData, class NFunc, class LFunc, class Pack>
+    void iterateCombinedPost(Policy, PData &, LData &, const NFunc &nFunc, const LFunc &lFunc, const Pack &) {
+        static_assert(detail::is_default_functor<NFunc, HiDEMDefNeighForceFunc> &&
+                          detail::is_default_functor<LFunc, HiDEMDefLinkForceFunc>,
+                      "CombinedAccelHip runs the default force functors only");
+        iterateCombinedPost(nFunc, lFunc);
+    }
+    template <class Policy, class PData, class LData, class BFunc, class Pack>
+    void iterateLinks(Policy, PData &, LData &, const BFunc &bFunc, const Pack &) {
+        static_assert(detail::is_default_functor<BFunc, HiDEMDefLinkBreakFunc>,
+                      "CombinedAccelHip runs the default link break functor only (HiDEMDefLinkBreakFunc)");
+        iterateLinks(bFunc);
     }
 
     // applyForce (HiDEMLib.h:303-314) with a native integrator.

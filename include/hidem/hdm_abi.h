@@ -217,6 +217,7 @@ int  hdm_skin_heuristic(hdm_ctx *ctx, int32_t stepSize, double *dist);
 int  hdm_skin_snapshot(hdm_ctx *ctx);
 int  hdm_skin_exact(hdm_ctx *ctx, double *maxDisp);
 
+
 /* ---- the timestep loop: nSteps of gridService + computePartForce +
  *      applyForce + linkUpdate (when ts*dt > breakTime) + finalizeTs
  *      (MainIceBlock.cpp:409-447).  rebuildLog (nSteps bytes, may be NULL)
@@ -224,6 +225,15 @@ int  hdm_skin_exact(hdm_ctx *ctx, double *maxDisp);
 int  hdm_run(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t *rebuildLog, hdm_run_stats *stats);
 int  hdm_set_timestep(hdm_ctx *ctx, int64_t ts);
 int  hdm_count_broken(hdm_ctx *ctx, int64_t *broken);
+/* finalizeTs statistics: computeActiveLinks and computeIteratedNeighbours
+ * (HiDEMLib.h:197-232) and their sums over the owned particles
+ * (HiDEMLib.h:518-523).  activeLinks[n]: the particle's links with
+ * state != 0; iteratedNeighbours[n]: the entries of its neighbour list as the
+ * reference builds it (NeighAccel, link partners within the skin included
+ * also where the FP32 form's list leaves them out).  Host arrays; any
+ * argument may be NULL.  Needs a built neighbour list and link lists. */
+int  hdm_particle_stats(hdm_ctx *ctx, int32_t *activeLinks, int32_t *iteratedNeighbours, int64_t *activeLinksSum,
+                        int64_t *iteratedNeighboursSum);
 
 /* ---- learned rebuild scheduler (no reference counterpart; it schedules the
  *      gridService rebuild check, HiDEMLib.h:544-559) -------------------------

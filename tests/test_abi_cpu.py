@@ -3,6 +3,7 @@ header declares, rejects use without a gfx950 device, and its (host-side)
 seeded generator produces the documented lattice and links."""
 import ctypes as C
 import math
+import os
 import re
 import subprocess
 
@@ -11,6 +12,8 @@ import pytest
 
 from hidem_amd import _abi as A
 from hidem_amd import params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -122,3 +125,31 @@ def test_io_library_exports_every_declared_symbol():
     exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
     missing = [s for s in declared if s not in exported]
     assert not missing, f"declared in hdm_io.h but not exported: {missing}"
+
+
+CALLSITES = os.path.join(ROOT, "tests", "adapter_callsites.cpp")
+
+
+def build_callsites(out, *extra):
+    cmd = ["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), CALLSITES, *extra]
+    if out is None:
+        cmd.append("-fsyntax-only")
+    else:
+        cmd += ["-o", str(out), "-L", os.path.dirname(A.LIB_PATH), "-lhidem_amd",
+                "-Wl,-rpath," + os.path.dirname(A.LIB_PATH)]
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def test_adapter_takes_the_reference_call_expressions(tmp_path):
+    """HiDEMLib.h's accelerator calls (generateNeighbours(pData, lData, s, i),
+    iterateCombinedPre(policy, pData, lData, nF, lF, Pack4), iterateLinks(
+    policy, pData, lData, bF, Pack3), lAccel.sortSideData, grid.toVTKHDF ...)
+    compile against CombinedAccelHip and link against the library."""
+    r = build_callsites(tmp_path / "callsites")
+    assert r.returncode == 0, r.stderr
+
+
+def test_adapter_rejects_other_functors():
+    r = build_callsites(None, "-DHDM_WRONG_FUNCTOR")
+    assert r.returncode != 0
+    assert "runs the default link force functor only" in r.stderr

@@ -78,3 +78,50 @@ def test_batch_graph_autotuned_matches_eager(batch):
     for k in range(6):
         assert bits_equal(kl.ct[k], ke.ct[k])
     assert info["tuned"] and info["graph_replays"] > 0
+
+
+def test_batch_at_config_sizes_against_fp64_oracle(orc):
+    """Blocks of the batched config's sizes (log-uniform in [5k, 200k], four
+    blocks, each with its own threshold mean and waterline), 300 steps against
+    the FP64 oracle of the same batch: the FP64 form bit-identical, and the
+    FP32 form (graph-replayed, autotuned, device guard) within position
+    max|dx| / max|x| <= 1e-5 and broken-link count within 0.1% (or one link)."""
+    b = A.gen_batch(n_blocks=4, min_particles=5000, max_particles=200000, seed=5)
+    s = params.derive(params.load_config("iceblock_16k"))
+    counts = np.bincount(b.block_id)
+    say("batch at config sizes", counts.tolist(), b.links.n)
+    assert counts.min() >= 5000 and counts.max() <= 200000
+    steps = 300
+    sim = orc.OracleSim(b.particles, b.links, s, block_id=b.block_id, block_water=b.water)
+    sim.run(steps)
+    x64 = np.stack([sim.p["xCord"], sim.p["yCord"], sim.p["zCord"]], 1)
+    b64 = int((sim.l.state == 0).sum())
+    with A.Context(A.HDM_PREC_FP64) as c:
+        c.upload(b.particles, b.links)
+        A.set_blocks(c, b.block_id, b.water)
+        c.run(s.run, steps)
+        q = c.download_particles()
+        ql = c.download_links()
+    for f in ("xCord", "yCord", "zCord", "xTheta", "yTheta", "zTheta"):
+        assert bits_equal(q[f], sim.p[f]), f
+    assert np.array_equal(ql.state, sim.l.state)
+    rp = A.HdmRunParams.from_buffer_copy(s.run)
+    rp.policy = A.HDM_REBUILD_LEARNED
+    with A.Context(A.HDM_PREC_MIXED) as c:
+        c.upload(b.particles, b.links)
+        A.set_blocks(c, b.block_id, b.water)
+        c.sched_configure(min_batch=16, max_batch=64)
+        c.set_exec(autotune=True, graphs=True)
+        c.run(rp, steps)
+        q = c.download_particles()
+        b32 = c.count_broken()
+        info = c.exec_info()
+    x32 = np.stack([q["xCord"], q["yCord"], q["zCord"]], 1)
+    pos = float(np.linalg.norm(x32 - x64, axis=1).max() / np.abs(x64).max())
+    disp = float(np.abs(x64 - np.stack([b.particles[f] for f in ("xCord", "yCord", "zCord")], 1)).max())
+    say(f"  FP32 batched vs FP64 oracle: position {pos:.3e} (max displacement {disp:.3e} m), "
+        f"broken {b32} / {b64}, graph replays {info['graph_replays']}")
+    sim.close()
+    assert info["tuned"] and info["graph_replays"] > 0
+    assert pos <= 1e-5
+    assert abs(b32 - b64) <= max(1, 1e-3 * b64)

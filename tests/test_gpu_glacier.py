@@ -1,0 +1,208 @@
+"""The glacier slab workload (hidem_amd/configs/glacier_8m.toml: 400 x 100 x
+50 FCC unit cells, 8,000,000 particles, 47,481,100 links) on the GPU.
+
+* Full-size structure and first step of the plain FP64 method: the grid
+  (dimensions, cell of every particle, iteration order), the neighbour list
+  and the per-particle link lists of all 8M particles bit-exact against the
+  oracle, then one FP64 timestep bit-identical (positions, rotations, Diff,
+  forces, link states, CT).
+* The FP32 form on a slice of the slab against the FP64 oracle: position,
+  force and broken-link-count error printed at steps 100 to 2,000 and
+  bounded there, beside the FP64 run started one ulp away.
+* The learned rebuild scheduler (the slab's trained network, captured
+  graphs, autotuned launch shape) against the exact test on the full slab in
+  the FP32 form, 300 steps with a skin thin enough for several rebuilds: the
+  same rebuild steps and the same bits.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from hidem_amd import _abi as A
+from hidem_amd import params
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def bits_equal(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.fixture(scope="module")
+def slab():
+    say("generating the 8M slab")
+    return params.build_inputs(params.load_config("glacier_8m"))
+
+
+def test_slab_structure_and_first_fp64_step(slab, orc):
+    s, p, l = slab
+    assert p.n == 8_000_000 and l.n == 47_481_100
+    say("slab: grid, neighbour list, link lists on the GPU")
+    with A.Context(A.HDM_PREC_FP64) as ctx:
+        ctx.upload(p, l)
+        ctx.grid_build(s.search_distance)
+        info = ctx.grid_info()
+        colours, order = ctx.grid_download()
+        ctx.neighbours_build(s.interact_dist)
+        counts, starts, lst = ctx.neighbours_download()
+        ctx.links_build()
+        lcounts, lstarts, lid, oth, end = ctx.linklists_download()
+    say("slab: oracle grid")
+    _, g = orc.grid_build(p, s.search_distance)
+    assert (info.X, info.Y, info.Z) == g["dims"] and info.totalCells == g["totalCells"]
+    assert info.totalCells > 3_000_000
+    assert np.array_equal(colours, g["colours"])
+    assert np.array_equal(order, g["order"])
+    del colours, order, g
+    say("slab: oracle neighbour list")
+    oc, ost, ol = orc.neigh_build(p, s.search_distance, s.interact_dist)
+    assert np.array_equal(counts, oc) and np.array_equal(starts, ost) and np.array_equal(lst, ol)
+    assert ol.size > 12 * p.n * 0.9
+    del counts, starts, lst, oc, ost, ol
+    say("slab: oracle link lists")
+    occ, ost, olid, ooth, oside = orc.link_lists(p.n, l)
+    assert np.array_equal(lcounts, occ) and np.array_equal(lstarts, ost)
+    assert np.array_equal(lid, olid) and np.array_equal(oth, ooth) and np.array_equal(end, oside & 1)
+    del lcounts, lstarts, lid, oth, end, occ, ost, olid, ooth, oside
+
+    say("slab: one FP64 step")
+    rp = A.HdmRunParams.from_buffer_copy(s.run)
+    with A.Context(A.HDM_PREC_FP64) as ctx:
+        ctx.upload(p, l)
+        st = ctx.run(rp, 1)
+        gp = ctx.download_particles()
+        gl = ctx.download_links()
+    assert st.rebuilds == 1
+    sim = orc.OracleSim(p, l, s)
+    sim.run(1)
+    for f in A.PARTICLE_FIELDS:
+        assert bits_equal(gp[f], sim.p[f]), f
+    assert np.array_equal(gl.state, sim.l.state)
+    for k in range(6):
+        assert bits_equal(gl.ct[k], sim.l.ct[k]), f"CT{k}"
+    moved = np.abs(gp["zCord"] - p["zCord"]).max()
+    say(f"slab: one step bit-identical, max |dz| = {moved:.3e} m")
+    assert moved > 0
+    sim.close()
+
+
+def slice_inputs():
+    cfg = params.load_config("glacier_8m")
+    cfg["hdm"].update(bedXEnd=40.0, frictionXEnd=40.0)
+    return params.build_inputs(cfg, [60, 10, 12])
+
+
+def test_mixed_slice_error_against_fp64_oracle(orc):
+    """The FP32 form (FP32 contact and beam arithmetic and CT, FP64
+    integration and strain) of a 28,800-particle slice of the slab, 20 m of
+    it over deep water, against the FP64 oracle at fixed steps.
+
+    Up to step 500 (the slice has broken 5% of its links by then) the FP32
+    form tracks FP64 closely: position max |dx| / max |x| <= 1e-9, force
+    rel. L2 <= 1e-5, the same broken-link count.  Later, links whose strain
+    sits within the FP32 form's position error (~1e-9 relative) of their
+    threshold break one way or the other, and each such event moves the
+    trajectory by its own amount: from step 1,000 on the bound is 1e-3 on
+    position and 1% on the broken-link count.  The FP64 runs started one ulp
+    apart (printed beside) stay within 1e-13: FP64 is not chaotic here; the
+    divergence comes from those threshold decisions."""
+    s, p, l = slice_inputs()
+    say("slice", p.n, "particles,", l.n, "links")
+    sim = orc.OracleSim(p, l, s)
+    q1 = p.copy()
+    for f in ("xCord", "yCord", "zCord"):
+        q1.arrays[f] = np.nextafter(q1.arrays[f], np.inf)
+    done = 0
+    rows = []
+    with A.Context(A.HDM_PREC_MIXED) as ctx, A.Context(A.HDM_PREC_FP64) as ulp:
+        ctx.upload(p, l)
+        ulp.upload(q1, l)
+        for step in (100, 300, 500, 1000, 2000):
+            ctx.run(s.run, step - done)
+            ulp.run(s.run, step - done)
+            sim.run(step - done)
+            done = step
+            x64 = np.stack([sim.p["xCord"], sim.p["yCord"], sim.p["zCord"]], 1)
+            f64 = np.stack([sim.p[f] for f in ("xForce", "yForce", "zForce")], 1)
+            b64 = int((sim.l.state == 0).sum())
+            err = {}
+            for name, c in (("fp32", ctx), ("fp64+1ulp", ulp)):
+                q = c.download_particles()
+                x = np.stack([q["xCord"], q["yCord"], q["zCord"]], 1)
+                f = np.stack([q[k] for k in ("xForce", "yForce", "zForce")], 1)
+                err[name] = (float(np.linalg.norm(x - x64, axis=1).max() / np.abs(x64).max()),
+                             float(np.linalg.norm(f - f64) / np.linalg.norm(f64)), c.count_broken())
+            rows.append((step, b64, err["fp32"]))
+            say(f"  step {step}: broken fp64 {b64}; fp32: position {err['fp32'][0]:.3e}, force {err['fp32'][1]:.3e}, "
+                f"broken {err['fp32'][2]}; fp64 one ulp apart: position {err['fp64+1ulp'][0]:.3e}, "
+                f"force {err['fp64+1ulp'][1]:.3e}, broken {err['fp64+1ulp'][2]}")
+    sim.close()
+    for step, b64, (pos, frel, b32) in rows:
+        if step <= 500:
+            assert pos <= 1e-9 and frel <= 1e-5 and b32 == b64, (step, pos, frel, b32, b64)
+        else:
+            assert pos <= 1e-3 and abs(b32 - b64) <= 0.01 * b64, (step, pos, b32, b64)
+    assert rows[2][1] > 0.03 * l.n, "the slice must fracture within the window"
+
+
+def test_learned_scheduler_matches_exact_on_full_slab(slab):
+    from hidem_amd import sched_train
+    s, p, l = slab
+    model = sched_train.load_model(os.path.join(ROOT, "hidem_amd", "models", "sched_glacier_8m.bin"))
+    steps = 300
+    rp = A.HdmRunParams.from_buffer_copy(s.run)
+    rp.policy = A.HDM_REBUILD_EXACT
+    # skin from the slab's own motion: a third of the largest displacement
+    # over the window, so the exact test trips several times
+    with A.Context(A.HDM_PREC_MIXED) as ctx:
+        ctx.upload(p, l)
+        ctx.run(rp, steps)
+        ctx.sync()
+        q = ctx.download_particles()
+    disp = float(np.sqrt((q["xCord"] - p["xCord"]) ** 2 + (q["yCord"] - p["yCord"]) ** 2 +
+                         (q["zCord"] - p["zCord"]) ** 2).max())
+    del q
+    rp.interactDist = rp.force.attractDistance + 2.0 * disp / 3.0
+    say(f"slab: max displacement over {steps} steps {disp:.3e} m, skin {rp.interactDist:.4e} m")
+
+    def run(policy, learned):
+        r = A.HdmRunParams.from_buffer_copy(rp)
+        r.policy = policy
+        with A.Context(A.HDM_PREC_MIXED) as ctx:
+            ctx.upload(p, l)
+            ctx.links_build()
+            if learned:
+                ctx.sched_configure(block_cells=4, min_batch=8, max_batch=256, safety=0.5)
+                ctx.sched_set_model(model)
+                ctx.set_exec(autotune=True, graphs=True)
+            logs = []
+            for k in (150, 150):
+                _, log = ctx.run(r, k, log_rebuilds=True)
+                logs.append(log)
+            info = ctx.exec_info()
+            sched = ctx.sched_stats()
+            return np.concatenate(logs), ctx.download_particles(), ctx.download_links(), info, sched
+
+    say("slab: exact test, 300 steps")
+    log_e, pe, le, _, _ = run(A.HDM_REBUILD_EXACT, False)
+    say("slab: learned scheduler, 300 steps")
+    log_l, pl, ll, info, sched = run(A.HDM_REBUILD_LEARNED, True)
+    say(f"  rebuild steps {np.nonzero(log_e)[0].tolist()}, learned {np.nonzero(log_l)[0].tolist()}, "
+        f"sched {sched}, shape {info['name']}, captures {info['graph_captures']}, replays {info['graph_replays']}")
+    assert log_e.sum() >= 3   # the first step's build and at least two trips
+    assert np.array_equal(log_l, log_e)
+    for f in A.PARTICLE_FIELDS:
+        assert bits_equal(pl[f], pe[f]), f
+    assert np.array_equal(ll.state, le.state)
+    for k in range(6):
+        assert bits_equal(ll.ct[k], le.ct[k]), f"CT{k}"
+    assert info["tuned"] and info["graph_replays"] > 0
+    assert sched["steps_executed"] == steps and sched["host_checks"] < steps

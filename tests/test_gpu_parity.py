@@ -137,6 +137,37 @@ def test_linklists_bit_exact(case, request, orc):
     assert np.array_equal(end, oside & 1)
 
 
+@pytest.mark.parametrize("prec", [A.HDM_PREC_FP64, A.HDM_PREC_MIXED])
+@pytest.mark.parametrize("case", ["block", "squeezed"])
+def test_particle_stats(case, prec, request, orc):
+    """finalizeTs statistics (computeActiveLinks, computeIteratedNeighbours,
+    HiDEMLib.h:197-232): each particle's intact links and its entries in the
+    reference's neighbour list, also in the FP32 form, whose list leaves the
+    link partners out."""
+    s, p, l = request.getfixturevalue(case)
+    l = l.copy()
+    rng = np.random.default_rng(5)
+    l.state[rng.random(l.n) < 0.1] = 0
+    say("stats", case, prec)
+    with new_ctx(p, l, prec) as ctx:
+        ctx.links_build()
+        ctx.grid_build(s.search_distance)
+        ctx.neighbours_build(s.interact_dist)
+        merged = ctx.exec_info()["merged_contacts"]
+        counts = ctx.neighbours_download()[0]
+        act, nbr, sa, sn = ctx.particle_stats()
+    oc, _, _ = orc.neigh_build(p, s.search_distance, s.interact_dist)
+    intact = l.state != 0
+    want_act = (np.bincount(l.a[intact], minlength=p.n) + np.bincount(l.b[intact], minlength=p.n)).astype(np.int32)
+    assert np.array_equal(act, want_act)
+    assert np.array_equal(nbr, oc)
+    assert sa == int(want_act.sum()) == 2 * int(intact.sum())
+    assert sn == int(oc.sum())
+    assert merged == (prec == A.HDM_PREC_MIXED)
+    if merged:   # the list the FP32 form iterates is shorter by the partners within the skin
+        assert counts.sum() < oc.sum()
+
+
 def test_ct_roundtrip_through_side_layout():
     s, p, l = cfg_inputs("iceblock_16k", cells=[6, 5, 4])
     rng = np.random.default_rng(2)
