@@ -234,6 +234,7 @@ def _declare(L):
         "hdm_skin_heuristic": ([P, C.c_int32, _dp], C.c_int),
         "hdm_skin_snapshot": ([P], C.c_int),
         "hdm_skin_exact": ([P, _dp], C.c_int),
+        "hdm_skin_heuristic_noalloc": ([P, P, P, C.c_int64, C.c_int32, P, _dp], C.c_int),
         "hdm_run": ([P, C.POINTER(HdmRunParams), C.c_int64, _u8p, C.POINTER(HdmRunStats)], C.c_int),
         "hdm_bw_probe": ([P, C.c_int32, C.c_int64, C.c_int32, _dp], C.c_int),
         "hdm_set_timestep": ([P, C.c_int64], C.c_int),

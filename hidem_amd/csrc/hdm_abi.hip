@@ -24,6 +24,8 @@ struct hdm_ctx {
 
 namespace {
 
+thread_local std::string g_free_error;
+
 int fail(hdm_ctx *ctx, int code, const std::string &msg) {
     if (ctx) ctx->c.err = msg;
     return code;
@@ -160,6 +162,7 @@ bool valid(const hdm_ctx *ctx) { return ctx != nullptr; }
 
 namespace hdm {
 int xcd_grid(int64_t nBlocks) { return (int)nBlocks; }
+void set_free_error(const std::string &msg) { g_free_error = msg; }
 }  // namespace hdm
 
 extern "C" {
@@ -218,7 +221,10 @@ int hdm_ctx_destroy(hdm_ctx *ctx) {
     return HDM_OK;
 }
 
-const char *hdm_last_error(const hdm_ctx *ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+const char *hdm_last_error(const hdm_ctx *ctx) {
+    if (ctx) return ctx->c.err.c_str();
+    return g_free_error.empty() ? "null context" : g_free_error.c_str();
+}
 
 int hdm_sync(hdm_ctx *ctx) {
     if (!valid(ctx)) return HDM_E_INVALID;

@@ -241,6 +241,10 @@ constexpr int32_t SLOT_LINK_MASK = 0x7fffffff;
 // Launch helpers shared by the kernel files.
 int xcd_grid(int64_t nBlocks);
 
+// Message of the last failed call on this thread that has no context (the
+// allocation-free entry point); hdm_last_error(NULL) returns it.
+void set_free_error(const std::string &msg);
+
 // Block grid of the learned scheduler.
 struct SchedGeometry {
     double ox, oy, oz, edge;

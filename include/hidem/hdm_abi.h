@@ -217,6 +217,20 @@ int  hdm_skin_heuristic(hdm_ctx *ctx, int32_t stepSize, double *dist);
 int  hdm_skin_snapshot(hdm_ctx *ctx);
 int  hdm_skin_exact(hdm_ctx *ctx, double *maxDisp);
 
+/* Allocation-free variant of hdm_skin_heuristic.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It takes no context: xDiff, yDiff, zDiff are n doubles in device memory of
+ * the current GPU (e.g. HiDEMParticleData's Diff buffers), the work is
+ * ordered on `stream` (a hipStream_t; NULL = the null stream) and *dist (host)
+ * receives stepSize * |max+ Diff| after the call synchronises that stream.
+ * Its partial maxima live in a fixed device array of 393,240 B that each GPU
+ * holds until the process ends; the call allocates nothing.  n beyond
+ * 134,217,728 gives HDM_E_CAPACITY.  Calls that overlap on one GPU share
+ * that array and race.  hdm_last_error(NULL) returns the message of this
+ * call's last failure on the calling thread. */
+int  hdm_skin_heuristic_noalloc(const double *xDiff, const double *yDiff, const double *zDiff, int64_t n,
+                                int32_t stepSize, void *stream, double *dist);
 
 /* ---- the timestep loop: nSteps of gridService + computePartForce +
  *      applyForce + linkUpdate (when ts*dt > breakTime) + finalizeTs
