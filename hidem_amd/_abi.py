@@ -125,13 +125,13 @@ class HdmMlpWeights(C.Structure):
 
 class HdmExecParams(C.Structure):
     _fields_ = [("autotune", C.c_int32), ("graphs", C.c_int32), ("variant", C.c_int32),
-                ("listWithPartners", C.c_int32)]
+                ("listWithPartners", C.c_int32), ("persistent", C.c_int32)]
 
 
 class HdmExecInfo(C.Structure):
     _fields_ = [("variant", C.c_int32), ("variants", C.c_int32), ("tuned", C.c_int32), ("name", C.c_char * 32),
                 ("tuneMs", C.c_double * 8), ("graphCaptures", C.c_int64), ("graphReplays", C.c_int64),
-                ("mergedContacts", C.c_int32)]
+                ("mergedContacts", C.c_int32), ("persistentLaunches", C.c_int64)]
 
 
 class HdmSchedParams(C.Structure):
@@ -644,8 +644,8 @@ class Context:
             arr[k] = C.pointer(kv[0]) if kv is not None else C.POINTER(HdmMap2d)()
         self._c(lib().hdm_set_maps(self._h, arr), "hdm_set_maps")
 
-    def set_exec(self, autotune=False, graphs=False, variant=-1, list_with_partners=False):
-        ep = HdmExecParams(int(autotune), int(graphs), variant, int(list_with_partners))
+    def set_exec(self, autotune=False, graphs=False, variant=-1, list_with_partners=False, persistent=False):
+        ep = HdmExecParams(int(autotune), int(graphs), variant, int(list_with_partners), int(persistent))
         self._c(lib().hdm_set_exec(self._h, C.byref(ep)), "hdm_set_exec")
 
     def exec_info(self):
@@ -653,7 +653,8 @@ class Context:
         self._c(lib().hdm_exec_info_get(self._h, C.byref(i)), "hdm_exec_info_get")
         return {"variant": i.variant, "name": i.name.decode(), "tuned": bool(i.tuned),
                 "tune_ms": [i.tuneMs[v] for v in range(i.variants)], "graph_captures": i.graphCaptures,
-                "graph_replays": i.graphReplays, "merged_contacts": bool(i.mergedContacts)}
+                "graph_replays": i.graphReplays, "merged_contacts": bool(i.mergedContacts),
+                "persistent_launches": i.persistentLaunches}
 
     def sched_stats(self):
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()

@@ -192,6 +192,7 @@ int hdm_ctx_create(int32_t precision, int32_t device, hdm_ctx **out) {
     c.prec = precision;
     c.device = device;
     c.numCU = prop.multiProcessorCount;
+    c.coopLaunch = prop.cooperativeLaunch != 0;
     if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return HDM_E_HIP;
@@ -216,6 +217,7 @@ int hdm_ctx_destroy(hdm_ctx *ctx) {
     }
     for (auto e : c.eventPool) (void)hipEventDestroy(e);
     if (c.graphExec) (void)hipGraphExecDestroy(c.graphExec);
+    if (c.graphExecPair) (void)hipGraphExecDestroy(c.graphExecPair);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     delete ctx;
     return HDM_OK;
@@ -351,12 +353,14 @@ int hdm_links_upload(hdm_ctx *ctx, const hdm_links_host *l) {
     if (!valid(ctx) || !l || l->n < 0) return HDM_E_INVALID;
     Context &c = ctx->c;
     c.stateEpoch++;
+    c.brokenCountValid = false;
     if (l->n > 0 && (!l->localPartIndA || !l->localPartIndB || !l->state || !l->initialLen))
         return fail(ctx, HDM_E_INVALID, "hdm_links_upload: missing required field");
     if (2 * l->n >= (int64_t)INT32_MAX) return fail(ctx, HDM_E_CAPACITY, "more than int32 link sides");
     hipStream_t s = c.stream;
     c.L = l->n;
     size_t L = (size_t)(c.L > 0 ? c.L : 1);
+    HIP_CHECK(ctx, c.brokenCount.ensure(sizeof(uint64_t)));
     HIP_CHECK(ctx, c.la.ensure(sizeof(int32_t) * L));
     HIP_CHECK(ctx, c.lb.ensure(sizeof(int32_t) * L));
     HIP_CHECK(ctx, c.state.ensure(L));
@@ -641,6 +645,7 @@ int hdm_break(hdm_ctx *ctx, const hdm_break_params *bp, int64_t *nBroken) {
     if (!valid(ctx) || !bp) return HDM_E_INVALID;
     Context &c = ctx->c;
     c.stateEpoch++;
+    c.brokenCountValid = false;   // k_break counts this call's breaks only
     if (!c.linksBuilt) return fail(ctx, HDM_E_STATE, "link lists not built (link end indices unchecked)");
     uint64_t *counter = c.red.as<uint64_t>() + 16;
     if (nBroken) HIP_CHECK(ctx, hdm::launch_fill_u64(c.stream, counter, 0ull, 1));
@@ -694,11 +699,17 @@ int hdm_skin_exact(hdm_ctx *ctx, double *maxDisp) {
     return HDM_OK;
 }
 
+// The count every link-breaking kernel keeps (Context::brokenCount); a full
+// count of the states refreshes it after anything else changed them.
 int hdm_count_broken(hdm_ctx *ctx, int64_t *broken) {
     if (!valid(ctx) || !broken) return HDM_E_INVALID;
     Context &c = ctx->c;
-    uint64_t *counter = c.red.as<uint64_t>() + 28;
-    HIP_CHECK(ctx, hdm::launch_count_broken(c, counter));
+    HIP_CHECK(ctx, c.brokenCount.ensure(sizeof(uint64_t)));
+    uint64_t *counter = c.brokenCount.as<uint64_t>();
+    if (!c.brokenCountValid) {
+        HIP_CHECK(ctx, hdm::launch_count_broken(c, counter));
+        c.brokenCountValid = true;
+    }
     uint64_t v = 0;
     HIP_CHECK(ctx, d2h(&v, counter, sizeof(v), c.stream));
     *broken = (int64_t)v;
@@ -1066,6 +1077,9 @@ std::vector<unsigned char> graph_key(const Context &c, const hdm_run_params *rp,
         put(&c.force[q].ptr, sizeof(void *));
     }
     put(&c.ct.ptr, sizeof(void *));
+    put(&c.brokenCount.ptr, sizeof(void *));
+    put(&c.guardCur, sizeof(c.guardCur));
+    put(&c.nbrExcludesLinked, sizeof(c.nbrExcludesLinked));
     for (int q = 0; q < 12; q++) put(&c.alt[q].ptr, sizeof(void *));
     put(&c.n, sizeof(c.n));
     put(&c.L, sizeof(c.L));
@@ -1087,42 +1101,49 @@ std::vector<unsigned char> graph_key(const Context &c, const hdm_run_params *rp,
     return k;
 }
 
-// Two fused steps (each with the previous step's break, no force store)
-// replayed from a captured hipGraph; captured again whenever anything baked
-// into the arguments changed (a rebuild that moved the neighbour list, new
-// parameters, a new launch shape).  The pair swaps the state buffers twice,
-// so the host's view of the current state is unchanged by a replay.
-int graph_pair(hdm_ctx *ctx, const hdm_run_params *rp, uint64_t *skin, const hdm::StepGuard *guard) {
+// G fused steps (each with the previous step's break, no force store)
+// replayed from a captured hipGraph, G = GRAPH_STEPS_LONG or 2 (one cached
+// graph each); captured again whenever anything baked into the arguments
+// changed (new parameters, a new launch shape, a neighbour list that needed
+// a wider ELL).  G is even, so the steps swap the state buffers an even
+// number of times and the host's view of the current state is unchanged by
+// a replay.
+constexpr int GRAPH_STEPS_LONG = 8;
+
+int graph_steps(hdm_ctx *ctx, const hdm_run_params *rp, uint64_t *skin, const hdm::StepGuard *guard, int G) {
     Context &c = ctx->c;
+    hipGraphExec_t &exec = G == 2 ? c.graphExecPair : c.graphExec;
+    std::vector<unsigned char> &cached = G == 2 ? c.graphKeyPair : c.graphKey;
     std::vector<unsigned char> key = graph_key(c, rp, skin);
-    if (!c.graphExec || key != c.graphKey) {
-        if (c.graphExec) (void)hipGraphExecDestroy(c.graphExec);
-        c.graphExec = nullptr;
+    if (!exec || key != cached) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
         hipGraph_t graph = nullptr;
         HIP_CHECK(ctx, hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
         c.capturing = true;
-        hipError_t e1 = hdm::launch_step(c, rp->force, rp->integ, skin, true, &rp->brk, false, guard);
-        hipError_t e2 = e1 == hipSuccess ? hdm::launch_step(c, rp->force, rp->integ, skin, true, &rp->brk, false, guard)
-                                         : e1;
-        if (e1 == hipSuccess && e2 != hipSuccess) {  // keep the host's buffer view consistent with the device
-            hdm::swap_step_state(c);
-        }
+        hipError_t e = hipSuccess;
+        int launched = 0;
+        for (; launched < G && e == hipSuccess; launched++)
+            e = hdm::launch_step(c, rp->force, rp->integ, skin, true, &rp->brk, false, guard);
+        if (e != hipSuccess) launched--;
+        if (launched & 1) hdm::swap_step_state(c);   // keep the host's buffer view as before the capture
         hipError_t e3 = hipStreamEndCapture(c.stream, &graph);
         c.capturing = false;
-        if (e2 != hipSuccess || e3 != hipSuccess) {
+        if (e != hipSuccess || e3 != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
-            return hip_fail(ctx, e2 != hipSuccess ? e2 : e3, "step pair capture");
+            return hip_fail(ctx, e != hipSuccess ? e : e3, "timestep graph capture");
         }
-        hipGraphExec_t exec = nullptr;
         hipError_t e4 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
-        if (e4 != hipSuccess) return hip_fail(ctx, e4, "hipGraphInstantiate");
-        c.graphExec = exec;
-        c.graphKey = std::move(key);
+        if (e4 != hipSuccess) {
+            exec = nullptr;
+            return hip_fail(ctx, e4, "hipGraphInstantiate");
+        }
+        cached = std::move(key);
         c.graphCaptures++;
     }
     Timer t(c, HDM_T_STEP);
-    HIP_CHECK(ctx, hipGraphLaunch(c.graphExec, c.stream));
+    HIP_CHECK(ctx, hipGraphLaunch(exec, c.stream));
     c.graphReplays++;
     // the replayed steps advanced the records, as launch_step notes for its
     // own launches: the SoA coordinates and Diff now lag them
@@ -1142,9 +1163,12 @@ int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t 
     auto t0 = std::chrono::steady_clock::now();
     int64_t rebuilds = 0;
     if (rebuildLog) std::memset(rebuildLog, 0, (size_t)nSteps);
-    // initial decision as HDM_REBUILD_EXACT takes it
+    // initial decision as HDM_REBUILD_EXACT takes it: known from the trip
+    // words of the last learned run when nothing touched the state since
     bool regen = !c.neighBuilt || !c.snapshotValid;
-    if (!regen) {
+    if (!regen && c.tripKnown && c.tripEpoch == c.stateEpoch && c.tripTs == c.ts) {
+        regen = c.tripPending;
+    } else if (!regen) {
         double d = 0;
         if (c.skinFused && c.skinFusedDisp) {
             uint64_t k = 0;
@@ -1185,12 +1209,39 @@ int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t 
         int64_t i = 0;
         while (i < H) {
             const int64_t s = step + i;
-            const bool pairOk = !tuning && c.useGraphs && i + 1 < H && breakPending && s + 1 < nSteps - 1 &&
-                                (double)(c.ts + i) * rp->dt > rp->breakTime && c.recCurrent;
-            if (pairOk) {
-                int r2 = graph_pair(ctx, rp, skin, &guard);
+            // G replayed steps: inside the batch, not the run's last step
+            // (which stores the force), each with the previous step's break
+            auto graphOk = [&](int64_t G) {
+                return !tuning && c.useGraphs && i + G <= H && breakPending && s + G < nSteps &&
+                       (double)(c.ts + i) * rp->dt > rp->breakTime && c.recCurrent;
+            };
+            if (c.usePersistent && !tuning && breakPending && c.recCurrent &&
+                (double)(c.ts + i) * rp->dt > rp->breakTime) {
+                // the rest of the batch in one launch, but not the run's last
+                // step (it stores the force)
+                int64_t S = H - i;
+                if (s + S >= nSteps) S = nSteps - 1 - s;
+                if (S >= 2) {
+                    bool failed = false;
+                    {
+                        Timer t(c, HDM_T_STEP);
+                        HIP_CHECK(ctx, hdm::launch_step_loop(c, rp->force, rp->integ, skin, rp->brk, guard, (int32_t)S,
+                                                             &failed));
+                    }
+                    if (failed)
+                        return fail(ctx, HDM_E_HIP, "persistent step: a grid barrier timed out "
+                                                    "(workgroups not co-resident)");
+                    c.loopLaunches++;
+                    i += S;
+                    breakPending = (double)(c.ts + i - 1) * rp->dt > rp->breakTime;
+                    continue;
+                }
+            }
+            const int64_t G = graphOk(GRAPH_STEPS_LONG) ? GRAPH_STEPS_LONG : graphOk(2) ? 2 : 0;
+            if (G) {
+                int r2 = graph_steps(ctx, rp, skin, &guard, (int)G);
                 if (r2 != HDM_OK) return r2;
-                i += 2;
+                i += G;
                 breakPending = (double)(c.ts + i - 1) * rp->dt > rp->breakTime;
                 continue;
             }
@@ -1240,7 +1291,7 @@ int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t 
                 c.stepVariant = 0;
             }
         }
-        regen = words[hdm::GUARD_FLAG] != 0;
+        regen = (words[hdm::GUARD_FLAG] | words[hdm::GUARD_FLAG + 1]) != 0;
         if ((H - k) & 1) {  // launch_step swapped once per launch; undo the skipped ones
             hdm::swap_step_state(c);
         }
@@ -1254,8 +1305,9 @@ int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t 
         c.schedLaunched += H;
         c.schedExecuted += k;
         step += k;
-        c.skinFused = true;
-        c.skinFusedDisp = true;
+        // the guarded launches keep no maxima for the host-checked policies
+        c.skinFused = false;
+        c.skinFusedDisp = false;
         if (k == 0 && !regen) return fail(ctx, HDM_E_STATE, "guarded batch executed no step");
         if (!regen && k == H && H < Hplanned) {   // the run ended inside the batch
             c.schedCarry = Hplanned - H;
@@ -1263,6 +1315,12 @@ int run_learned(hdm_ctx *ctx, const hdm_run_params *rp, int64_t nSteps, uint8_t 
             c.schedCarryEpoch = c.stateEpoch;
         }
     }
+    // whether the state left trips the exact test (the link break below
+    // does not move particles), for the next learned run's first decision
+    c.tripKnown = true;
+    c.tripPending = regen;
+    c.tripEpoch = c.stateEpoch;
+    c.tripTs = c.ts;
     if (breakPending) {
         Timer t(c, HDM_T_BREAK);
         HIP_CHECK(ctx, hdm::launch_break_records(c, rp->brk, nullptr));
@@ -1545,7 +1603,10 @@ int hdm_set_exec(hdm_ctx *ctx, const hdm_exec_params *ep) {
     if (ep->variant < -1 || ep->variant >= hdm::STEP_VARIANTS)
         return fail(ctx, HDM_E_INVALID, "hdm_exec_params.variant out of range");
     Context &c = ctx->c;
+    if (ep->persistent && !c.coopLaunch)
+        return fail(ctx, HDM_E_INVALID, "hdm_exec_params.persistent: the device takes no cooperative launches");
     c.useGraphs = ep->graphs != 0;
+    c.usePersistent = ep->persistent != 0;
     if (ep->autotune) {
         c.autotune = true;
         c.tuned = false;
@@ -1574,6 +1635,7 @@ int hdm_exec_info_get(hdm_ctx *ctx, hdm_exec_info *info) {
     info->graphCaptures = c.graphCaptures;
     info->graphReplays = c.graphReplays;
     info->mergedContacts = c.nbrExcludesLinked ? 1 : 0;
+    info->persistentLaunches = c.loopLaunches;
     return HDM_OK;
 }
 

@@ -99,6 +99,7 @@ struct Context {
     hipStream_t stream = nullptr;
     std::string err;
     int32_t numCU = 0;
+    bool coopLaunch = false;     // the device takes cooperative (co-resident) launches
 
     // particles
     int64_t n = 0;
@@ -128,6 +129,16 @@ struct Context {
     // sync_state unpacks them.
     DevBuf rec, recAlt, drec, drecAlt, dtail, dtailAlt;
     bool recCurrent = false;     // rec/drec/dtail hold the current state
+    int32_t guardCur = 0;        // trip word of rec (0/1); swaps with the records
+    // Whether the state the last learned run left trips the exact test
+    // (valid while stateEpoch and ts are those it recorded).
+    bool tripKnown = false, tripPending = false;
+    uint64_t tripEpoch = 0;
+    int64_t tripTs = -1;
+    // Broken-link count kept by every kernel that breaks links (brokenCount,
+    // u64 on the device); a full count refreshes it when not valid.
+    DevBuf brokenCount;
+    bool brokenCountValid = false;
     bool soaStale = false;       // x, y, z, diff[] lag the records
     bool altHoldsPrev = false;   // recAlt/drecAlt/dtailAlt hold the state before the last fused step
     // FP32 force: the neighbour list leaves out each particle's link partners
@@ -184,8 +195,12 @@ struct Context {
     bool autotune = false, tuned = false, useGraphs = false, capturing = false;
     double tuneMs[8] = {0};
     int64_t tuneCount[8] = {0};
-    hipGraphExec_t graphExec = nullptr;
-    std::vector<unsigned char> graphKey;
+    bool usePersistent = false;  // hdm_exec_params.persistent
+    int64_t loopLaunches = 0;    // k_step_loop launches
+    DevBuf loopBarrier;          // grid-barrier words of k_step_loop (+1: timeout flag)
+    hipGraphExec_t graphExec = nullptr;      // replays GRAPH_STEPS_LONG steps
+    hipGraphExec_t graphExecPair = nullptr;  // replays 2 steps
+    std::vector<unsigned char> graphKey, graphKeyPair;
     int64_t graphCaptures = 0, graphReplays = 0;
     // learned rebuild scheduler
     hdm_sched_params sched = {4, 8, 256, 0.5f};
@@ -302,15 +317,21 @@ const char *step_variant_name(int v);
 int pick_step_variant(const double ms[STEP_VARIANTS]);
 
 // Guard words in Context::guardState.
-enum { GUARD_TICKET = 0, GUARD_FLAG = 1, GUARD_STEPS = 2, GUARD_WORDS = 4 };
+// GUARD_FLAG + guardCur is the trip word of the current record buffer, the
+// other one that of the alternate buffer (they swap with the records).
+enum { GUARD_TICKET = 0, GUARD_FLAG = 1, GUARD_STEPS = 3, GUARD_WORDS = 4 };
 // u64 slot of Context::red where the skin keys of the last integrating
 // launch start (max+ Diff per axis at SKIN_KEYS + 0..2, max |x - x0|^2 at
 // SKIN_KEYS + 4).
 enum { SKIN_KEYS = 32 };
 
-// Exact rebuild test evaluated on the device by launch_step: the step's last
-// workgroup sets the guard flag when 2 max |x - x0| >= threshold; steps
-// launched after that do nothing until the host clears the flag.
+// Exact rebuild test evaluated on the device by launch_step: every particle
+// whose new position lies 2 |x - x0| >= threshold from its snapshot sets the
+// trip word of the record buffer the launch writes (2 |x - x0| >= threshold
+// for some particle exactly when it holds for the largest |x - x0|); a launch
+// whose input buffer's word is set does nothing but set its output word, so
+// steps launched after a trip do nothing until the host clears the words.
+// Workgroup 0 of each executed launch counts it (GUARD_STEPS).
 struct StepGuard {
     double threshold;   // interactDist - attractDistance
 };
@@ -326,6 +347,13 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
                        const StepGuard *guard = nullptr);
 // Clears the guard flag and the executed-step counter (GUARD_FLAG, GUARD_STEPS).
 hipError_t launch_guard_reset(Context &c);
+// `steps` guarded fused steps in one co-resident launch (k_step_loop), for
+// HDM_REBUILD_LEARNED with hdm_exec_params.persistent; every step applies
+// the previous step's break and none stores the force.  Ends early after a
+// step that trips the exact test, like the per-step launches.
+// *barrierFailed: a grid-barrier wait timed out (the steps are not valid).
+hipError_t launch_step_loop(Context &c, const hdm_force_params &fp, const hdm_integ_params &ip, uint64_t *skin,
+                            const hdm_break_params &brk, const StepGuard &guard, int32_t steps, bool *barrierFailed);
 hipError_t launch_break(Context &c, const hdm_break_params &bp, uint64_t *counter);
 // The same break per particle from the current records (hdm_run's last step;
 // launch_break when the records are not current).  counter: links broken.

@@ -249,6 +249,7 @@ struct ForceArgs {
     const double *tx, *ty, *tz, *initialLen, *maxLoad;
     uint8_t *state;
     double bendBreak, scalarMaxLoad;
+    uint64_t *brokenCount;  // fused break: links broken are added here (nullptr: not counted)
 };
 
 // Force on particle p (CombinedAccel::iterateCombined body): contacts in list
@@ -375,9 +376,15 @@ struct IntegArgs {
     // doubles per workgroup, max(0, max Diff) per axis (HiDEMLib.h:424-461)
     // and max |x - x0|^2, folded by k_skin_finalize
     double *partials;
-    // exact-test guard (nullptr: off): a launch that finds *guard set does
-    // nothing (k_skin_finalize sets it when the list no longer covers a step)
+    // exact-test guard (nullptr: off; see StepGuard): a launch that finds
+    // *guard (the trip word of its input records) set does nothing but set
+    // *guardOut; otherwise a particle whose new position trips the exact test
+    // sets *guardOut, and workgroup 0 counts the executed launch in
+    // *stepsDone.  k_skin_finalize evaluates the same test for the
+    // host-checked policies.
     const uint32_t *guard;
+    uint32_t *guardOut, *stepsDone;
+    double guardThr;
     const double *x0, *y0, *z0;   // nullptr: no displacement reduction
     const int32_t *blockId;       // batched form: waterline of the particle's block
     const double *blockWater;     // (nullptr: ip.waterLevel for every particle)
@@ -609,18 +616,13 @@ struct FinalizeArgs {
     double *stage;        // [gridDim.x][4]
     uint32_t *ticket;
     uint64_t *skin;
-    uint32_t *guard;      // nullptr: no guard
-    uint32_t *stepsDone;
-    double guardThr;
 };
 
 // Folds the rows of the integrating launch: each workgroup reduces a slice
 // into stage, and the last one (a ticket after a release fence: one fence
 // per workgroup of this small launch) stores skin[0..2], skin[4] as ordered
-// keys, evaluates the exact-test guard and counts the executed step.  With the
-// guard set on entry (the step was skipped) it does nothing.
+// keys for the host-checked rebuild policies.
 __global__ void __launch_bounds__(TPB) k_skin_finalize(FinalizeArgs a) {
-    if (a.guard && *a.guard) return;
     __shared__ double part[TPB / 64][4];
     __shared__ int last;
     double m[4] = {0.0, 0.0, 0.0, 0.0};
@@ -656,10 +658,6 @@ __global__ void __launch_bounds__(TPB) k_skin_finalize(FinalizeArgs a) {
     for (int c = 0; c < 3; c++) a.skin[c] = ordered_key(f[c]);
     a.skin[4] = ordered_key(f[3]);
     *a.ticket = 0;
-    if (a.guard) {
-        if (2.0 * sqrt(f[3]) >= a.guardThr) *a.guard = 1u;
-        *a.stepsDone += 1u;
-    }
 }
 
 template <class FO>
@@ -874,15 +872,14 @@ constexpr int step_waves(bool occ) {
     return sizeof(R) == 4 ? (occ ? 5 : 4) : (occ ? 4 : 3);
 }
 
-template <class R, class CT, int BLOCK, bool XCD, bool OCC>
-__global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R, CT, R> f, IntegArgs g,
-                                                                    StepRecs<R> r) {
-    if (g.guard && *g.guard) return;  // an earlier step found the neighbour list stale
-    const int64_t wg = XCD ? xcd_block_index(gridDim.x) : (int64_t)blockIdx.x;
-    const int64_t p = wg * BLOCK + threadIdx.x;
-    const bool active = p < f.n;
-    double cordN[3] = {0, 0, 0}, diffN[3] = {0, 0, 0};
-    if (active) {
+// One particle of the fused step (the body of k_step and k_step_loop):
+// break of the previous step, force, integration and the new records;
+// cordN / diffN receive the integrated coordinates and translation Diff, and
+// broke counts the links it broke at their smaller-index end.
+template <class R, class CT>
+__device__ __forceinline__ void step_particle(const ForceArgs<R, CT, R> &f, const IntegArgs &g, const StepRecs<R> &r,
+                                              const int64_t p, double cordN[3], double diffN[3], uint32_t &broke) {
+    {
         const auto P = r.rec[p];
         const double xCord = P.x, yCord = P.y, zCord = P.z;
         double diff[6];
@@ -977,6 +974,7 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
                         f.slotLinkW[sidx] = word | SLOT_BROKEN;
                         f.slotOtherW[sidx] = ow | SLOT_BROKEN;
                         f.state[word] = 0;
+                        broke += p < q ? 1u : 0u;
                     }
                 }
                 R ct[6] = {(R)cs[0], (R)cs[1], (R)cs[2], (R)cs[3], (R)cs[4], (R)cs[5]};
@@ -1018,7 +1016,34 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
             diffN[c] = diff[c];
         }
     }
-    if (g.partials) skin_partials<BLOCK>(g, active, p, cordN, diffN);
+}
+
+template <class R, class CT, int BLOCK, bool XCD, bool OCC>
+__global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R, CT, R> f, IntegArgs g,
+                                                                    StepRecs<R> r) {
+    if (g.guard && *g.guard) {  // an earlier step found the neighbour list stale
+        if (blockIdx.x == 0 && threadIdx.x == 0) *g.guardOut = 1u;
+        return;
+    }
+    const int64_t wg = XCD ? xcd_block_index(gridDim.x) : (int64_t)blockIdx.x;
+    uint32_t broke = 0;  // links this lane's particle broke (counted at its smaller-index end)
+    const int64_t p = wg * BLOCK + threadIdx.x;
+    const bool active = p < f.n;
+    double cordN[3] = {0, 0, 0}, diffN[3] = {0, 0, 0};
+    if (active) step_particle<R, CT>(f, g, r, p, cordN, diffN, broke);
+    if (f.brokenCount) {
+        broke = wave_sum_u32(broke);
+        if ((threadIdx.x & 63) == 0 && broke) atomicAdd((unsigned long long *)f.brokenCount, (unsigned long long)broke);
+    }
+    if (g.guardOut) {
+        if (active) {
+            const double ddx = cordN[0] - g.x0[p], ddy = cordN[1] - g.y0[p], ddz = cordN[2] - g.z0[p];
+            if (2.0 * sqrt(ddx * ddx + ddy * ddy + ddz * ddz) >= g.guardThr) *g.guardOut = 1u;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(g.stepsDone, 1u);
+    } else if (g.partials) {
+        skin_partials<BLOCK>(g, active, p, cordN, diffN);
+    }
 }
 
 // The link break that ends a run: linkUpdate of its last step
@@ -1065,8 +1090,124 @@ __global__ void __launch_bounds__(TPB) k_break_recs(ForceArgs<R, R, R> f, StepRe
         }
     }
     broke = wave_sum_u32(broke);
-    if ((threadIdx.x & 63) == 0 && broke && counter)
-        atomicAdd((unsigned long long *)counter, (unsigned long long)broke);
+    if ((threadIdx.x & 63) == 0 && broke) {
+        if (counter) atomicAdd((unsigned long long *)counter, (unsigned long long)broke);
+        if (f.brokenCount) atomicAdd((unsigned long long *)f.brokenCount, (unsigned long long)broke);
+    }
+}
+
+// ---- persistent timestep ------------------------------------------------------
+// Grid barrier of a co-resident (cooperative) launch, in two levels so no
+// word takes more than a few dozen arrivals: workgroup b arrives at group
+// b % GRID_GROUPS, the last arrival of a group at the top word, and the last
+// arrival there re-arms the counters and advances the generation the others
+// wait on.  The words sit 128 B apart (their own L2 lines).  A wait that
+// outlasts maxSpins sets *error and returns, so a launch that could not make
+// every workgroup resident ends instead of hanging (the host then fails the
+// run).
+constexpr int GRID_GROUPS = 64;
+constexpr int BARRIER_STRIDE = 32;   // u32 words between counters
+constexpr int BARRIER_WORDS = (GRID_GROUPS + 2) * BARRIER_STRIDE;
+
+__device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t *error, uint32_t maxSpins) {
+    __threadfence();   // every wave's stores of this step reach L2 before the arrival
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t *top = bar + GRID_GROUPS * BARRIER_STRIDE;
+        uint32_t *gen = top + BARRIER_STRIDE;
+        const uint32_t g = blockIdx.x % GRID_GROUPS;
+        const uint32_t groups = gridDim.x < GRID_GROUPS ? gridDim.x : GRID_GROUPS;
+        const uint32_t members = gridDim.x / GRID_GROUPS + (g < gridDim.x % GRID_GROUPS ? 1u : 0u);
+        const uint32_t seen = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        if (atomicAdd(bar + g * BARRIER_STRIDE, 1u) == members - 1) {
+            __hip_atomic_store(bar + g * BARRIER_STRIDE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            if (atomicAdd(top, 1u) == groups - 1) {
+                __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __threadfence();
+                atomicAdd(gen, 1u);
+            }
+        }
+        uint32_t spins = 0;
+        while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == seen) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > maxSpins) {
+                atomicOr(error, 1u);
+                break;
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+    __threadfence();   // and every wave reads the next step's inputs past its L1
+}
+
+// Tile of the k-th visit of workgroup b in a grid of G workgroups (G a
+// multiple of 8) over T tiles: visit v = b + k G is dispatched on XCD v % 8,
+// and each XCD's visits cover one contiguous range of tiles (the numbering
+// of xcd_block_index over the T visits).
+__device__ __forceinline__ int64_t xcd_tile(int64_t v, int64_t T) {
+    const int64_t q = T / 8, rem = T % 8, xcd = v % 8, local = v / 8;
+    return (xcd < rem) ? xcd * (q + 1) + local : rem * (q + 1) + (xcd - rem) * q + local;
+}
+
+struct LoopArgs {
+    int32_t steps;        // steps to run; fewer when one trips the exact test
+    uint32_t *barrier;    // BARRIER_WORDS, zero before the first launch
+    uint32_t *error;      // set when a barrier wait timed out
+    uint32_t maxSpins;
+};
+
+// Several fused steps in one co-resident launch (HDM_REBUILD_LEARNED with
+// hdm_exec_params.persistent): every workgroup runs the particles of its
+// tiles through step_particle, the grid synchronises, the ping-pong roles
+// swap, and the next step starts -- unless the step just done tripped the
+// exact test (its output trip word, read after the barrier by every
+// workgroup alike), in which case the launch ends there as the guarded
+// per-step launches would have ended.  Same arithmetic per particle as
+// k_step, so the same bits.
+template <class R, class CT, int BLOCK>
+__global__ void __launch_bounds__(BLOCK, step_waves<R>(false)) k_step_loop(ForceArgs<R, CT, R> f, IntegArgs g,
+                                                                            StepRecs<R> r, LoopArgs la) {
+    const int64_t T = (f.n + BLOCK - 1) / BLOCK;
+    for (int32_t s = 0; s < la.steps; s++) {
+        if (__hip_atomic_load(g.guard, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) *g.guardOut = 1u;
+            return;
+        }
+        uint32_t broke = 0;
+        bool trip = false;
+        for (int64_t v = blockIdx.x; v < T; v += gridDim.x) {
+            const int64_t p = xcd_tile(v, T) * BLOCK + threadIdx.x;
+            if (p < f.n) {
+                double cordN[3], diffN[3];
+                step_particle<R, CT>(f, g, r, p, cordN, diffN, broke);
+                const double ddx = cordN[0] - g.x0[p], ddy = cordN[1] - g.y0[p], ddz = cordN[2] - g.z0[p];
+                trip |= 2.0 * sqrt(ddx * ddx + ddy * ddy + ddz * ddz) >= g.guardThr;
+            }
+        }
+        if (trip) *g.guardOut = 1u;
+        if (f.brokenCount) {
+            broke = wave_sum_u32(broke);
+            if ((threadIdx.x & 63) == 0 && broke)
+                atomicAdd((unsigned long long *)f.brokenCount, (unsigned long long)broke);
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(g.stepsDone, 1u);
+        grid_barrier(la.barrier, la.error, la.maxSpins);
+        if (__hip_atomic_load(la.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+        // the integrated state becomes the input of the next step
+        { auto t = r.rec; r.rec = r.recOut; r.recOut = const_cast<typename StepRecs<R>::Rec *>(t); }
+        { auto t = r.drec; r.drec = r.drecOut; r.drecOut = const_cast<double *>(t); }
+        { auto t = r.tail; r.tail = r.tailOut; r.tailOut = const_cast<MixedTail *>(t); }
+        { auto t = g.tx; g.tx = g.otx; g.otx = const_cast<double *>(t); }
+        { auto t = g.ty; g.ty = g.oty; g.oty = const_cast<double *>(t); }
+        { auto t = g.tz; g.tz = g.otz; g.otz = const_cast<double *>(t); }
+        f.tx = g.tx;
+        f.ty = g.ty;
+        f.tz = g.tz;
+        { auto t = g.guard; g.guard = g.guardOut; g.guardOut = const_cast<uint32_t *>(t); }
+    }
 }
 
 // SoA coordinates and Diff from the records (sync_state).
@@ -1224,6 +1365,9 @@ IntegArgs integ_args(Context &c, const hdm_integ_params &ip, uint64_t *skin, boo
     a.ip = ip;
     a.partials = skin ? c.skinPartials.as<double>() : nullptr;
     a.guard = nullptr;
+    a.guardOut = nullptr;
+    a.stepsDone = nullptr;
+    a.guardThr = 0.0;
     a.x0 = withDisplacement ? c.x0.as<double>() : nullptr;
     a.y0 = withDisplacement ? c.y0.as<double>() : nullptr;
     a.z0 = withDisplacement ? c.z0.as<double>() : nullptr;
@@ -1259,6 +1403,7 @@ ForceArgs<R, CT, FO> force_args(Context &c, const hdm_force_params &fp) {
     a.state = c.state.as<uint8_t>();
     a.bendBreak = 0;
     a.scalarMaxLoad = 0;
+    a.brokenCount = c.brokenCount.as<uint64_t>();
     return a;
 }
 
@@ -1295,16 +1440,13 @@ hipError_t ensure_partials(Context &c) {
 }
 
 // k_skin_finalize over the rows of a launch of rows_for(n, block) workgroups.
-hipError_t skin_finalize(Context &c, uint64_t *skin, const StepGuard *guard, int block = TPB) {
+hipError_t skin_finalize(Context &c, uint64_t *skin, int block = TPB) {
     FinalizeArgs f;
     f.rows = rows_for(c.n, block);
     f.partials = c.skinPartials.as<double>();
     f.stage = c.skinPartials.as<double>() + 4 * f.rows;
     f.ticket = c.guardState.as<uint32_t>() + GUARD_TICKET;
     f.skin = skin;
-    f.guard = guard ? c.guardState.as<uint32_t>() + GUARD_FLAG : nullptr;
-    f.stepsDone = guard ? c.guardState.as<uint32_t>() + GUARD_STEPS : nullptr;
-    f.guardThr = guard ? guard->threshold : 0.0;
     hipLaunchKernelGGL(k_skin_finalize, dim3(FINALIZE_BLOCKS), dim3(TPB), 0, c.stream, f);
     return hipGetLastError();
 }
@@ -1314,7 +1456,7 @@ hipError_t skin_finalize(Context &c, uint64_t *skin, const StepGuard *guard, int
 hipError_t launch_guard_reset(Context &c) {
     hipError_t e = ensure_partials(c);
     if (e != hipSuccess) return e;
-    return launch_fill_u32(c.stream, c.guardState.as<uint32_t>() + GUARD_FLAG, 0u, 2);
+    return launch_fill_u32(c.stream, c.guardState.as<uint32_t>() + GUARD_FLAG, 0u, 3);
 }
 
 hipError_t launch_force(Context &c, const hdm_force_params &fp) {
@@ -1342,7 +1484,7 @@ hipError_t launch_integrate(Context &c, const hdm_integ_params &ip, uint64_t *sk
                            c.force[3].as<float>(), c.force[4].as<float>(), c.force[5].as<float>());
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !skin) return e;
-    return skin_finalize(c, skin, nullptr);
+    return skin_finalize(c, skin);
 }
 
 namespace {
@@ -1439,6 +1581,7 @@ hipError_t launch_pack_records(Context &c, double tame) {
 }
 
 void swap_step_state(Context &c) {
+    c.guardCur ^= 1;
     swap_buf(c.tx, c.alt[3]);
     swap_buf(c.ty, c.alt[4]);
     swap_buf(c.tz, c.alt[5]);
@@ -1483,7 +1626,12 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
     IntegArgs g = integ_args(c, ip, skin, withDisplacement, c.alt);
     if (guard) {
         if (!skin || !withDisplacement) return hipErrorInvalidValue;
-        g.guard = c.guardState.as<uint32_t>() + GUARD_FLAG;
+        uint32_t *words = c.guardState.as<uint32_t>();
+        g.guard = words + GUARD_FLAG + c.guardCur;
+        g.guardOut = words + GUARD_FLAG + (c.guardCur ^ 1);
+        g.stepsDone = words + GUARD_STEPS;
+        g.guardThr = guard->threshold;
+        g.partials = nullptr;   // the trip words replace the maxima
     }
     const int v = (c.stepVariant >= 0 && c.stepVariant < STEP_VARIANTS) ? c.stepVariant : 0;
     const StepShape shape = kShapes[v];
@@ -1516,8 +1664,8 @@ hipError_t launch_step(Context &c, const hdm_force_params &fp, const hdm_integ_p
     // coordinates and Diff are unpacked on demand
     swap_step_state(c);
     c.soaStale = true;
-    if (!skin) return hipSuccess;
-    return skin_finalize(c, skin, guard, shape.block);
+    if (!skin || guard) return hipSuccess;
+    return skin_finalize(c, skin, shape.block);
 }
 
 hipError_t launch_break_records(Context &c, const hdm_break_params &bp, uint64_t *counter) {
@@ -1539,6 +1687,78 @@ hipError_t launch_break_records(Context &c, const hdm_break_params &bp, uint64_t
     if (c.prec == HDM_PREC_FP64) go(force_args<double, double, double>(c, fp), step_recs<double>(c, c.recTame, &bp));
     else go(force_args<float, float, float>(c, fp), step_recs<float>(c, c.recTame, &bp));
     return hipGetLastError();
+}
+
+hipError_t launch_step_loop(Context &c, const hdm_force_params &fp, const hdm_integ_params &ip, uint64_t *skin,
+                            const hdm_break_params &brk, const StepGuard &guard, int32_t steps, bool *barrierFailed) {
+    *barrierFailed = false;
+    if (c.n == 0 || steps <= 0) return hipSuccess;
+    if (!c.coopLaunch) return hipErrorNotSupported;
+    const size_t b = sizeof(double) * (size_t)c.n;
+    for (int k = 3; k < 6; k++) {
+        hipError_t e = c.alt[k].ensure(b);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = ensure_partials(c);
+    if (e != hipSuccess) return e;
+    if (!c.loopBarrier.ptr) {
+        e = c.loopBarrier.ensure(sizeof(uint32_t) * (BARRIER_WORDS + 1));
+        if (e == hipSuccess) e = launch_fill_u32(c.stream, c.loopBarrier.as<uint32_t>(), 0u, BARRIER_WORDS + 1);
+        if (e != hipSuccess) return e;
+    }
+    const double tame = tame_bound(c, fp);
+    if (!c.recCurrent || c.recTame != tame) {
+        e = launch_pack_records(c, tame);
+        if (e != hipSuccess) return e;
+    }
+    IntegArgs g = integ_args(c, ip, skin, true, c.alt);
+    uint32_t *words = c.guardState.as<uint32_t>();
+    g.guard = words + GUARD_FLAG + c.guardCur;
+    g.guardOut = words + GUARD_FLAG + (c.guardCur ^ 1);
+    g.stepsDone = words + GUARD_STEPS;
+    g.guardThr = guard.threshold;
+    g.partials = nullptr;
+    LoopArgs la;
+    la.steps = steps;
+    la.barrier = c.loopBarrier.as<uint32_t>();
+    la.error = la.barrier + BARRIER_WORDS;
+    la.maxSpins = 1u << 22;   // seconds of waiting; a step takes milliseconds
+    auto go = [&](auto f, auto r, const void *kernel) -> hipError_t {
+        r.storeForce = 0;
+        f.breakCheck = c.L > 0 ? 1 : 0;
+        f.bendBreak = brk.bendBreak;
+        f.scalarMaxLoad = brk.maxLoad;
+        int perCU = 0;
+        hipError_t e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, TPB, 0);
+        if (e2 != hipSuccess) return e2;
+        const int64_t tiles = (c.n + TPB - 1) / TPB;
+        int64_t grid = (int64_t)perCU * (c.numCU > 0 ? c.numCU : 1);
+        if (grid > tiles) grid = tiles;
+        grid -= grid % 8;
+        if (grid < 8) return hipErrorInvalidConfiguration;
+        void *args[] = {&f, &g, &r, &la};
+        return hipLaunchCooperativeKernel(kernel, dim3((unsigned)grid), dim3(TPB), args, 0, c.stream);
+    };
+    if (c.prec == HDM_PREC_FP64)
+        e = go(force_args<double, double, double>(c, fp), step_recs<double>(c, tame, &brk),
+               reinterpret_cast<const void *>(&k_step_loop<double, double, TPB>));
+    else
+        e = go(force_args<float, float, float>(c, fp), step_recs<float>(c, tame, &brk),
+               reinterpret_cast<const void *>(&k_step_loop<float, float, TPB>));
+    if (e != hipSuccess) return e;
+    // the launch swapped the roles `steps` times (the host undoes the ones a
+    // trip skipped, as for skipped per-step launches)
+    if (steps & 1) swap_step_state(c);
+    c.soaStale = true;
+    uint32_t err = 0;
+    e = hipMemcpyAsync(&err, la.error, sizeof(err), hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) return e;
+    if (err) {   // re-arm the barrier words for the next launch
+        *barrierFailed = true;
+        return launch_fill_u32(c.stream, c.loopBarrier.as<uint32_t>(), 0u, BARRIER_WORDS + 1);
+    }
+    return hipSuccess;
 }
 
 const char *step_variant_name(int v) { return (v >= 0 && v < STEP_VARIANTS) ? kShapes[v].name : "?"; }

@@ -333,6 +333,9 @@ typedef struct hdm_exec_params {
                                    (the reference's list); 0 (default) leaves them out and the
                                    link loop applies their contact law (fewer gathers).  FP64
                                    always uses the reference's list. */
+    int32_t persistent;         /* 1: the steps of a batch run in one co-resident launch that
+                                   synchronises the grid between steps (256-thread workgroups,
+                                   no autotuned shape, no graphs); same results */
 } hdm_exec_params;
 
 typedef struct hdm_exec_info {
@@ -341,6 +344,7 @@ typedef struct hdm_exec_info {
     double tuneMs[8];           /* best step time per shape seen by the autotuner (ms) */
     int64_t graphCaptures, graphReplays;
     int32_t mergedContacts;     /* the current neighbour list leaves out link partners */
+    int64_t persistentLaunches; /* co-resident multi-step launches (hdm_exec_params.persistent) */
 } hdm_exec_info;
 
 int  hdm_set_exec(hdm_ctx *ctx, const hdm_exec_params *ep);

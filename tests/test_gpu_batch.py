@@ -51,12 +51,13 @@ def test_batch_fp64_matches_oracle(batch, orc):
     assert np.ptp(per_block) > 0
 
 
-def test_batch_graph_autotuned_matches_eager(batch):
+@pytest.mark.parametrize("exec_", [dict(autotune=True, graphs=True), dict(persistent=True)],
+                         ids=["graphs", "persistent"])
+def test_batch_graph_autotuned_matches_eager(batch, exec_):
     s, b = batch
-    say("batch learned+graph vs exact")
+    say("batch learned vs exact", exec_)
     out = {}
-    for name, policy, exec_ in (("exact", A.HDM_REBUILD_EXACT, None),
-                                ("learned", A.HDM_REBUILD_LEARNED, dict(autotune=True, graphs=True))):
+    for name, policy, ex in (("exact", A.HDM_REBUILD_EXACT, None), ("learned", A.HDM_REBUILD_LEARNED, exec_)):
         rp = A.HdmRunParams.from_buffer_copy(s.run)
         rp.policy = policy
         rp.interactDist = rp.force.attractDistance + 0.004     # frequent rebuilds: recaptures and guard trips
@@ -64,8 +65,8 @@ def test_batch_graph_autotuned_matches_eager(batch):
             c.upload(b.particles, b.links)
             A.set_blocks(c, b.block_id, b.water)
             c.sched_configure(min_batch=16, max_batch=64)
-            if exec_:
-                c.set_exec(**exec_)
+            if ex:
+                c.set_exec(**ex)
             logs = [c.run(rp, k, log_rebuilds=True)[1] for k in (120, 180)]
             out[name] = (np.concatenate(logs), c.download_particles(), c.download_links(), c.exec_info())
     (le, pe, ke, _), (ll, pl, kl, info) = out["exact"], out["learned"]
@@ -77,7 +78,10 @@ def test_batch_graph_autotuned_matches_eager(batch):
     assert np.array_equal(kl.state, ke.state)
     for k in range(6):
         assert bits_equal(kl.ct[k], ke.ct[k])
-    assert info["tuned"] and info["graph_replays"] > 0
+    if exec_.get("graphs"):
+        assert info["tuned"] and info["graph_replays"] > 0
+    else:
+        assert info["persistent_launches"] > 0
 
 
 def test_batch_at_config_sizes_against_fp64_oracle(orc):

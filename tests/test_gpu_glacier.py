@@ -173,7 +173,7 @@ def test_learned_scheduler_matches_exact_on_full_slab(slab):
     rp.interactDist = rp.force.attractDistance + 2.0 * disp / 3.0
     say(f"slab: max displacement over {steps} steps {disp:.3e} m, skin {rp.interactDist:.4e} m")
 
-    def run(policy, learned):
+    def run(policy, learned, persistent=False):
         r = A.HdmRunParams.from_buffer_copy(rp)
         r.policy = policy
         with A.Context(A.HDM_PREC_MIXED) as ctx:
@@ -182,7 +182,10 @@ def test_learned_scheduler_matches_exact_on_full_slab(slab):
             if learned:
                 ctx.sched_configure(block_cells=4, min_batch=8, max_batch=256, safety=0.5)
                 ctx.sched_set_model(model)
-                ctx.set_exec(autotune=True, graphs=True)
+                if persistent:
+                    ctx.set_exec(persistent=True)
+                else:
+                    ctx.set_exec(autotune=True, graphs=True)
             logs = []
             for k in (150, 150):
                 _, log = ctx.run(r, k, log_rebuilds=True)
@@ -195,6 +198,13 @@ def test_learned_scheduler_matches_exact_on_full_slab(slab):
     log_e, pe, le, _, _ = run(A.HDM_REBUILD_EXACT, False)
     say("slab: learned scheduler, 300 steps")
     log_l, pl, ll, info, sched = run(A.HDM_REBUILD_LEARNED, True)
+    say("slab: learned scheduler, one launch per batch, 300 steps")
+    log_p, pp, lp, info_p, _ = run(A.HDM_REBUILD_LEARNED, True, persistent=True)
+    assert np.array_equal(log_p, log_e) and info_p["persistent_launches"] > 0
+    for f in A.PARTICLE_FIELDS:
+        assert bits_equal(pp[f], pe[f]), f
+    assert np.array_equal(lp.state, le.state)
+    del pp, lp
     say(f"  rebuild steps {np.nonzero(log_e)[0].tolist()}, learned {np.nonzero(log_l)[0].tolist()}, "
         f"sched {sched}, shape {info['name']}, captures {info['graph_captures']}, replays {info['graph_replays']}")
     assert log_e.sum() >= 3   # the first step's build and at least two trips
