@@ -198,6 +198,7 @@ struct Context {
     bool usePersistent = false;  // hdm_exec_params.persistent
     int64_t loopLaunches = 0;    // k_step_loop launches
     DevBuf loopBarrier;          // grid-barrier words of k_step_loop (+1: timeout flag)
+    DevBuf loopSets;             // k_step_loop argument sets (this step's, roles swapped)
     hipGraphExec_t graphExec = nullptr;      // replays GRAPH_STEPS_LONG steps
     hipGraphExec_t graphExecPair = nullptr;  // replays 2 steps
     std::vector<unsigned char> graphKey, graphKeyPair;

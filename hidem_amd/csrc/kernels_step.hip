@@ -226,6 +226,7 @@ __device__ __forceinline__ void beam_law(R mask, R xDist, R yDist, R zDist, cons
 
 template <class R, class CT, class FO>
 struct ForceArgs {
+    using CTType = CT;
     const double *x, *y, *z, *d0, *d1, *d2, *d3, *d4, *d5, *radius;
     int64_t n;
     const int32_t *nbrCount, *nbr;
@@ -685,6 +686,7 @@ __device__ __forceinline__ float rec_radius(const MixedRec &P) { return P.radius
 // Gather records (see Context::rec / drec) and the break prefilter inputs.
 template <class R>
 struct StepRecs {
+    using Real = R;
     using Rec = typename RecOf<R>::type;
     const Rec *rec;            // current state
     const double *drec;        // FP64 form: every particle's FP64 Diff; FP32 form: the escapes
@@ -1101,16 +1103,20 @@ __global__ void __launch_bounds__(TPB) k_break_recs(ForceArgs<R, R, R> f, StepRe
 // word takes more than a few dozen arrivals: workgroup b arrives at group
 // b % GRID_GROUPS, the last arrival of a group at the top word, and the last
 // arrival there re-arms the counters and advances the generation the others
-// wait on.  The words sit 128 B apart (their own L2 lines).  A wait that
-// outlasts maxSpins sets *error and returns, so a launch that could not make
-// every workgroup resident ends instead of hanging (the host then fails the
-// run).
+// wait on.  The words sit 128 B apart (their own L2 lines).  Only the first
+// thread of a workgroup fences: the workgroup barrier before it has every
+// wave's stores in its XCD's L2, which its release fence writes back; its
+// acquire fence invalidates the L2 and the CU's L1 that the workgroup's
+// waves read through after the closing workgroup barrier.  (Agent-scope
+// fences write back and invalidate a whole XCD's L2, so one per workgroup
+// and barrier rather than one per wave.)  A wait that outlasts maxSpins sets
+// *error and returns, so a launch that could not make every workgroup
+// resident ends instead of hanging (the host then fails the run).
 constexpr int GRID_GROUPS = 64;
 constexpr int BARRIER_STRIDE = 32;   // u32 words between counters
 constexpr int BARRIER_WORDS = (GRID_GROUPS + 2) * BARRIER_STRIDE;
 
 __device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t *error, uint32_t maxSpins) {
-    __threadfence();   // every wave's stores of this step reach L2 before the arrival
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t *top = bar + GRID_GROUPS * BARRIER_STRIDE;
@@ -1118,15 +1124,14 @@ __device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t *error, uin
         const uint32_t g = blockIdx.x % GRID_GROUPS;
         const uint32_t groups = gridDim.x < GRID_GROUPS ? gridDim.x : GRID_GROUPS;
         const uint32_t members = gridDim.x / GRID_GROUPS + (g < gridDim.x % GRID_GROUPS ? 1u : 0u);
+        // the generation cannot advance before this workgroup arrives
         const uint32_t seen = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (atomicAdd(bar + g * BARRIER_STRIDE, 1u) == members - 1) {
             __hip_atomic_store(bar + g * BARRIER_STRIDE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
-            if (atomicAdd(top, 1u) == groups - 1) {
+            if (__hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == groups - 1) {
                 __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __threadfence();
-                atomicAdd(gen, 1u);
+                __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         uint32_t spins = 0;
@@ -1137,10 +1142,9 @@ __device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t *error, uin
                 break;
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    __threadfence();   // and every wave reads the next step's inputs past its L1
 }
 
 // Tile of the k-th visit of workgroup b in a grid of G workgroups (G a
@@ -1159,55 +1163,92 @@ struct LoopArgs {
     uint32_t maxSpins;
 };
 
+// The arguments of one step of k_step_loop.  Two sets sit in device memory:
+// the first step's and the one with the ping-pong roles swapped, and step s
+// reads set s % 2.  Loading a field from the set of the step's parity at its
+// point of use keeps it out of the registers between uses; roles swapped in
+// registers stay live across the whole loop and spill (and two sets passed
+// as kernel arguments and selected by parity are copied to scratch).
+template <class R, class CT>
+struct LoopSet {
+    ForceArgs<R, CT, R> f;
+    IntegArgs g;
+    StepRecs<R> r;
+};
+
 // Several fused steps in one co-resident launch (HDM_REBUILD_LEARNED with
 // hdm_exec_params.persistent): every workgroup runs the particles of its
-// tiles through step_particle, the grid synchronises, the ping-pong roles
-// swap, and the next step starts -- unless the step just done tripped the
+// tiles through step_particle, the grid synchronises, and the next step
+// starts on the other argument set -- unless the step just done tripped the
 // exact test (its output trip word, read after the barrier by every
 // workgroup alike), in which case the launch ends there as the guarded
 // per-step launches would have ended.  Same arithmetic per particle as
 // k_step, so the same bits.
 template <class R, class CT, int BLOCK>
-__global__ void __launch_bounds__(BLOCK, step_waves<R>(false)) k_step_loop(ForceArgs<R, CT, R> f, IntegArgs g,
-                                                                            StepRecs<R> r, LoopArgs la) {
-    const int64_t T = (f.n + BLOCK - 1) / BLOCK;
+__global__ void __launch_bounds__(BLOCK, step_waves<R>(false)) k_step_loop(const LoopSet<R, CT> *__restrict__ sets,
+                                                                            LoopArgs la) {
+    const int64_t T = (sets[0].f.n + BLOCK - 1) / BLOCK;
     for (int32_t s = 0; s < la.steps; s++) {
-        if (__hip_atomic_load(g.guard, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) *g.guardOut = 1u;
+        const LoopSet<R, CT> &a = sets[s & 1];
+        if (__hip_atomic_load(a.g.guard, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) *a.g.guardOut = 1u;
             return;
         }
         uint32_t broke = 0;
         bool trip = false;
         for (int64_t v = blockIdx.x; v < T; v += gridDim.x) {
             const int64_t p = xcd_tile(v, T) * BLOCK + threadIdx.x;
-            if (p < f.n) {
+            if (p < a.f.n) {
                 double cordN[3], diffN[3];
-                step_particle<R, CT>(f, g, r, p, cordN, diffN, broke);
-                const double ddx = cordN[0] - g.x0[p], ddy = cordN[1] - g.y0[p], ddz = cordN[2] - g.z0[p];
-                trip |= 2.0 * sqrt(ddx * ddx + ddy * ddy + ddz * ddz) >= g.guardThr;
+                step_particle<R, CT>(a.f, a.g, a.r, p, cordN, diffN, broke);
+                const double ddx = cordN[0] - a.g.x0[p], ddy = cordN[1] - a.g.y0[p], ddz = cordN[2] - a.g.z0[p];
+                trip |= 2.0 * sqrt(ddx * ddx + ddy * ddy + ddz * ddz) >= a.g.guardThr;
             }
         }
-        if (trip) *g.guardOut = 1u;
-        if (f.brokenCount) {
+        if (trip) *a.g.guardOut = 1u;
+        if (a.f.brokenCount) {
             broke = wave_sum_u32(broke);
             if ((threadIdx.x & 63) == 0 && broke)
-                atomicAdd((unsigned long long *)f.brokenCount, (unsigned long long)broke);
+                atomicAdd((unsigned long long *)a.f.brokenCount, (unsigned long long)broke);
         }
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(g.stepsDone, 1u);
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.g.stepsDone, 1u);
         grid_barrier(la.barrier, la.error, la.maxSpins);
         if (__hip_atomic_load(la.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-        // the integrated state becomes the input of the next step
-        { auto t = r.rec; r.rec = r.recOut; r.recOut = const_cast<typename StepRecs<R>::Rec *>(t); }
-        { auto t = r.drec; r.drec = r.drecOut; r.drecOut = const_cast<double *>(t); }
-        { auto t = r.tail; r.tail = r.tailOut; r.tailOut = const_cast<MixedTail *>(t); }
-        { auto t = g.tx; g.tx = g.otx; g.otx = const_cast<double *>(t); }
-        { auto t = g.ty; g.ty = g.oty; g.oty = const_cast<double *>(t); }
-        { auto t = g.tz; g.tz = g.otz; g.otz = const_cast<double *>(t); }
-        f.tx = g.tx;
-        f.ty = g.ty;
-        f.tz = g.tz;
-        { auto t = g.guard; g.guard = g.guardOut; g.guardOut = const_cast<uint32_t *>(t); }
     }
+}
+
+// Stores the two argument sets of a k_step_loop launch.
+template <class R, class CT>
+__global__ void k_store_loop_sets(LoopSet<R, CT> even, LoopSet<R, CT> odd, LoopSet<R, CT> *out) {
+    if (threadIdx.x == 0) {
+        out[0] = even;
+        out[1] = odd;
+    }
+}
+
+// The argument set of the step after `a`: the integrated state becomes the
+// input, the trip words change roles.
+template <class R, class CT>
+LoopSet<R, CT> swapped_roles(const LoopSet<R, CT> &a) {
+    LoopSet<R, CT> b = a;
+    b.r.rec = a.r.recOut;
+    b.r.recOut = const_cast<typename StepRecs<R>::Rec *>(a.r.rec);
+    b.r.drec = a.r.drecOut;
+    b.r.drecOut = const_cast<double *>(a.r.drec);
+    b.r.tail = a.r.tailOut;
+    b.r.tailOut = const_cast<MixedTail *>(a.r.tail);
+    b.g.tx = a.g.otx;
+    b.g.ty = a.g.oty;
+    b.g.tz = a.g.otz;
+    b.g.otx = const_cast<double *>(a.g.tx);
+    b.g.oty = const_cast<double *>(a.g.ty);
+    b.g.otz = const_cast<double *>(a.g.tz);
+    b.f.tx = b.g.tx;
+    b.f.ty = b.g.ty;
+    b.f.tz = b.g.tz;
+    b.g.guard = a.g.guardOut;
+    b.g.guardOut = const_cast<uint32_t *>(a.g.guard);
+    return b;
 }
 
 // SoA coordinates and Diff from the records (sync_state).
@@ -1736,7 +1777,18 @@ hipError_t launch_step_loop(Context &c, const hdm_force_params &fp, const hdm_in
         if (grid > tiles) grid = tiles;
         grid -= grid % 8;
         if (grid < 8) return hipErrorInvalidConfiguration;
-        void *args[] = {&f, &g, &r, &la};
+        using Real = typename decltype(r)::Real;
+        using CTT = typename decltype(f)::CTType;
+        using Set = LoopSet<Real, CTT>;
+        e2 = c.loopSets.ensure(2 * sizeof(Set));
+        if (e2 != hipSuccess) return e2;
+        Set *sets = static_cast<Set *>(c.loopSets.ptr);
+        const Set even{f, g, r};
+        hipLaunchKernelGGL((k_store_loop_sets<Real, CTT>), dim3(1), dim3(64), 0, c.stream, even, swapped_roles(even),
+                           sets);
+        e2 = hipGetLastError();
+        if (e2 != hipSuccess) return e2;
+        void *args[] = {&sets, &la};
         return hipLaunchCooperativeKernel(kernel, dim3((unsigned)grid), dim3(TPB), args, 0, c.stream);
     };
     if (c.prec == HDM_PREC_FP64)
