@@ -252,6 +252,8 @@ struct NeighArgs {
 
 // Largest link-slot width whose partner lists are staged in LDS (32 KiB).
 constexpr int32_t NEIGH_LDS_WIDTH = 32;
+// Candidate records a lane loads together in the list build.
+constexpr int NEIGH_BATCH = 4;
 
 // One lane per particle.  NeighbourGrid::iteratePossibleNeighbours visits
 // the 27 cells col + xD + yD*X + zD*X*Y in z, y, x order, skipping targets
@@ -303,29 +305,39 @@ __global__ void __launch_bounds__(TPB) k_neigh(const double4 *__restrict__ crec,
         hi = hi >= a.totalCells ? a.totalCells - 1 : hi;
         if (lo > hi) continue;
         const int32_t s = cellStart[lo], e = cellStart[hi + 1];
-        for (int32_t i = s; i < e; ++i) {
-            if (i == self) continue;
-            const double4 Q = crec[i];
-            double xDist = xCord - Q.x;
-            double xDist2 = xDist * xDist;
-            double yDist = yCord - Q.y;
-            double yDist2 = yDist * yDist;
-            double zDist = zCord - Q.z;
-            double zDist2 = zDist * zDist;
-            double particleDistance2 = xDist2 + yDist2 + zDist2;
-            double particleDistance = sqrt(particleDistance2);
-            double radSum = rad + Q.w;
-            double dist = particleDistance - radSum;
-            if (dist < a.interactDist) {
-                const int32_t nInd = order[i];
-                if (pcnt && is_partner(nInd)) {
-                    skipped++;
-                    continue;
+        // candidates in groups of NEIGH_BATCH: the group's records are
+        // loaded together (the last candidate of the row stands in for
+        // positions past its end), so a lane waits once per group rather
+        // than once per candidate; tests and appends keep candidate order
+        for (int32_t i = s; i < e; i += NEIGH_BATCH) {
+            double4 Q[NEIGH_BATCH];
+#pragma unroll
+            for (int k = 0; k < NEIGH_BATCH; k++) Q[k] = crec[i + k < e ? i + k : e - 1];
+#pragma unroll
+            for (int k = 0; k < NEIGH_BATCH; k++) {
+                const int32_t ik = i + k;
+                if (ik >= e || ik == self) continue;
+                double xDist = xCord - Q[k].x;
+                double xDist2 = xDist * xDist;
+                double yDist = yCord - Q[k].y;
+                double yDist2 = yDist * yDist;
+                double zDist = zCord - Q[k].z;
+                double zDist2 = zDist * zDist;
+                double particleDistance2 = xDist2 + yDist2 + zDist2;
+                double particleDistance = sqrt(particleDistance2);
+                double radSum = rad + Q[k].w;
+                double dist = particleDistance - radSum;
+                if (dist < a.interactDist) {
+                    const int32_t nInd = order[ik];
+                    if (pcnt && is_partner(nInd)) {
+                        skipped++;
+                        continue;
+                    }
+                    if (FILL) {
+                        if (neighbours < a.width) out[(int64_t)neighbours * 64] = nInd;
+                    }
+                    neighbours++;
                 }
-                if (FILL) {
-                    if (neighbours < a.width) out[(int64_t)neighbours * 64] = nInd;
-                }
-                neighbours++;
             }
         }
     }

@@ -13,12 +13,11 @@
 // :116-137); each link's two CT sides move with it.
 //
 // Both orders are least-significant-key-first passes of a stable radix sort
-// (hipcub::DeviceRadixSort::SortPairs, 32-bit keys gathered through the
-// permutation built so far), so the result is the lexicographic order with
-// the old index as the last key.  Every array is then gathered once through
-// the permutation into a scratch buffer and copied back.
+// (32-bit keys gathered through the permutation built so far), so the result
+// is the lexicographic order with the old index as the last key.  Every
+// array is then gathered once through the permutation into a scratch buffer
+// and copied back.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <vector>
@@ -86,29 +85,136 @@ __global__ void __launch_bounds__(TPB) k_gather_sides(const double *__restrict__
     if (s < 2 * L) out[s] = in[2 * (int64_t)lo[s >> 1] + (s & 1)];
 }
 
+// ---- stable LSD radix sort of (32-bit key, index) pairs -----------------------
+// One pass per 8-bit digit: k_radix_hist counts the digits of each tile of
+// RADIX_TILE items (digit-major table hist[d * tiles + t]), an exclusive scan
+// of that table gives every (digit, tile) its first output slot, and
+// k_radix_scatter moves the tile's items there in input order: rounds of TPB
+// items, each wave ranking its lanes among equal digits by ballots, the
+// waves of a round in order, a running offset per digit across rounds.
+constexpr int RADIX = 256;
+constexpr int RADIX_ROUNDS = 16;
+constexpr int RADIX_TILE = TPB * RADIX_ROUNDS;
+static_assert(TPB == RADIX, "one thread per digit in the radix kernels");
+
+__device__ __forceinline__ uint32_t radix_digit(uint32_t key, int shift, uint32_t mask) {
+    return (key >> shift) & mask;
+}
+
+__global__ void __launch_bounds__(TPB) k_radix_hist(const uint32_t *__restrict__ key, int64_t n, int shift,
+                                                     uint32_t mask, int32_t *__restrict__ hist) {
+    __shared__ uint32_t cnt[RADIX];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RADIX_TILE;
+    for (int r = 0; r < RADIX_ROUNDS; r++) {
+        const int64_t i = base + (int64_t)r * TPB + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[radix_digit(key[i], shift, mask)], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = (int32_t)cnt[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(TPB) k_radix_scatter(const uint32_t *__restrict__ key,
+                                                        const int32_t *__restrict__ val, int64_t n, int shift,
+                                                        uint32_t mask, const int32_t *__restrict__ offsets,
+                                                        uint32_t *__restrict__ keyOut, int32_t *__restrict__ valOut) {
+    constexpr int W = TPB / 64;
+    __shared__ int32_t run[RADIX];     // next output slot of each digit
+    __shared__ int32_t wcnt[W][RADIX]; // per wave: digit count, then its first slot
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    run[threadIdx.x] = offsets[(int64_t)threadIdx.x * gridDim.x + blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * RADIX_TILE;
+    for (int r = 0; r < RADIX_ROUNDS; r++) {
+#pragma unroll
+        for (int k = 0; k < W; k++) wcnt[k][threadIdx.x] = 0;
+        __syncthreads();
+        const int64_t i = base + (int64_t)r * TPB + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t kv = valid ? key[i] : 0u;
+        const int32_t vv = valid ? val[i] : 0;
+        const uint32_t d = radix_digit(kv, shift, mask);
+        // lanes of this wave holding the same digit
+        uint64_t same = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            same &= bit ? bal : ~bal;
+        }
+        const int rank = __popcll(same & below);
+        if (valid && rank == 0) wcnt[w][d] = __popcll(same);
+        __syncthreads();
+        {
+            int32_t acc = run[threadIdx.x];
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                const int32_t cnt = wcnt[k][threadIdx.x];
+                wcnt[k][threadIdx.x] = acc;
+                acc += cnt;
+            }
+            run[threadIdx.x] = acc;
+        }
+        __syncthreads();
+        if (valid) {
+            const int32_t pos = wcnt[w][d] + rank;
+            keyOut[pos] = kv;
+            valOut[pos] = vv;
+        }
+        __syncthreads();
+    }
+}
+
 int key_bits(int64_t cells) {
     int b = 1;
     while (b < 32 && ((int64_t)1 << b) < cells) b++;
     return b;
 }
 
-// Stable sort of perm by key (both length n) into (keyOut, permOut) with
-// hipCUB's radix sort over the low `bits` bits; tmp is sized beforehand
-// (sort_temp_bytes) so no pass frees memory an earlier one still uses.
-hipError_t radix_pass(Context &c, DevBuf &tmp, const uint32_t *key, uint32_t *keyOut, const int32_t *perm,
-                      int32_t *permOut, int64_t n, int bits) {
-    size_t bytes = tmp.bytes;
-    return hipcub::DeviceRadixSort::SortPairs(tmp.ptr, bytes, key, keyOut, perm, permOut, (int)n, 0, bits, c.stream);
+int64_t radix_tiles(int64_t n) { return (n + RADIX_TILE - 1) / RADIX_TILE; }
+
+// Temporary storage of radix_pass over n items: a key and an index buffer
+// for the passes between input and output, the digit table and its scan.
+size_t radix_temp_bytes(int64_t n) {
+    const size_t table = (size_t)RADIX * (size_t)radix_tiles(n) + 1;
+    return 2 * sizeof(uint32_t) * (size_t)n + 2 * sizeof(int32_t) * table + 256;
 }
 
-// Grows *bytes to the temporary storage a pass over n items and `bits` bits needs.
-hipError_t sort_temp_bytes(Context &c, int64_t n, int bits, size_t *bytes) {
-    size_t b = 0;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                                                      (const int32_t *)nullptr, (int32_t *)nullptr, (int)n, 0, bits,
-                                                      c.stream);
-    if (b > *bytes) *bytes = b;
-    return e;
+// Stable sort of perm by the low `bits` bits of key (both length n) into
+// (keyOut, permOut); tmp holds radix_temp_bytes(n) (sized beforehand, so no
+// pass frees memory an earlier one still uses).
+hipError_t radix_pass(Context &c, DevBuf &tmp, const uint32_t *key, uint32_t *keyOut, const int32_t *perm,
+                      int32_t *permOut, int64_t n, int bits) {
+    if (n == 0) return hipSuccess;
+    if (n > INT32_MAX || bits < 1 || bits > 32 || tmp.bytes < radix_temp_bytes(n)) return hipErrorInvalidValue;
+    const int64_t tiles = radix_tiles(n);
+    const size_t table = (size_t)RADIX * (size_t)tiles + 1;
+    uint32_t *tKey = tmp.as<uint32_t>();
+    int32_t *tVal = reinterpret_cast<int32_t *>(tKey + n);
+    int32_t *hist = tVal + n;
+    int32_t *offs = hist + table;
+    const int passes = (bits + 7) / 8;
+    const uint32_t *srcKey = key;
+    const int32_t *srcVal = perm;
+    for (int ps = 0; ps < passes; ps++) {
+        const int shift = 8 * ps;
+        const int width = bits - shift < 8 ? bits - shift : 8;
+        const uint32_t mask = (1u << width) - 1u;
+        // the last pass writes the output; earlier ones alternate so that it does
+        const bool toOut = ((passes - 1 - ps) & 1) == 0;
+        uint32_t *dKey = toOut ? keyOut : tKey;
+        int32_t *dVal = toOut ? permOut : tVal;
+        hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(TPB), 0, c.stream, srcKey, n, shift, mask, hist);
+        hipError_t e = launch_exclusive_scan_i32(c, hist, offs, (int64_t)RADIX * tiles);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)tiles), dim3(TPB), 0, c.stream, srcKey, srcVal, n, shift,
+                           mask, offs, dKey, dVal);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        srcKey = dKey;
+        srcVal = dVal;
+    }
+    return hipSuccess;
 }
 
 // Gathers `b` (n elements of T) through perm and copies the result back;
@@ -157,10 +263,9 @@ hipError_t launch_reorder(Context &c, double searchDistance, int64_t *order, int
         const double cells = (double)dimX[lvl] * (double)dimY[lvl] * (double)(1 + (int32_t)ez);
         if (cells > (double)(1ll << 30)) return hipErrorInvalidValue;
         levelBits[lvl] = key_bits((int64_t)cells);
-        if ((e = sort_temp_bytes(c, n, levelBits[lvl], &tmpBytes)) != hipSuccess) return e;
     }
     const int linkBits = key_bits(n);
-    if (L > 0 && (e = sort_temp_bytes(c, L, linkBits, &tmpBytes)) != hipSuccess) return e;
+    tmpBytes = radix_temp_bytes(most);
     if ((e = tmp.ensure(tmpBytes)) != hipSuccess) return e;
     if ((e = scratch.ensure(sizeof(double) * (size_t)(2 * L > n ? 2 * L : n))) != hipSuccess) return e;
     // particles: passes on the colour at s, 2s, 4s, 8s (the last pass is
