@@ -269,7 +269,7 @@ int hdm_device_bytes(hdm_ctx *ctx, int64_t *bytes) {
     const hdm::DevBuf *bufs[] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz, &c.radius, &c.type, &c.la, &c.lb,
                                  &c.state, &c.initialLen, &c.maxLoad, &c.linkCount, &c.linkCursor, &c.slotLink,
                                  &c.slotOther, &c.sidePos, &c.nbrCount, &c.nbrLinked, &c.nbr, &c.colour, &c.order,
-                                 &c.cellStart, &c.cellCursor, &c.scanPartials, &c.red, &c.x0, &c.y0, &c.z0,
+                                 &c.cellStart, &c.cellCursor, &c.scanPartials, &c.red, &c.x0, &c.y0, &c.z0, &c.x0f, &c.y0f, &c.z0f,
                                  &c.skinPartials, &c.blockId, &c.blockWater, &c.probe, &c.rec, &c.recAlt,
                                  &c.drec, &c.drecAlt, &c.dtail, &c.dtailAlt, &c.slotThr};
     for (auto *b : bufs) t += b->bytes;
@@ -314,6 +314,7 @@ int hdm_particles_upload(hdm_ctx *ctx, const hdm_particles_host *p) {
         else HIP_CHECK(ctx, hipMemsetAsync(c.type.ptr, 0, (size_t)c.n, s));
     }
     for (hdm::DevBuf *b : {&c.x0, &c.y0, &c.z0}) HIP_CHECK(ctx, b->ensure(sizeof(double) * (size_t)(c.n > 0 ? c.n : 1)));
+    for (hdm::DevBuf *b : {&c.x0f, &c.y0f, &c.z0f}) HIP_CHECK(ctx, b->ensure(sizeof(float) * (size_t)(c.n > 0 ? c.n : 1)));
     HIP_CHECK(ctx, hipStreamSynchronize(s));
     c.grid.built = false;
     c.neighBuilt = false;
@@ -1070,7 +1071,7 @@ std::vector<unsigned char> graph_key(const Context &c, const hdm_run_params *rp,
     const hdm::DevBuf *bufs[] = {&c.x, &c.y, &c.z, &c.tx, &c.ty, &c.tz, &c.radius, &c.type, &c.rec, &c.recAlt,
                                  &c.drec, &c.drecAlt, &c.dtail, &c.dtailAlt, &c.nbrCount, &c.nbr, &c.linkCount, &c.slotLink, &c.slotOther,
                                  &c.state, &c.initialLen, &c.maxLoad, &c.slotThr, &c.skinPartials, &c.guardState,
-                                 &c.x0, &c.y0, &c.z0, &c.blockId, &c.blockWater};
+                                 &c.x0, &c.y0, &c.z0, &c.x0f, &c.y0f, &c.z0f, &c.blockId, &c.blockWater};
     for (auto *b : bufs) put(&b->ptr, sizeof(void *));
     for (int q = 0; q < 6; q++) {
         put(&c.diff[q].ptr, sizeof(void *));

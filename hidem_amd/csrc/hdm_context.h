@@ -186,6 +186,7 @@ struct Context {
     DevBuf skinPartials;   // f64 [workgroups][4]: rebuild-test maxima per integrating workgroup
     DevBuf guardState;     // u32 [GUARD_WORDS]: ticket, guard flag, executed-step counter
     DevBuf x0, y0, z0;     // positions at the last neighbour-list build
+    DevBuf x0f, y0f, z0f;  // the same rounded to FP32 (the device guard's first test)
     DevBuf probe;          // bandwidth-probe buffer (hdm_bw_probe)
     bool snapshotValid = false;
     bool skinFused = false;      // red[32..34] hold max+ Diff from the last integrate of hdm_run

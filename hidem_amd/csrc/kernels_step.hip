@@ -387,6 +387,7 @@ struct IntegArgs {
     uint32_t *guardOut, *stepsDone;
     double guardThr;
     const double *x0, *y0, *z0;   // nullptr: no displacement reduction
+    const float *x0f, *y0f, *z0f; // the snapshot rounded to FP32 (guard_trips)
     const int32_t *blockId;       // batched form: waterline of the particle's block
     const double *blockWater;     // (nullptr: ip.waterLevel for every particle)
     const MapDev *maps;           // GlacierDem map inputs, slot k valid when bit k of mapMask
@@ -874,6 +875,28 @@ constexpr int step_waves(bool occ) {
     return sizeof(R) == 4 ? (occ ? 5 : 4) : (occ ? 4 : 3);
 }
 
+// The device guard's test for particle p at its new position c: the exact
+// test 2 |c - x0| >= guardThr in FP64, decided first from the snapshot
+// rounded to FP32 (12 B instead of 24 B read per particle).  Rounding to
+// nearest moves a component by less than 2^-23 of its FP32 value (and by
+// less than 1e-30 below the normal range), so the FP64 distance lies within
+// e = (|x0f| + |y0f| + |z0f|) 2^-23 + 1e-30 of the distance d to the FP32
+// snapshot.  With a margin of 1e-12 (d + e) for the rounding of both
+// evaluations, d outside [thr/2 - e, thr/2 + e] decides the test as the FP64
+// evaluation would; only the rest (and non-finite values, which fail both
+// comparisons) read the FP64 snapshot.
+__device__ __forceinline__ bool guard_trips(const IntegArgs &g, int64_t p, const double c[3]) {
+    const float fx = ld_stream(g.x0f + p), fy = ld_stream(g.y0f + p), fz = ld_stream(g.z0f + p);
+    const double ax = c[0] - (double)fx, ay = c[1] - (double)fy, az = c[2] - (double)fz;
+    const double d = sqrt(ax * ax + ay * ay + az * az);
+    const double e = (fabs((double)fx) + fabs((double)fy) + fabs((double)fz)) * 0x1p-23 + 1e-30;
+    const double m = (d + e) * 1e-12;
+    if (2.0 * (d + e + m) < g.guardThr) return false;
+    if (d - e - m > 0.0 && 2.0 * (d - e - m) >= g.guardThr) return true;
+    const double ddx = c[0] - g.x0[p], ddy = c[1] - g.y0[p], ddz = c[2] - g.z0[p];
+    return 2.0 * sqrt(ddx * ddx + ddy * ddy + ddz * ddz) >= g.guardThr;
+}
+
 // One particle of the fused step (the body of k_step and k_step_loop):
 // break of the previous step, force, integration and the new records;
 // cordN / diffN receive the integrated coordinates and translation Diff, and
@@ -1038,10 +1061,7 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
         if ((threadIdx.x & 63) == 0 && broke) atomicAdd((unsigned long long *)f.brokenCount, (unsigned long long)broke);
     }
     if (g.guardOut) {
-        if (active) {
-            const double ddx = cordN[0] - g.x0[p], ddy = cordN[1] - g.y0[p], ddz = cordN[2] - g.z0[p];
-            if (2.0 * sqrt(ddx * ddx + ddy * ddy + ddz * ddz) >= g.guardThr) *g.guardOut = 1u;
-        }
+        if (active && guard_trips(g, p, cordN)) *g.guardOut = 1u;
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(g.stepsDone, 1u);
     } else if (g.partials) {
         skin_partials<BLOCK>(g, active, p, cordN, diffN);
@@ -1201,8 +1221,7 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(false)) k_step_loop(const
             if (p < a.f.n) {
                 double cordN[3], diffN[3];
                 step_particle<R, CT>(a.f, a.g, a.r, p, cordN, diffN, broke);
-                const double ddx = cordN[0] - a.g.x0[p], ddy = cordN[1] - a.g.y0[p], ddz = cordN[2] - a.g.z0[p];
-                trip |= 2.0 * sqrt(ddx * ddx + ddy * ddy + ddz * ddz) >= a.g.guardThr;
+                trip |= guard_trips(a.g, p, cordN);
             }
         }
         if (trip) *a.g.guardOut = 1u;
@@ -1364,6 +1383,23 @@ __global__ void __launch_bounds__(TPB) k_skin_heuristic(const double *__restrict
     }
 }
 
+// Rebuild snapshot: the positions in FP64 and rounded to FP32.
+__global__ void __launch_bounds__(TPB) k_snapshot(const double *__restrict__ x, const double *__restrict__ y,
+                                                   const double *__restrict__ z, int64_t n, double *__restrict__ x0,
+                                                   double *__restrict__ y0, double *__restrict__ z0,
+                                                   float *__restrict__ x0f, float *__restrict__ y0f,
+                                                   float *__restrict__ z0f) {
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    const double a = x[i], b = y[i], c = z[i];
+    x0[i] = a;
+    y0[i] = b;
+    z0[i] = c;
+    x0f[i] = (float)a;
+    y0f[i] = (float)b;
+    z0f[i] = (float)c;
+}
+
 // Exact test: max over particles of |x - x0|^2 (FP64).
 __global__ void __launch_bounds__(TPB) k_skin_exact(const double *__restrict__ x, const double *__restrict__ y,
                                                      const double *__restrict__ z, const double *__restrict__ x0,
@@ -1412,6 +1448,9 @@ IntegArgs integ_args(Context &c, const hdm_integ_params &ip, uint64_t *skin, boo
     a.x0 = withDisplacement ? c.x0.as<double>() : nullptr;
     a.y0 = withDisplacement ? c.y0.as<double>() : nullptr;
     a.z0 = withDisplacement ? c.z0.as<double>() : nullptr;
+    a.x0f = withDisplacement ? c.x0f.as<float>() : nullptr;
+    a.y0f = withDisplacement ? c.y0f.as<float>() : nullptr;
+    a.z0f = withDisplacement ? c.z0f.as<float>() : nullptr;
     a.blockId = c.nBlocks > 0 ? c.blockId.as<int32_t>() : nullptr;
     a.blockWater = c.nBlocks > 0 ? c.blockWater.as<double>() : nullptr;
     a.mapMask = ip.variant == HDM_INTEG_GLACIER ? c.mapMask : 0;
@@ -1872,11 +1911,11 @@ hipError_t launch_skin_exact(Context &c, uint64_t *out1) {
 
 hipError_t launch_snapshot(Context &c) {
     if (hipError_t se = sync_state(c); se != hipSuccess) return se;
-    size_t b = sizeof(double) * (size_t)c.n;
-    hipError_t e = hipMemcpyAsync(c.x0.ptr, c.x.ptr, b, hipMemcpyDeviceToDevice, c.stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c.y0.ptr, c.y.ptr, b, hipMemcpyDeviceToDevice, c.stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c.z0.ptr, c.z.ptr, b, hipMemcpyDeviceToDevice, c.stream);
-    return e;
+    if (c.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_snapshot, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(), c.y.as<double>(),
+                       c.z.as<double>(), c.n, c.x0.as<double>(), c.y0.as<double>(), c.z0.as<double>(),
+                       c.x0f.as<float>(), c.y0f.as<float>(), c.z0f.as<float>());
+    return hipGetLastError();
 }
 
 }  // namespace hdm
