@@ -180,6 +180,7 @@ struct Context {
     GridState grid;
     DevBuf colour, order, cellStart, cellCursor, scanPartials;
     DevBuf cellRec;        // double4 [n]: {x, y, z, radius} of order[i] (the list build's candidate records)
+    DevBuf cellRecF;       // float4 [n]: the same rounded to FP32 (the list build's first test)
     DevBuf cellPos;        // i32 [n]: position of each particle in order
     // reductions / scalars
     DevBuf red;            // u64 scratch for bbox / max reductions and counters

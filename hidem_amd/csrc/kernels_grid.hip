@@ -228,11 +228,14 @@ __global__ void __launch_bounds__(TPB) k_cell_sort(const int32_t *__restrict__ c
 __global__ void __launch_bounds__(TPB) k_cell_records(const double *__restrict__ x, const double *__restrict__ y,
                                                        const double *__restrict__ z, const double *__restrict__ radius,
                                                        const int32_t *__restrict__ order, int64_t n,
-                                                       double4 *__restrict__ rec, int32_t *__restrict__ pos) {
+                                                       double4 *__restrict__ rec, float4 *__restrict__ recF,
+                                                       int32_t *__restrict__ pos) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= n) return;
     const int32_t q = order[i];
-    rec[i] = make_double4(x[q], y[q], z[q], radius[q]);
+    const double4 r = make_double4(x[q], y[q], z[q], radius[q]);
+    rec[i] = r;
+    recF[i] = make_float4((float)r.x, (float)r.y, (float)r.z, (float)r.w);
     pos[q] = (int32_t)i;
 }
 
@@ -247,7 +250,7 @@ struct NeighArgs {
     const uint32_t *linkCount;
     const int32_t *slotOther;
     int32_t linkWidth;
-    int32_t partnersInLds;  // copy each lane's partners to LDS first (linkWidth * TPB words)
+    int32_t partnersInLds;  // stage each lane's partners' iteration positions in LDS, sorted (linkWidth * TPB words)
 };
 
 // Largest link-slot width whose partner lists are staged in LDS (32 KiB).
@@ -266,7 +269,8 @@ constexpr int NEIGH_BATCH = 4;
 // the first `width` entries to the wave-tiled list; every pass writes the
 // count and the skipped link partners.
 template <bool FILL>
-__global__ void __launch_bounds__(TPB) k_neigh(const double4 *__restrict__ crec, const int32_t *__restrict__ cpos,
+__global__ void __launch_bounds__(TPB) k_neigh(const double4 *__restrict__ crec, const float4 *__restrict__ crecF,
+                                                const int32_t *__restrict__ cpos,
                                                 int64_t n, const int32_t *__restrict__ colour,
                                                 const int32_t *__restrict__ cellStart,
                                                 const int32_t *__restrict__ order, NeighArgs a,
@@ -275,26 +279,47 @@ __global__ void __launch_bounds__(TPB) k_neigh(const double4 *__restrict__ crec,
     extern __shared__ int32_t ldsPartners[];  // [linkWidth][TPB]
     int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
     if (p >= n) return;
-    // p's link partners (list exclusion): from LDS when staged, else global
+    // p's link partners (list exclusion).  Staged in LDS: their positions
+    // in the iteration order, ascending.  The stencil rows are visited in
+    // ascending colour, so candidate positions ascend too and one pointer
+    // merges the two sequences (restarting should a position ever go back,
+    // as overlapping rows of a grid under 3 cells wide make it); otherwise
+    // the partners are compared by index from global memory.
     uint32_t pcnt = 0;
     const int64_t pbase = (p >> 6) * (int64_t)64 * a.linkWidth + (p & 63);
     if (a.linkCount) {
         pcnt = a.linkCount[p];
-        if (a.partnersInLds)
-            for (uint32_t k = 0; k < pcnt; k++)
-                ldsPartners[k * TPB + threadIdx.x] = a.slotOther[pbase + (int64_t)k * 64] & SLOT_LINK_MASK;
-    }
-    auto is_partner = [&](int32_t q) {
-        for (uint32_t k = 0; k < pcnt; k++) {
-            const int32_t v = a.partnersInLds ? ldsPartners[k * TPB + threadIdx.x]
-                                              : (a.slotOther[pbase + (int64_t)k * 64] & SLOT_LINK_MASK);
-            if (v == q) return true;
+        if (a.partnersInLds) {
+            for (uint32_t k = 0; k < pcnt; k++) {
+                const int32_t v = cpos[a.slotOther[pbase + (int64_t)k * 64] & SLOT_LINK_MASK];
+                uint32_t j = k;
+                while (j > 0 && ldsPartners[(j - 1) * TPB + threadIdx.x] > v) {
+                    ldsPartners[j * TPB + threadIdx.x] = ldsPartners[(j - 1) * TPB + threadIdx.x];
+                    j--;
+                }
+                ldsPartners[j * TPB + threadIdx.x] = v;
+            }
         }
+    }
+    uint32_t mergeAt = 0;
+    int32_t lastPos = -1;
+    auto partner_at = [&](int32_t ik) {
+        if (ik < lastPos) mergeAt = 0;
+        lastPos = ik;
+        while (mergeAt < pcnt && ldsPartners[mergeAt * TPB + threadIdx.x] < ik) mergeAt++;
+        return mergeAt < pcnt && ldsPartners[mergeAt * TPB + threadIdx.x] == ik;
+    };
+    auto is_partner = [&](int32_t q) {
+        for (uint32_t k = 0; k < pcnt; k++)
+            if ((a.slotOther[pbase + (int64_t)k * 64] & SLOT_LINK_MASK) == q) return true;
         return false;
     };
     const int32_t self = cpos[p];
     const double4 P = crec[self];
     const double xCord = P.x, yCord = P.y, zCord = P.z, rad = P.w;
+    const float4 Pf = crecF[self];
+    const float sumP = fabsf(Pf.x) + fabsf(Pf.y) + fabsf(Pf.z);
+    const float idf = (float)a.interactDist;
     const int64_t col = colour[p];
     int32_t neighbours = 0, skipped = 0;
     int32_t *out = nullptr;
@@ -310,26 +335,55 @@ __global__ void __launch_bounds__(TPB) k_neigh(const double4 *__restrict__ crec,
         // positions past its end), so a lane waits once per group rather
         // than once per candidate; tests and appends keep candidate order
         for (int32_t i = s; i < e; i += NEIGH_BATCH) {
-            double4 Q[NEIGH_BATCH];
+            float4 Q[NEIGH_BATCH];
 #pragma unroll
-            for (int k = 0; k < NEIGH_BATCH; k++) Q[k] = crec[i + k < e ? i + k : e - 1];
+            for (int k = 0; k < NEIGH_BATCH; k++) Q[k] = crecF[i + k < e ? i + k : e - 1];
 #pragma unroll
             for (int k = 0; k < NEIGH_BATCH; k++) {
                 const int32_t ik = i + k;
                 if (ik >= e || ik == self) continue;
-                double xDist = xCord - Q[k].x;
-                double xDist2 = xDist * xDist;
-                double yDist = yCord - Q[k].y;
-                double yDist2 = yDist * yDist;
-                double zDist = zCord - Q[k].z;
-                double zDist2 = zDist * zDist;
-                double particleDistance2 = xDist2 + yDist2 + zDist2;
-                double particleDistance = sqrt(particleDistance2);
-                double radSum = rad + Q[k].w;
-                double dist = particleDistance - radSum;
-                if (dist < a.interactDist) {
+                // FP32 first test.  Each FP32 record component is within
+                // 2^-24 (relative, round to nearest) of its FP64 value, and
+                // the FP32 differences, squares, sum, square root and
+                // subtractions add a few ulps of what they produce, so fd is
+                // within 2^-24 (sum |P| + sum |Q| + 7 fdist + 3 |frs|) of the
+                // FP64 dist and idf within 2^-24 |idf| of interactDist; E is
+                // eight times that (plus 1e-30 below the normal range).
+                // Only candidates within E of interactDist, and non-finite
+                // ones (which fail both comparisons), take the FP64 test.
+                const float fx = Pf.x - Q[k].x, fy = Pf.y - Q[k].y, fz = Pf.z - Q[k].z;
+                const float fdist2 = fx * fx + fy * fy + fz * fz;
+                const float fdist = sqrtf(fdist2);
+                const float frs = Pf.w + Q[k].w;
+                const float fd = fdist - frs;
+                const float E = (sumP + fabsf(Q[k].x) + fabsf(Q[k].y) + fabsf(Q[k].z) + 4.0f * fdist +
+                                 3.0f * fabsf(frs) + fabsf(idf)) * 0x1p-21f + 1e-30f;
+                bool near;
+                if (fd + E < idf) {
+                    near = true;
+                } else if (fd - E >= idf) {
+                    near = false;
+                } else {
+                    const double4 Qd = crec[ik];
+                    double xDist = xCord - Qd.x;
+                    double xDist2 = xDist * xDist;
+                    double yDist = yCord - Qd.y;
+                    double yDist2 = yDist * yDist;
+                    double zDist = zCord - Qd.z;
+                    double zDist2 = zDist * zDist;
+                    double particleDistance2 = xDist2 + yDist2 + zDist2;
+                    double particleDistance = sqrt(particleDistance2);
+                    double radSum = rad + Qd.w;
+                    double dist = particleDistance - radSum;
+                    near = dist < a.interactDist;
+                }
+                if (near) {
+                    if (pcnt && a.partnersInLds && partner_at(ik)) {
+                        skipped++;
+                        continue;
+                    }
                     const int32_t nInd = order[ik];
-                    if (pcnt && is_partner(nInd)) {
+                    if (pcnt && !a.partnersInLds && is_partner(nInd)) {
                         skipped++;
                         continue;
                     }
@@ -425,11 +479,12 @@ hipError_t launch_scatter_sort(Context &c) {
 hipError_t launch_cell_records(Context &c) {
     if (hipError_t se = sync_state(c); se != hipSuccess) return se;
     hipError_t e = c.cellRec.ensure(sizeof(double4) * (size_t)c.n);
+    if (e == hipSuccess) e = c.cellRecF.ensure(sizeof(float4) * (size_t)c.n);
     if (e == hipSuccess) e = c.cellPos.ensure(sizeof(int32_t) * (size_t)c.n);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_cell_records, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, c.x.as<double>(),
                        c.y.as<double>(), c.z.as<double>(), c.radius.as<double>(), c.order.as<int32_t>(), c.n,
-                       c.cellRec.as<double4>(), c.cellPos.as<int32_t>());
+                       c.cellRec.as<double4>(), c.cellRecF.as<float4>(), c.cellPos.as<int32_t>());
     c.grid.recEpoch = c.stateEpoch;
     c.grid.recTs = c.ts;
     return hipGetLastError();
@@ -459,7 +514,8 @@ static size_t neigh_lds(const NeighArgs &a) {
 hipError_t launch_neigh_count(Context &c, double interactDist) {
     NeighArgs a = neigh_args(c, interactDist);
     hipLaunchKernelGGL(k_neigh<false>, dim3(blocks_for(c.n)), dim3(TPB), neigh_lds(a), c.stream,
-                       c.cellRec.as<double4>(), c.cellPos.as<int32_t>(), c.n, c.colour.as<int32_t>(),
+                       c.cellRec.as<double4>(), c.cellRecF.as<float4>(), c.cellPos.as<int32_t>(), c.n,
+                       c.colour.as<int32_t>(),
                        c.cellStart.as<int32_t>(), c.order.as<int32_t>(), a, c.nbrCount.as<int32_t>(),
                        c.nbrLinked.as<int32_t>(), (int32_t *)nullptr);
     return hipGetLastError();
@@ -468,7 +524,8 @@ hipError_t launch_neigh_count(Context &c, double interactDist) {
 hipError_t launch_neigh_fill(Context &c, double interactDist) {
     NeighArgs a = neigh_args(c, interactDist);
     hipLaunchKernelGGL(k_neigh<true>, dim3(blocks_for(c.n)), dim3(TPB), neigh_lds(a), c.stream,
-                       c.cellRec.as<double4>(), c.cellPos.as<int32_t>(), c.n, c.colour.as<int32_t>(),
+                       c.cellRec.as<double4>(), c.cellRecF.as<float4>(), c.cellPos.as<int32_t>(), c.n,
+                       c.colour.as<int32_t>(),
                        c.cellStart.as<int32_t>(), c.order.as<int32_t>(), a, c.nbrCount.as<int32_t>(),
                        c.nbrLinked.as<int32_t>(), c.nbr.as<int32_t>());
     return hipGetLastError();

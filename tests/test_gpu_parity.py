@@ -123,6 +123,34 @@ def test_neighbours_bit_exact(case, request, orc):
 
 
 @pytest.mark.parametrize("case", ["block", "squeezed"])
+def test_mixed_list_is_oracle_list_without_partners(case, request, orc):
+    """The FP32 form's list (built after the links) is the oracle's list with
+    each particle's link partners taken out, in the oracle's order; the
+    number taken out per particle is the partners within the skin."""
+    s, p, l = request.getfixturevalue(case)
+    say("mixed list without partners", case)
+    with new_ctx(p, l, A.HDM_PREC_MIXED) as ctx:
+        ctx.links_build()
+        ctx.grid_build(s.search_distance)
+        ctx.neighbours_build(s.interact_dist)
+        assert ctx.exec_info()["merged_contacts"]
+        counts, starts, lst = ctx.neighbours_download()
+    oc, os_, ol = orc.neigh_build(p, s.search_distance, s.interact_dist)
+    a, b = np.asarray(l.a), np.asarray(l.b)
+    partners = [set() for _ in range(p.n)]
+    for x, y in zip(a.tolist(), b.tolist()):
+        partners[x].add(y)
+        partners[y].add(x)
+    removed = 0
+    for i in range(p.n):
+        want = [q for q in ol[os_[i]:os_[i] + oc[i]].tolist() if q not in partners[i]]
+        got = lst[starts[i]:starts[i] + counts[i]].tolist()
+        assert got == want, i
+        removed += int(oc[i]) - len(want)
+    assert removed > 0
+
+
+@pytest.mark.parametrize("case", ["block", "squeezed"])
 def test_linklists_bit_exact(case, request, orc):
     s, p, l = request.getfixturevalue(case)
     say("linklists", case)
