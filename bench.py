@@ -221,16 +221,21 @@ def probe_traffic(setup, p, l, forms):
     """Kernels of known size for calibrating the FETCH_SIZE / WRITE_SIZE
     counters, then fused step launches on the real workload."""
     for prec in forms:
+        # the headline FP32 form runs the learned policy's guarded step (here
+        # as per-step launches, so every dispatch is counted)
+        rp = A.HdmRunParams.from_buffer_copy(setup.run)
+        if prec == A.HDM_PREC_MIXED:
+            rp.policy = A.HDM_REBUILD_LEARNED
         with A.Context(prec) as ctx:
             ctx.upload(p, l)
             ctx.links_build()
             # the same autotuned launch shape the timed bench steps use
             ctx.set_exec(autotune=True)
-            tune_shape(ctx, setup.run)
+            tune_shape(ctx, rp)
             if prec == forms[0]:
                 g = ctx.bw_probe(A.HDM_PROBE_COPY, 1 << 30, 1)  # k_probe_copy: 512 MiB read + 512 MiB written
                 log(f"copy probe 512 MiB read + 512 MiB written: {g:.0f} GB/s")
-            ctx.run(setup.run, 3)
+            ctx.run(rp, 3)
             log(f"launch shape {ctx.exec_info()['name']}")
             ctx.sync()
             counts, _, _ = ctx.neighbours_download()
