@@ -1087,17 +1087,23 @@ __global__ void __launch_bounds__(TPB) k_break_recs(ForceArgs<R, R, R> f, StepRe
         else rotSelf = (float)r.drec[p * 8 + 6];
         const int32_t cnt = (int32_t)f.linkCount[p];
         const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + (p & 63);
+        // as in the fused step: the partner word one side ahead, the side's
+        // record and code issued together (also for a broken side, whose
+        // word still names a valid partner), one wait per side
+        int32_t owNext = cnt > 0 ? ld_stream(f.slotOther + base) : 0;
         for (int32_t k = 0; k < cnt; k++) {
             const int64_t sidx = base + (int64_t)k * 64;
-            const int32_t ow = ld_stream(f.slotOther + sidx);
-            if (ow < 0) continue;  // broken already
+            const int32_t ow = owNext;
             const int32_t q = ow & SLOT_LINK_MASK;
             const auto Q = r.rec[q];
             float rotOther;
             if constexpr (sizeof(R) == 4) rotOther = Q.rot;
             else rotOther = (float)r.drec[(int64_t)q * 8 + 6];
+            const uint16_t code = ld_stream(r.slotThr + sidx);
+            owNext = ld_stream(f.slotOther + (k + 1 < cnt ? sidx + 64 : sidx));
+            if (ow < 0) continue;  // broken already
             const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
-            const float thr = (float)ld_stream(r.slotThr + sidx) * r.thrScale;
+            const float thr = (float)code * r.thrScale;
             const float T = r.perLinkMaxLoad ? thr : thr * (1.0f + r.scalarMaxLoad);
             if (r.prefilter != 0 && certainly_intact(T, r.bendBreak, r.l0Max, dx, dy, dz, rotSelf + rotOther))
                 continue;
