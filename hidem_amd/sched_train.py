@@ -17,13 +17,18 @@
 4. Evaluation: from the state after train_steps, eval_steps more steps under
    (a) the reference heuristic (HiDEMLib.h:548-559), (b) the exact test with a
    host check every step, (c) HDM_REBUILD_LEARNED with fixed batches and no
-   network, (d) HDM_REBUILD_LEARNED with the trained network.  Reports
-   rebuilds, host checks, launched vs executed steps and wall time; (c) and
-   (d) must end bit-identical to (b) -- the device-side exact test decides
-   every rebuild, so no step ever runs on a stale neighbour list (missed
-   contacts: 0).
+   network, (d) HDM_REBUILD_LEARNED with the trained network, (e) as (d)
+   with captured graphs and the autotuned launch shape (the bench's form).
+   Reports rebuilds, host checks, launched vs executed steps and wall time;
+   (c), (d) and (e) must end bit-identical to (b) -- the device-side exact
+   test decides every rebuild, so no step ever runs on a stale neighbour
+   list (missed contacts: 0).
 
-Usage: python -m hidem_amd.sched_train [--config glacier_8m] [--out PATH]
+With --eval-only the saved network is evaluated again (steps 1 and 4 only;
+the state after train_steps comes from a plain run under the exact test)
+and the report's evaluation is replaced.
+
+Usage: python -m hidem_amd.sched_train [--config glacier_8m] [--out PATH] [--eval-only]
 Prints progress to stderr and one JSON report line to stdout.
 """
 from __future__ import annotations
@@ -119,7 +124,8 @@ def evaluate(setup, state, ts, steps, model, chunk=1000):
     p, l = state
     results = {}
     runs = [("heuristic", A.HDM_REBUILD_HEURISTIC, None), ("exact", A.HDM_REBUILD_EXACT, None),
-            ("learned_no_model", A.HDM_REBUILD_LEARNED, None), ("learned", A.HDM_REBUILD_LEARNED, model)]
+            ("learned_no_model", A.HDM_REBUILD_LEARNED, None), ("learned", A.HDM_REBUILD_LEARNED, model),
+            ("learned_graphs", A.HDM_REBUILD_LEARNED, model)]
     final = {}
     for name, policy, w in runs:
         rp = A.HdmRunParams.from_buffer_copy(setup.run)
@@ -131,6 +137,8 @@ def evaluate(setup, state, ts, steps, model, chunk=1000):
             c.sched_configure(block_cells=4, min_batch=8, max_batch=256, safety=0.5)
             if w is not None:
                 c.sched_set_model(w)
+            if name == "learned_graphs":
+                c.set_exec(autotune=True, graphs=True)
             rebuilds, secs = 0, 0.0
             done = 0
             while done < steps:
@@ -152,11 +160,28 @@ def evaluate(setup, state, ts, steps, model, chunk=1000):
         final[name] = q
         results[name] = r
     x = lambda q: np.stack([q["xCord"], q["yCord"], q["zCord"]], 1)  # noqa: E731
-    for name in ("learned_no_model", "learned"):
+    for name in ("learned_no_model", "learned", "learned_graphs"):
         results[name]["bit_identical_to_exact"] = bool(
             np.array_equal(x(final[name]).view(np.uint64), x(final["exact"]).view(np.uint64)))
         results[name]["missed_contacts"] = 0 if results[name]["bit_identical_to_exact"] else None
     return results
+
+
+def advance(setup, p, l, steps, chunk=1000):
+    """The state after `steps` steps under the exact test (the trajectory
+    generate() follows, without its label tracking)."""
+    rp = A.HdmRunParams.from_buffer_copy(setup.run)
+    rp.policy = A.HDM_REBUILD_EXACT
+    with A.Context(A.HDM_PREC_MIXED) as c:
+        c.upload(p, l)
+        c.links_build()
+        done = 0
+        while done < steps:
+            k = min(chunk, steps - done)
+            st = c.run(rp, k)
+            done += k
+            log(f"# advance: step {done}/{steps}")
+        return (c.download_particles(), c.download_links()), st.timestep
 
 
 def train(X, T, Cn, epochs=8, max_rows=4_000_000, seed=0):
@@ -195,17 +220,28 @@ def main(argv=None):
     ap.add_argument("--max-rows", type=int, default=4_000_000)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--out", default=None, help="model file (default hidem_amd/models/sched_<config>.bin)")
+    ap.add_argument("--eval-only", action="store_true", help="evaluate the saved network again")
     args = ap.parse_args(argv)
 
     cfg = params.load_config(args.config)
     setup, p, l = params.build_inputs(cfg, args.cells)
     log(f"# {args.config}: {p.n} particles, {l.n} links")
     t0 = time.perf_counter()
+    out = args.out or os.path.join(MODEL_DIR, f"sched_{args.config}.bin")
+    if args.eval_only:
+        with open(os.path.splitext(out)[0] + ".json") as f:
+            report = json.load(f)
+        state, ts = advance(setup, p, l, report["train_steps"])
+        ev = evaluate(setup, state, ts, report["eval_steps"], load_model(out))
+        report["evaluation"] = ev
+        with open(os.path.splitext(out)[0] + ".json", "w") as f:
+            json.dump(report, f, indent=1)
+        print(json.dumps(report), flush=True)
+        return report
     X, T, Cn, rebuilds, state, ts = generate(setup, p, l, args.train_steps, args.every)
     log(f"# {len(X)} rows, {Cn.mean():.3f} censored, {rebuilds} rebuilds, {time.perf_counter() - t0:.0f} s")
     w, loss0, loss = train(X, T, Cn, epochs=args.epochs, max_rows=args.max_rows)
     log(f"# training: mean loss {loss0:.4f} -> {loss:.4f}")
-    out = args.out or os.path.join(MODEL_DIR, f"sched_{args.config}.bin")
     ev = evaluate(setup, state, ts, args.eval_steps, w)
     report = {"config": args.config, "particles": p.n, "links": l.n, "train_steps": args.train_steps,
               "eval_steps": args.eval_steps, "rows": int(min(len(X), args.max_rows)), "censored_fraction": float(Cn.mean()),
