@@ -27,7 +27,10 @@ roofline   for the dominant kernel (k_step: link break of the previous step,
            launch (compulsory traffic of the arrays it must touch, see
            step_bytes) / its mean launch time from HIP events recorded on the
            context's stream during the timed steps, against 8 TB/s.  traffic
-           is the HBM bytes per launch measured with rocprofv3 counters.
+           is the HBM bytes per launch measured with rocprofv3 counters;
+           copy_probe_GBps the streaming copy probe on the same box, the
+           practical ceiling (traffic_over_copy_probe: the step's measured
+           traffic rate against it).
 error      each mixed form against the FP64 form after the window: position
            max|dx| / max|x|, force ||dF||/||F|| (translational, last step),
            broken-link count difference.
@@ -307,6 +310,13 @@ def main():
             f"shape {r['exec']['name']} (tuned in {r['tune_steps']} steps), rebuilds {r['stats'].rebuilds}, "
             f"broken {r['broken']}, forced rebuild {1e3 * r['rebuild_s']:.1f} ms, sched {r['sched']}")
 
+    # the practical HBM ceiling on this box: the streaming copy probe
+    # (k_probe_copy over 2 x 2 GiB, best of the timed passes), outside any
+    # timed window
+    with A.Context(A.HDM_PREC_MIXED) as ctx:
+        copy_gbs = max(ctx.bw_probe(A.HDM_PROBE_COPY, 4 << 30, 3) for _ in range(3))
+    log(f"copy probe: {copy_gbs:.0f} GB/s")
+
     errors = {}
     if "fp64" in results:
         for name in ("mixed", "mixed+learned"):
@@ -380,6 +390,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_over_algorithmic": ratio(traffic, h["step_bytes"]) if traffic else None,
                      "traffic_GBps": traffic / (h["step_avg_ms"] * 1e-3) / 1e9 if traffic else None,
+                     "copy_probe_GBps": copy_gbs,
+                     "traffic_over_copy_probe": (traffic / (h["step_avg_ms"] * 1e-3) / 1e9 / copy_gbs
+                                                 if traffic else None),
                      "kernel": "k_step (link break + force gather + integration)",
                      "bytes_per_launch": h["step_bytes"], "avg_launch_ms": h["step_avg_ms"]},
         "error_vs_fp64": errors.get(head) if head != "fp64" else None,

@@ -934,7 +934,7 @@ int hdm_bw_probe(hdm_ctx *ctx, int32_t kind, int64_t bytes, int32_t iters, doubl
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     // copy: bytes/2 read + bytes/2 written; rmw6: bytes read + bytes written
-    double moved = (kind == HDM_PROBE_COPY) ? (double)(bytes / 2 / 32 * 32) * 2.0 : (double)(bytes / 48 * 48) * 2.0;
+    double moved = (kind == HDM_PROBE_COPY) ? (double)(bytes / 2 / 16 * 16) * 2.0 : (double)(bytes / 48 * 48) * 2.0;
     if (gbps) *gbps = moved * iters / ((double)ms * 1e-3) / 1e9;
     return HDM_OK;
 }
