@@ -250,11 +250,11 @@ struct NeighArgs {
     const uint32_t *linkCount;
     const int32_t *slotOther;
     int32_t linkWidth;
-    int32_t partnersInLds;  // stage each lane's partners' iteration positions in LDS, sorted (linkWidth * TPB words)
 };
 
-// Largest link-slot width whose partner lists are staged in LDS (32 KiB).
-constexpr int32_t NEIGH_LDS_WIDTH = 32;
+// Longest candidate stream whose link partners are marked in a bit mask
+// (longer streams compare each near candidate with the partners instead).
+constexpr int32_t NEIGH_MASK_BITS = 128;  // two 64-bit words
 // Candidate records a lane loads together in the list build.
 constexpr int NEIGH_BATCH = 4;
 
@@ -276,92 +276,140 @@ __global__ void __launch_bounds__(TPB) k_neigh(const double4 *__restrict__ crec,
                                                 const int32_t *__restrict__ order, NeighArgs a,
                                                 int32_t *__restrict__ count, int32_t *__restrict__ linked,
                                                 int32_t *__restrict__ list) {
-    extern __shared__ int32_t ldsPartners[];  // [linkWidth][TPB]
     int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
     if (p >= n) return;
-    // p's link partners (list exclusion).  Staged in LDS: their positions
-    // in the iteration order, ascending.  The stencil rows are visited in
-    // ascending colour, so candidate positions ascend too and one pointer
-    // merges the two sequences (restarting should a position ever go back,
-    // as overlapping rows of a grid under 3 cells wide make it); otherwise
-    // the partners are compared by index from global memory.
+    const int32_t self = cpos[p];
+    const double4 P = crec[self];
+    const double xCord = P.x, yCord = P.y, zCord = P.z, rad = P.w;
+    const float4 Pf = crecF[self];
+    const float idf = (float)a.interactDist;
+    const float Rp = idf + Pf.w;
+    const float Cp = fabsf(Pf.x) + fabsf(Pf.y) + fabsf(Pf.z) + 4.0f * (fabsf(idf) + fabsf(Pf.w));
+    const int64_t col = colour[p];
+    // The 9 stencil rows as runs [s, s + len) of the iteration order, all
+    // bounds loaded at once.  Each lane walks its rows as one stream,
+    // NEIGH_BATCH candidates per step: a step takes what is left of the
+    // current row and fills up from the next one, so a wave steps about
+    // sum(len) / NEIGH_BATCH times for its longest lane rather than the sum
+    // over rows of each row's longest lane.  Candidate order is unchanged.
+    __shared__ int2 rowRun[9][TPB];
+    int32_t rs[9], rl[9];
+#pragma unroll
+    for (int row = 0; row < 9; ++row) {
+        int64_t lo = col + a.rowOffsets[row] - 1, hi = col + a.rowOffsets[row] + 1;
+        lo = lo < 0 ? 0 : lo;
+        hi = hi >= a.totalCells ? a.totalCells - 1 : hi;
+        const bool ok = lo <= hi;
+        rs[row] = cellStart[ok ? lo : 0];
+        rl[row] = ok ? cellStart[hi + 1] - rs[row] : 0;
+    }
+#pragma unroll
+    for (int row = 0; row < 9; ++row) rowRun[row][threadIdx.x] = make_int2(rs[row], rl[row]);
+    // p's link partners (list exclusion): each partner's iteration position
+    // is found in the runs and its index in the candidate stream marked in
+    // a bit mask (once per run holding it, as a position can recur when rows
+    // of a grid under 3 cells wide overlap); a near candidate is a partner
+    // when its stream index is marked.  Streams longer than the mask compare
+    // near candidates with the partners' indices from global memory.
     uint32_t pcnt = 0;
+    uint64_t pmaskLo = 0, pmaskHi = 0;  // stream indices 0-63, 64-127
+    int32_t streamLen = 0;
+#pragma unroll
+    for (int row = 0; row < 9; ++row) streamLen += rl[row];
+    const bool useMask = streamLen <= NEIGH_MASK_BITS;
     const int64_t pbase = (p >> 6) * (int64_t)64 * a.linkWidth + (p & 63);
     if (a.linkCount) {
         pcnt = a.linkCount[p];
-        if (a.partnersInLds) {
-            for (uint32_t k = 0; k < pcnt; k++) {
-                const int32_t v = cpos[a.slotOther[pbase + (int64_t)k * 64] & SLOT_LINK_MASK];
-                uint32_t j = k;
-                while (j > 0 && ldsPartners[(j - 1) * TPB + threadIdx.x] > v) {
-                    ldsPartners[j * TPB + threadIdx.x] = ldsPartners[(j - 1) * TPB + threadIdx.x];
-                    j--;
+        if (useMask) {
+            // four partners at a time: their slot words, then their
+            // positions, are loaded together
+            for (uint32_t k0 = 0; k0 < pcnt; k0 += 4) {
+                int32_t w[4], v[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    w[t] = a.slotOther[pbase + (int64_t)(k0 + t < pcnt ? k0 + t : k0) * 64] & SLOT_LINK_MASK;
+#pragma unroll
+                for (int t = 0; t < 4; t++) v[t] = cpos[w[t]];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    if (k0 + t >= pcnt) break;
+                    int32_t pre = 0;
+#pragma unroll
+                    for (int row = 0; row < 9; ++row) {
+                        const uint32_t off = (uint32_t)(v[t] - rs[row]);
+                        if (off < (uint32_t)rl[row]) {
+                            const uint32_t j = (uint32_t)pre + off;
+                            if (j < 64)
+                                pmaskLo |= 1ull << j;
+                            else
+                                pmaskHi |= 1ull << (j - 64);
+                        }
+                        pre += rl[row];
+                    }
                 }
-                ldsPartners[j * TPB + threadIdx.x] = v;
             }
         }
     }
-    uint32_t mergeAt = 0;
-    int32_t lastPos = -1;
-    auto partner_at = [&](int32_t ik) {
-        if (ik < lastPos) mergeAt = 0;
-        lastPos = ik;
-        while (mergeAt < pcnt && ldsPartners[mergeAt * TPB + threadIdx.x] < ik) mergeAt++;
-        return mergeAt < pcnt && ldsPartners[mergeAt * TPB + threadIdx.x] == ik;
-    };
     auto is_partner = [&](int32_t q) {
         for (uint32_t k = 0; k < pcnt; k++)
             if ((a.slotOther[pbase + (int64_t)k * 64] & SLOT_LINK_MASK) == q) return true;
         return false;
     };
-    const int32_t self = cpos[p];
-    const double4 P = crec[self];
-    const double xCord = P.x, yCord = P.y, zCord = P.z, rad = P.w;
-    const float4 Pf = crecF[self];
-    const float sumP = fabsf(Pf.x) + fabsf(Pf.y) + fabsf(Pf.z);
-    const float idf = (float)a.interactDist;
-    const int64_t col = colour[p];
     int32_t neighbours = 0, skipped = 0;
     int32_t *out = nullptr;
     if (FILL) out = list + (p >> 6) * (int64_t)64 * a.width + (p & 63);
-    for (int row = 0; row < 9; ++row) {
-        int64_t lo = col + a.rowOffsets[row] - 1, hi = col + a.rowOffsets[row] + 1;
-        lo = lo < 0 ? 0 : lo;
-        hi = hi >= a.totalCells ? a.totalCells - 1 : hi;
-        if (lo > hi) continue;
-        const int32_t s = cellStart[lo], e = cellStart[hi + 1];
-        // candidates in groups of NEIGH_BATCH: the group's records are
-        // loaded together (the last candidate of the row stands in for
-        // positions past its end), so a lane waits once per group rather
-        // than once per candidate; tests and appends keep candidate order
-        for (int32_t i = s; i < e; i += NEIGH_BATCH) {
-            float4 Q[NEIGH_BATCH];
+    int row = 0;
+    int32_t j0 = 0;  // stream index of the step's first candidate
+    int2 cur = rowRun[0][threadIdx.x], nxt = rowRun[1][threadIdx.x];
+    while (row < 9) {
+        const int32_t fromCur = cur.y < NEIGH_BATCH ? cur.y : NEIGH_BATCH;
+        const int32_t room = NEIGH_BATCH - fromCur;
+        const int32_t fromNext = nxt.y < room ? nxt.y : room;
+        int32_t cand[NEIGH_BATCH];
 #pragma unroll
-            for (int k = 0; k < NEIGH_BATCH; k++) Q[k] = crecF[i + k < e ? i + k : e - 1];
+        for (int k = 0; k < NEIGH_BATCH; k++) {
+            const int32_t c = k < fromCur ? cur.x + k : nxt.x + (k - fromCur);
+            cand[k] = k < fromCur + fromNext ? c : -1;
+        }
+        {
+            float4 Q[NEIGH_BATCH];
+            int32_t idx[NEIGH_BATCH];  // particle indices, loaded with the records (no wait when appending)
+#pragma unroll
+            for (int k = 0; k < NEIGH_BATCH; k++) Q[k] = crecF[cand[k] >= 0 ? cand[k] : self];
+#pragma unroll
+            for (int k = 0; k < NEIGH_BATCH; k++) idx[k] = order[cand[k] >= 0 ? cand[k] : self];
 #pragma unroll
             for (int k = 0; k < NEIGH_BATCH; k++) {
-                const int32_t ik = i + k;
-                if (ik >= e || ik == self) continue;
-                // FP32 first test.  Each FP32 record component is within
-                // 2^-24 (relative, round to nearest) of its FP64 value, and
-                // the FP32 differences, squares, sum, square root and
-                // subtractions add a few ulps of what they produce, so fd is
-                // within 2^-24 (sum |P| + sum |Q| + 7 fdist + 3 |frs|) of the
-                // FP64 dist and idf within 2^-24 |idf| of interactDist; E is
-                // eight times that (plus 1e-30 below the normal range).
-                // Only candidates within E of interactDist, and non-finite
-                // ones (which fail both comparisons), take the FP64 test.
+                const int32_t ik = cand[k];
+                if (ik < 0 || ik == self) continue;
+                // FP32 first test on squared values: with a = Pf - Qf, s2 =
+                // |a|^2 (two FMAs and a product) and T = fl(fl(idf + Pf.w) +
+                // Qf.w), the pair is near when T - E > 0 and s2 < (T - E)^2,
+                // far when s2 >= (T + E)^2 (finite), and otherwise takes the
+                // FP64 test.  In units of u = 2^-24: each record component is
+                // within u of its FP64 value (plus 2^-150 below the normal
+                // range), so |a - (P - Q)| <= u (sum|P| + sum|Q|) + sqrt(3) u
+                // |a|; s2 is within 3.01 u of |a|^2, so sqrt(s2) within 1.51 u
+                // of |a|; T is within 2.01 u (|idf| + |Pw| + |Qw|) + u |T| of
+                // interactDist + r_P + r_Q; rounding the squared bounds adds u
+                // more.  The FP32 decision is the FP64 one whenever E >= u
+                // (sum|P| + sum|Q| + 5.9 |T| + 2.01 (|idf| + |Pw| + |Qw|)),
+                // plus 2^-74 for underflowed squares (the FP64 evaluation's own
+                // roundings are within 2^-50 of its operands).  E is four times
+                // that plus 1e-20.  Non-finite or overflowing values make E, the
+                // bounds or the comparisons non-finite or false, so those
+                // candidates take the FP64 test.
                 const float fx = Pf.x - Q[k].x, fy = Pf.y - Q[k].y, fz = Pf.z - Q[k].z;
-                const float fdist2 = fx * fx + fy * fy + fz * fz;
-                const float fdist = sqrtf(fdist2);
-                const float frs = Pf.w + Q[k].w;
-                const float fd = fdist - frs;
-                const float E = (sumP + fabsf(Q[k].x) + fabsf(Q[k].y) + fabsf(Q[k].z) + 4.0f * fdist +
-                                 3.0f * fabsf(frs) + fabsf(idf)) * 0x1p-21f + 1e-30f;
+                const float s2 = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
+                const float T = Rp + Q[k].w;
+                const float E = __builtin_fmaf(Cp + fabsf(Q[k].x) + fabsf(Q[k].y) + fabsf(Q[k].z) +
+                                                   4.0f * fabsf(Q[k].w) + 8.0f * fabsf(T),
+                                               0x1p-22f, 1e-20f);
+                const float tlo = T - E, thi = T + E;
                 bool near;
-                if (fd + E < idf) {
+                if (tlo > 0.0f && s2 < tlo * tlo) {
                     near = true;
-                } else if (fd - E >= idf) {
+                } else if (s2 >= thi * thi && thi * thi <= 3.4028234663852886e38f) {
                     near = false;
                 } else {
                     const double4 Qd = crec[ik];
@@ -378,21 +426,34 @@ __global__ void __launch_bounds__(TPB) k_neigh(const double4 *__restrict__ crec,
                     near = dist < a.interactDist;
                 }
                 if (near) {
-                    if (pcnt && a.partnersInLds && partner_at(ik)) {
-                        skipped++;
-                        continue;
-                    }
-                    const int32_t nInd = order[ik];
-                    if (pcnt && !a.partnersInLds && is_partner(nInd)) {
-                        skipped++;
-                        continue;
+                    if (pcnt) {
+                        bool partner;
+                        if (useMask) {
+                            const uint32_t j = (uint32_t)(j0 + k);
+                            partner = ((j < 64 ? pmaskLo >> j : pmaskHi >> (j - 64)) & 1) != 0;
+                        } else {
+                            partner = is_partner(idx[k]);
+                        }
+                        if (partner) {
+                            skipped++;
+                            continue;
+                        }
                     }
                     if (FILL) {
-                        if (neighbours < a.width) out[(int64_t)neighbours * 64] = nInd;
+                        if (neighbours < a.width) out[(int64_t)neighbours * 64] = idx[k];
                     }
                     neighbours++;
                 }
             }
+        }
+        j0 += fromCur + fromNext;
+        if (cur.y > NEIGH_BATCH) {
+            cur.x += NEIGH_BATCH;
+            cur.y -= NEIGH_BATCH;
+        } else {
+            cur = make_int2(nxt.x + fromNext, nxt.y - fromNext);
+            ++row;
+            nxt = row + 1 < 9 ? rowRun[row + 1][threadIdx.x] : make_int2(0, 0);
         }
     }
     count[p] = neighbours;
@@ -503,17 +564,12 @@ static NeighArgs neigh_args(Context &c, double interactDist) {
     a.linkCount = exclude ? c.linkCount.as<uint32_t>() : nullptr;
     a.slotOther = exclude ? c.slotOther.as<int32_t>() : nullptr;
     a.linkWidth = c.maxLinks;
-    a.partnersInLds = exclude && c.maxLinks <= NEIGH_LDS_WIDTH ? 1 : 0;
     return a;
-}
-
-static size_t neigh_lds(const NeighArgs &a) {
-    return a.partnersInLds ? sizeof(int32_t) * (size_t)a.linkWidth * TPB : 0;
 }
 
 hipError_t launch_neigh_count(Context &c, double interactDist) {
     NeighArgs a = neigh_args(c, interactDist);
-    hipLaunchKernelGGL(k_neigh<false>, dim3(blocks_for(c.n)), dim3(TPB), neigh_lds(a), c.stream,
+    hipLaunchKernelGGL(k_neigh<false>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream,
                        c.cellRec.as<double4>(), c.cellRecF.as<float4>(), c.cellPos.as<int32_t>(), c.n,
                        c.colour.as<int32_t>(),
                        c.cellStart.as<int32_t>(), c.order.as<int32_t>(), a, c.nbrCount.as<int32_t>(),
@@ -523,7 +579,7 @@ hipError_t launch_neigh_count(Context &c, double interactDist) {
 
 hipError_t launch_neigh_fill(Context &c, double interactDist) {
     NeighArgs a = neigh_args(c, interactDist);
-    hipLaunchKernelGGL(k_neigh<true>, dim3(blocks_for(c.n)), dim3(TPB), neigh_lds(a), c.stream,
+    hipLaunchKernelGGL(k_neigh<true>, dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream,
                        c.cellRec.as<double4>(), c.cellRecF.as<float4>(), c.cellPos.as<int32_t>(), c.n,
                        c.colour.as<int32_t>(),
                        c.cellStart.as<int32_t>(), c.order.as<int32_t>(), a, c.nbrCount.as<int32_t>(),
