@@ -5,7 +5,9 @@ Bar: grid cells, iteration order, neighbour lists and link lists bit-exact;
 FP64 forces, CT, positions bit-identical (the kernels keep the oracle's
 operation order, no FMA contraction, correctly rounded sqrt/div) and in any
 case within 1e-12 relative; the broken-link set over 500 steps of the 16k
-block identical step by step.
+block identical step by step.  Inputs: the 16k block, the block squeezed into
+contact, and a slab one unit cell thick whose one-cell-wide grid makes the
+reference's stencil visit cells repeatedly (repeated list entries).
 """
 import numpy as np
 import pytest
@@ -51,6 +53,16 @@ def squeezed():
     return params.build_inputs(cfg)
 
 
+@pytest.fixture(scope="module")
+def thin():
+    """A slab one FCC unit cell thick in x (576 particles): the grid is one
+    cell wide, so the reference's 27 linear offsets visit cells more than
+    once and its lists hold repeated entries (6,688 of 11,012)."""
+    cfg = params.load_config("iceblock_16k")
+    cfg["hdm"]["cells"] = [1, 12, 12]
+    return params.build_inputs(cfg)
+
+
 def new_ctx(p, l, prec=A.HDM_PREC_FP64):
     ctx = A.Context(prec)
     ctx.upload(p, l)
@@ -72,7 +84,7 @@ def bits_equal(a, b):
 
 
 # ---- grid / lists -------------------------------------------------------------
-@pytest.mark.parametrize("case", ["block", "squeezed"])
+@pytest.mark.parametrize("case", ["block", "squeezed", "thin"])
 def test_grid_bit_exact(case, request, orc):
     s, p, l = request.getfixturevalue(case)
     say("grid", case, p.n)
@@ -108,7 +120,7 @@ def test_grid_reference_cases(orc):
     assert np.array_equal(order, g["order"])
 
 
-@pytest.mark.parametrize("case", ["block", "squeezed"])
+@pytest.mark.parametrize("case", ["block", "squeezed", "thin"])
 def test_neighbours_bit_exact(case, request, orc):
     s, p, l = request.getfixturevalue(case)
     say("neighbours", case)
@@ -122,20 +134,24 @@ def test_neighbours_bit_exact(case, request, orc):
     assert np.array_equal(lst, ol)
 
 
-@pytest.mark.parametrize("case", ["block", "squeezed"])
-def test_mixed_list_is_oracle_list_without_partners(case, request, orc):
+@pytest.mark.parametrize("case,cell_scale", [("block", 1.0), ("squeezed", 1.0), ("block", 3.0), ("thin", 1.0)])
+def test_mixed_list_is_oracle_list_without_partners(case, cell_scale, request, orc):
     """The FP32 form's list (built after the links) is the oracle's list with
     each particle's link partners taken out, in the oracle's order; the
-    number taken out per particle is the partners within the skin."""
+    number taken out per particle is the partners within the skin.  Grid
+    cells three times wider give every particle well over 128 candidates,
+    past the list build's partner bit mask, so its fallback (partner indices
+    compared one by one) is covered too."""
     s, p, l = request.getfixturevalue(case)
-    say("mixed list without partners", case)
+    search = s.search_distance * cell_scale
+    say("mixed list without partners", case, "cells x", cell_scale)
     with new_ctx(p, l, A.HDM_PREC_MIXED) as ctx:
         ctx.links_build()
-        ctx.grid_build(s.search_distance)
+        ctx.grid_build(search)
         ctx.neighbours_build(s.interact_dist)
         assert ctx.exec_info()["merged_contacts"]
         counts, starts, lst = ctx.neighbours_download()
-    oc, os_, ol = orc.neigh_build(p, s.search_distance, s.interact_dist)
+    oc, os_, ol = orc.neigh_build(p, search, s.interact_dist)
     a, b = np.asarray(l.a), np.asarray(l.b)
     partners = [set() for _ in range(p.n)]
     for x, y in zip(a.tolist(), b.tolist()):
@@ -251,7 +267,7 @@ def test_force_fp64_parity(case, request, orc):
     assert nz.mean() > 0.9
 
 
-@pytest.mark.parametrize("case", ["block", "squeezed"])
+@pytest.mark.parametrize("case", ["block", "squeezed", "thin"])
 def test_force_fp64_bit_identical(case, request, orc):
     s, p0, l0 = request.getfixturevalue(case)
     p = perturb(p0, 21)
