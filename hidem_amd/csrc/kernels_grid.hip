@@ -269,7 +269,7 @@ constexpr int NEIGH_BATCH = 4;
 // the first `width` entries to the wave-tiled list; every pass writes the
 // count and the skipped link partners.
 template <bool FILL>
-__global__ void __launch_bounds__(TPB) k_neigh(const double4 *__restrict__ crec, const float4 *__restrict__ crecF,
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(5))) k_neigh(const double4 *__restrict__ crec, const float4 *__restrict__ crecF,
                                                 const int32_t *__restrict__ cpos,
                                                 int64_t n, const int32_t *__restrict__ colour,
                                                 const int32_t *__restrict__ cellStart,
@@ -359,25 +359,58 @@ __global__ void __launch_bounds__(TPB) k_neigh(const double4 *__restrict__ crec,
     int32_t *out = nullptr;
     if (FILL) out = list + (p >> 6) * (int64_t)64 * a.width + (p & 63);
     int row = 0;
-    int32_t j0 = 0;  // stream index of the step's first candidate
+    int32_t jNext = 0;  // stream index of the next step's first candidate
     int2 cur = rowRun[0][threadIdx.x], nxt = rowRun[1][threadIdx.x];
-    while (row < 9) {
+    // One step of the stream: up to NEIGH_BATCH candidates (-1: none) and
+    // the stream index of the first; false once the runs are used up.
+    auto take = [&](int32_t (&cand)[NEIGH_BATCH], int32_t &jStart) {
+        jStart = jNext;
+        if (row >= 9) {
+#pragma unroll
+            for (int k = 0; k < NEIGH_BATCH; k++) cand[k] = -1;
+            return false;
+        }
         const int32_t fromCur = cur.y < NEIGH_BATCH ? cur.y : NEIGH_BATCH;
         const int32_t room = NEIGH_BATCH - fromCur;
         const int32_t fromNext = nxt.y < room ? nxt.y : room;
-        int32_t cand[NEIGH_BATCH];
 #pragma unroll
         for (int k = 0; k < NEIGH_BATCH; k++) {
             const int32_t c = k < fromCur ? cur.x + k : nxt.x + (k - fromCur);
             cand[k] = k < fromCur + fromNext ? c : -1;
         }
+        jNext += fromCur + fromNext;
+        if (cur.y > NEIGH_BATCH) {
+            cur.x += NEIGH_BATCH;
+            cur.y -= NEIGH_BATCH;
+        } else {
+            cur = make_int2(nxt.x + fromNext, nxt.y - fromNext);
+            ++row;
+            nxt = row + 1 < 9 ? rowRun[row + 1][threadIdx.x] : make_int2(0, 0);
+        }
+        return true;
+    };
+    // Software-pipelined: a step's records and indices are loaded while the
+    // previous step is tested, so the wait at a step is for loads issued a
+    // step earlier (and the list stores, which gfx9 counts with the loads,
+    // come after them).
+    int32_t cand[NEIGH_BATCH], j0;
+    bool live = take(cand, j0);
+    float4 Q[NEIGH_BATCH];
+    int32_t idx[NEIGH_BATCH];  // particle indices, loaded with the records (no wait when appending)
+#pragma unroll
+    for (int k = 0; k < NEIGH_BATCH; k++) Q[k] = crecF[cand[k] >= 0 ? cand[k] : self];
+#pragma unroll
+    for (int k = 0; k < NEIGH_BATCH; k++) idx[k] = order[cand[k] >= 0 ? cand[k] : self];
+    while (live) {
+        int32_t candN[NEIGH_BATCH], jN;
+        const bool liveN = take(candN, jN);
+        float4 QN[NEIGH_BATCH];
+        int32_t idxN[NEIGH_BATCH];
+#pragma unroll
+        for (int k = 0; k < NEIGH_BATCH; k++) QN[k] = crecF[candN[k] >= 0 ? candN[k] : self];
+#pragma unroll
+        for (int k = 0; k < NEIGH_BATCH; k++) idxN[k] = order[candN[k] >= 0 ? candN[k] : self];
         {
-            float4 Q[NEIGH_BATCH];
-            int32_t idx[NEIGH_BATCH];  // particle indices, loaded with the records (no wait when appending)
-#pragma unroll
-            for (int k = 0; k < NEIGH_BATCH; k++) Q[k] = crecF[cand[k] >= 0 ? cand[k] : self];
-#pragma unroll
-            for (int k = 0; k < NEIGH_BATCH; k++) idx[k] = order[cand[k] >= 0 ? cand[k] : self];
 #pragma unroll
             for (int k = 0; k < NEIGH_BATCH; k++) {
                 const int32_t ik = cand[k];
@@ -446,15 +479,14 @@ __global__ void __launch_bounds__(TPB) k_neigh(const double4 *__restrict__ crec,
                 }
             }
         }
-        j0 += fromCur + fromNext;
-        if (cur.y > NEIGH_BATCH) {
-            cur.x += NEIGH_BATCH;
-            cur.y -= NEIGH_BATCH;
-        } else {
-            cur = make_int2(nxt.x + fromNext, nxt.y - fromNext);
-            ++row;
-            nxt = row + 1 < 9 ? rowRun[row + 1][threadIdx.x] : make_int2(0, 0);
+#pragma unroll
+        for (int k = 0; k < NEIGH_BATCH; k++) {
+            cand[k] = candN[k];
+            Q[k] = QN[k];
+            idx[k] = idxN[k];
         }
+        j0 = jN;
+        live = liveN;
     }
     count[p] = neighbours;
     linked[p] = skipped;
