@@ -1087,26 +1087,15 @@ __global__ void __launch_bounds__(TPB) k_break_recs(ForceArgs<R, R, R> f, StepRe
         else rotSelf = (float)r.drec[p * 8 + 6];
         const int32_t cnt = (int32_t)f.linkCount[p];
         const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + (p & 63);
-        // as in the fused step: the partner word one side ahead, the side's
-        // record and code issued together (also for a broken side, whose
-        // word still names a valid partner), one wait per side
-        int32_t owNext = cnt > 0 ? ld_stream(f.slotOther + base) : 0;
-        for (int32_t k = 0; k < cnt; k++) {
-            const int64_t sidx = base + (int64_t)k * 64;
-            const int32_t ow = owNext;
+        // one intact side: prefilter, then the exact test where it cannot
+        // prove the link intact
+        auto side = [&](int64_t sidx, int32_t ow, double Qx, double Qy, double Qz, float rotOther, uint16_t code) {
             const int32_t q = ow & SLOT_LINK_MASK;
-            const auto Q = r.rec[q];
-            float rotOther;
-            if constexpr (sizeof(R) == 4) rotOther = Q.rot;
-            else rotOther = (float)r.drec[(int64_t)q * 8 + 6];
-            const uint16_t code = ld_stream(r.slotThr + sidx);
-            owNext = ld_stream(f.slotOther + (k + 1 < cnt ? sidx + 64 : sidx));
-            if (ow < 0) continue;  // broken already
-            const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+            const double dx = P.x - Qx, dy = P.y - Qy, dz = P.z - Qz;
             const float thr = (float)code * r.thrScale;
             const float T = r.perLinkMaxLoad ? thr : thr * (1.0f + r.scalarMaxLoad);
             if (r.prefilter != 0 && certainly_intact(T, r.bendBreak, r.l0Max, dx, dy, dz, rotSelf + rotOther))
-                continue;
+                return;
             const int32_t word = f.slotLink[sidx];
             const double theta[3] = {f.tx[p], f.ty[p], f.tz[p]};
             if (link_breaks(f, word, q, dx, dy, dz, theta)) {
@@ -1114,6 +1103,56 @@ __global__ void __launch_bounds__(TPB) k_break_recs(ForceArgs<R, R, R> f, StepRe
                 f.slotOtherW[sidx] = ow | SLOT_BROKEN;
                 f.state[word] = 0;
                 broke += p < q ? 1u : 0u;
+            }
+        };
+        if constexpr (sizeof(R) == 4) {
+            // Pipelined two deep: the partner word two sides ahead, the
+            // partner's coordinates and rotation bound and the side's code
+            // one side ahead, so a side waits for loads issued a side
+            // earlier.  Past the last side the loads re-read the last side
+            // (valid addresses, results unused), and a broken side's word
+            // still names a valid partner.
+            auto fetch = [&](int32_t ow, int64_t sidx, double &x, double &y, double &z, float &rot, uint16_t &c) {
+                const int32_t q = ow & SLOT_LINK_MASK;
+                x = r.rec[q].x;
+                y = r.rec[q].y;
+                z = r.rec[q].z;
+                rot = r.rec[q].rot;
+                c = ld_stream(r.slotThr + sidx);
+            };
+            int32_t owCur = cnt > 0 ? ld_stream(f.slotOther + base) : 0;
+            int32_t owNext = cnt > 1 ? ld_stream(f.slotOther + base + 64) : owCur;
+            double nx = 0, ny = 0, nz = 0;
+            float nrot = 0;
+            uint16_t ncode = 0;
+            if (cnt > 0) fetch(owCur, base, nx, ny, nz, nrot, ncode);
+            for (int32_t k = 0; k < cnt; k++) {
+                const int64_t sidx = base + (int64_t)k * 64;
+                const int32_t ow = owCur;
+                const double Qx = nx, Qy = ny, Qz = nz;
+                const float rotOther = nrot;
+                const uint16_t code = ncode;
+                const bool more = k + 1 < cnt;
+                fetch(more ? owNext : ow, more ? sidx + 64 : sidx, nx, ny, nz, nrot, ncode);
+                owCur = owNext;
+                owNext = ld_stream(f.slotOther + (k + 2 < cnt ? sidx + 128 : sidx));
+                if (ow >= 0) side(sidx, ow, Qx, Qy, Qz, rotOther, code);  // ow < 0: broken already
+            }
+        } else {
+            // as in the fused step: the partner word one side ahead, the
+            // side's record, rotation and code issued together, one wait per
+            // side (the deeper pipeline measured no faster for the FP64
+            // records)
+            int32_t owNext = cnt > 0 ? ld_stream(f.slotOther + base) : 0;
+            for (int32_t k = 0; k < cnt; k++) {
+                const int64_t sidx = base + (int64_t)k * 64;
+                const int32_t ow = owNext;
+                const int32_t q = ow & SLOT_LINK_MASK;
+                const auto Q = r.rec[q];
+                const float rotOther = (float)r.drec[(int64_t)q * 8 + 6];
+                const uint16_t code = ld_stream(r.slotThr + sidx);
+                owNext = ld_stream(f.slotOther + (k + 1 < cnt ? sidx + 64 : sidx));
+                if (ow >= 0) side(sidx, ow, Q.x, Q.y, Q.z, rotOther, code);  // ow < 0: broken already
             }
         }
     }
