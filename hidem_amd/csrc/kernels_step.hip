@@ -966,6 +966,11 @@ __device__ __forceinline__ void step_particle(const ForceArgs<R, CT, R> &f, cons
             // expcnt in 6:4 and lgkmcnt in 11:8 at their maxima (no wait).
             __builtin_amdgcn_s_waitcnt(0x0F70);
             for (int32_t k = 0; k < cnt; k++) {
+                // FP64 form: the wave issues the side's loads at raised
+                // priority, so the loads of all waves get in flight before
+                // any returns to its arithmetic (measured 2-3% faster; the
+                // FP32 form measured 1% slower with it)
+                if constexpr (sizeof(R) == 8) __builtin_amdgcn_s_setprio(1);
                 const int64_t sidx = base + (int64_t)k * 64;
                 const int32_t ow = owNext;
                 const int32_t q = ow & SLOT_LINK_MASK;
@@ -977,6 +982,7 @@ __device__ __forceinline__ void step_particle(const ForceArgs<R, CT, R> &f, cons
                 // splits the issue from the one wait
                 const uint16_t code = ld_stream(r.slotThr + sidx);
                 owNext = ld_stream(f.slotOther + (k + 1 < cnt ? sidx + 64 : sidx));
+                if constexpr (sizeof(R) == 8) __builtin_amdgcn_s_setprio(0);
                 R other[7], d[6];
                 float rotOther;
                 partner_diff<R>(r, q, Q, diff, other, d, rotOther);
