@@ -265,17 +265,18 @@ constexpr int NEIGH_BATCH = 4;
 // so their members are one contiguous run of the iteration order
 // [cellStart[lo], cellStart[hi + 1]) with lo..hi the row's targets clipped to
 // the grid: the same candidates in the same order, read as cell records.
-// The distance test is NeighAccel's (FP64, same expressions).  FILL writes
-// the first `width` entries to the wave-tiled list; every pass writes the
-// count and the skipped link partners.
+// The decisions are NeighAccel's FP64 ones: an FP32 test on squared
+// distances decides wherever its rounding bound allows, NeighAccel's FP64
+// expressions elsewhere.  FILL writes the first `width` entries to the
+// wave-tiled list; every pass writes the count and the skipped link
+// partners.  Five waves per SIMD (the software pipeline's registers fit
+// without spills).
 template <bool FILL>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(5))) k_neigh(const double4 *__restrict__ crec, const float4 *__restrict__ crecF,
-                                                const int32_t *__restrict__ cpos,
-                                                int64_t n, const int32_t *__restrict__ colour,
-                                                const int32_t *__restrict__ cellStart,
-                                                const int32_t *__restrict__ order, NeighArgs a,
-                                                int32_t *__restrict__ count, int32_t *__restrict__ linked,
-                                                int32_t *__restrict__ list) {
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(5)))
+k_neigh(const double4 *__restrict__ crec, const float4 *__restrict__ crecF, const int32_t *__restrict__ cpos,
+        int64_t n, const int32_t *__restrict__ colour, const int32_t *__restrict__ cellStart,
+        const int32_t *__restrict__ order, NeighArgs a, int32_t *__restrict__ count,
+        int32_t *__restrict__ linked, int32_t *__restrict__ list) {
     int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
     if (p >= n) return;
     const int32_t self = cpos[p];
