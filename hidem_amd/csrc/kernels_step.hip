@@ -1074,23 +1074,22 @@ __global__ void __launch_bounds__(BLOCK, step_waves<R>(OCC)) k_step(ForceArgs<R,
     }
 }
 
-// The link break that ends a run: linkUpdate of its last step
-// (HiDEMLib.h:256-269, HiDEMDefLinkBreakFunc) applied per particle from the
-// records, as the next fused step would apply it, so the SoA state need not
-// be unpacked first.  Every intact side of p takes the float prefilter and,
+// The link break that ends a run in the FP32 form: linkUpdate of its last
+// step (HiDEMLib.h:256-269, HiDEMDefLinkBreakFunc) applied per particle from
+// the records, as the next fused step would apply it, so the SoA state need
+// not be unpacked first.  Every intact side of p takes the float prefilter and,
 // when that cannot prove the link intact, the exact FP64 test; both ends
 // reach the same verdict (the pair differences are antisymmetric) and flag
 // their own slot, and the end with the smaller index counts the link.  The
 // links broken are those k_break would break.
 template <class R>
 __global__ void __launch_bounds__(TPB) k_break_recs(ForceArgs<R, R, R> f, StepRecs<R> r, uint64_t *counter) {
+    static_assert(sizeof(R) == 4, "the FP64 form's run-end break is k_break_slots");
     const int64_t p = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
     uint32_t broke = 0;
     if (p < f.n) {
         const auto P = r.rec[p];
-        float rotSelf;
-        if constexpr (sizeof(R) == 4) rotSelf = P.rot;
-        else rotSelf = (float)r.drec[p * 8 + 6];
+        const float rotSelf = P.rot;
         const int32_t cnt = (int32_t)f.linkCount[p];
         const int64_t base = (p >> 6) * (int64_t)64 * f.linkWidth + (p & 63);
         // one intact side: prefilter, then the exact test where it cannot
@@ -1111,54 +1110,92 @@ __global__ void __launch_bounds__(TPB) k_break_recs(ForceArgs<R, R, R> f, StepRe
                 broke += p < q ? 1u : 0u;
             }
         };
-        if constexpr (sizeof(R) == 4) {
-            // Pipelined two deep: the partner word two sides ahead, the
-            // partner's coordinates and rotation bound and the side's code
-            // one side ahead, so a side waits for loads issued a side
-            // earlier.  Past the last side the loads re-read the last side
-            // (valid addresses, results unused), and a broken side's word
-            // still names a valid partner.
-            auto fetch = [&](int32_t ow, int64_t sidx, double &x, double &y, double &z, float &rot, uint16_t &c) {
+        // Pipelined two deep: the partner word two sides ahead, the
+        // partner's coordinates and rotation bound and the side's code
+        // one side ahead, so a side waits for loads issued a side
+        // earlier.  Past the last side the loads re-read the last side
+        // (valid addresses, results unused), and a broken side's word
+        // still names a valid partner.
+        auto fetch = [&](int32_t ow, int64_t sidx, double &x, double &y, double &z, float &rot, uint16_t &c) {
+            const int32_t q = ow & SLOT_LINK_MASK;
+            x = r.rec[q].x;
+            y = r.rec[q].y;
+            z = r.rec[q].z;
+            rot = r.rec[q].rot;
+            c = ld_stream(r.slotThr + sidx);
+        };
+        int32_t owCur = cnt > 0 ? ld_stream(f.slotOther + base) : 0;
+        int32_t owNext = cnt > 1 ? ld_stream(f.slotOther + base + 64) : owCur;
+        double nx = 0, ny = 0, nz = 0;
+        float nrot = 0;
+        uint16_t ncode = 0;
+        if (cnt > 0) fetch(owCur, base, nx, ny, nz, nrot, ncode);
+        for (int32_t k = 0; k < cnt; k++) {
+            const int64_t sidx = base + (int64_t)k * 64;
+            const int32_t ow = owCur;
+            const double Qx = nx, Qy = ny, Qz = nz;
+            const float rotOther = nrot;
+            const uint16_t code = ncode;
+            const bool more = k + 1 < cnt;
+            fetch(more ? owNext : ow, more ? sidx + 64 : sidx, nx, ny, nz, nrot, ncode);
+            owCur = owNext;
+            owNext = ld_stream(f.slotOther + (k + 2 < cnt ? sidx + 128 : sidx));
+            if (ow >= 0) side(sidx, ow, Qx, Qy, Qz, rotOther, code);  // ow < 0: broken already
+        }
+    }
+    broke = wave_sum_u32(broke);
+    if ((threadIdx.x & 63) == 0 && broke) {
+        if (counter) atomicAdd((unsigned long long *)counter, (unsigned long long)broke);
+        if (f.brokenCount) atomicAdd((unsigned long long *)f.brokenCount, (unsigned long long)broke);
+    }
+}
+
+// The same run-end break with one thread per link slot instead of one per
+// particle: thread s is tiled slot s = (t * linkWidth + k) * 64 + lane, side k
+// of particle t * 64 + lane, so a wave holds one side of 64 consecutive
+// particles (its slot words, codes and own records coalesced) and no thread
+// waits on more than one side.  Same tests, flags and counting as
+// k_break_recs; the FP64 form's run-end break (0.68 against 0.85 ms for the
+// per-particle walk on the 8M slab, which stays the faster for the FP32
+// records: 0.62 against 0.81 ms).
+template <class R>
+__global__ void __launch_bounds__(TPB) k_break_slots(ForceArgs<R, R, R> f, StepRecs<R> r, int64_t nSlots,
+                                                     uint64_t *counter) {
+    const int64_t s = xcd_block_index(gridDim.x) * TPB + threadIdx.x;
+    uint32_t broke = 0;
+    if (s < nSlots) {
+        const int64_t w64 = (int64_t)f.linkWidth * 64;
+        const int64_t t = s / w64;
+        const int32_t k = (int32_t)((s - t * w64) >> 6);
+        const int64_t p = t * 64 + (s & 63);
+        if (p < f.n && k < (int32_t)f.linkCount[p]) {
+            const int32_t ow = ld_stream(f.slotOther + s);
+            if (ow >= 0) {  // ow < 0: broken already
                 const int32_t q = ow & SLOT_LINK_MASK;
-                x = r.rec[q].x;
-                y = r.rec[q].y;
-                z = r.rec[q].z;
-                rot = r.rec[q].rot;
-                c = ld_stream(r.slotThr + sidx);
-            };
-            int32_t owCur = cnt > 0 ? ld_stream(f.slotOther + base) : 0;
-            int32_t owNext = cnt > 1 ? ld_stream(f.slotOther + base + 64) : owCur;
-            double nx = 0, ny = 0, nz = 0;
-            float nrot = 0;
-            uint16_t ncode = 0;
-            if (cnt > 0) fetch(owCur, base, nx, ny, nz, nrot, ncode);
-            for (int32_t k = 0; k < cnt; k++) {
-                const int64_t sidx = base + (int64_t)k * 64;
-                const int32_t ow = owCur;
-                const double Qx = nx, Qy = ny, Qz = nz;
-                const float rotOther = nrot;
-                const uint16_t code = ncode;
-                const bool more = k + 1 < cnt;
-                fetch(more ? owNext : ow, more ? sidx + 64 : sidx, nx, ny, nz, nrot, ncode);
-                owCur = owNext;
-                owNext = ld_stream(f.slotOther + (k + 2 < cnt ? sidx + 128 : sidx));
-                if (ow >= 0) side(sidx, ow, Qx, Qy, Qz, rotOther, code);  // ow < 0: broken already
-            }
-        } else {
-            // as in the fused step: the partner word one side ahead, the
-            // side's record, rotation and code issued together, one wait per
-            // side (the deeper pipeline measured no faster for the FP64
-            // records)
-            int32_t owNext = cnt > 0 ? ld_stream(f.slotOther + base) : 0;
-            for (int32_t k = 0; k < cnt; k++) {
-                const int64_t sidx = base + (int64_t)k * 64;
-                const int32_t ow = owNext;
-                const int32_t q = ow & SLOT_LINK_MASK;
+                const auto P = r.rec[p];
                 const auto Q = r.rec[q];
-                const float rotOther = (float)r.drec[(int64_t)q * 8 + 6];
-                const uint16_t code = ld_stream(r.slotThr + sidx);
-                owNext = ld_stream(f.slotOther + (k + 1 < cnt ? sidx + 64 : sidx));
-                if (ow >= 0) side(sidx, ow, Q.x, Q.y, Q.z, rotOther, code);  // ow < 0: broken already
+                const uint16_t code = ld_stream(r.slotThr + s);
+                float rotSelf, rotOther;
+                if constexpr (sizeof(R) == 4) {
+                    rotSelf = P.rot;
+                    rotOther = Q.rot;
+                } else {
+                    rotSelf = (float)r.drec[p * 8 + 6];
+                    rotOther = (float)r.drec[(int64_t)q * 8 + 6];
+                }
+                const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+                const float thr = (float)code * r.thrScale;
+                const float T = r.perLinkMaxLoad ? thr : thr * (1.0f + r.scalarMaxLoad);
+                if (!(r.prefilter != 0 && certainly_intact(T, r.bendBreak, r.l0Max, dx, dy, dz, rotSelf + rotOther))) {
+                    const int32_t word = f.slotLink[s];
+                    const double theta[3] = {f.tx[p], f.ty[p], f.tz[p]};
+                    if (link_breaks(f, word, q, dx, dy, dz, theta)) {
+                        f.slotLinkW[s] = word | SLOT_BROKEN;
+                        f.slotOtherW[s] = ow | SLOT_BROKEN;
+                        f.state[word] = 0;
+                        broke = p < q ? 1u : 0u;
+                    }
+                }
             }
         }
     }
@@ -1813,7 +1850,13 @@ hipError_t launch_break_records(Context &c, const hdm_break_params &bp, uint64_t
                                             float>::type;
         f.bendBreak = bp.bendBreak;
         f.scalarMaxLoad = bp.maxLoad;
-        hipLaunchKernelGGL((k_break_recs<R>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, f, r, counter);
+        if constexpr (sizeof(R) == 8) {
+            const int64_t nSlots = (c.n + 63) / 64 * (int64_t)f.linkWidth * 64;
+            hipLaunchKernelGGL((k_break_slots<R>), dim3((unsigned)((nSlots + TPB - 1) / TPB)), dim3(TPB), 0,
+                               c.stream, f, r, nSlots, counter);
+        } else {
+            hipLaunchKernelGGL((k_break_recs<R>), dim3(blocks_for(c.n)), dim3(TPB), 0, c.stream, f, r, counter);
+        }
     };
     if (c.prec == HDM_PREC_FP64) go(force_args<double, double, double>(c, fp), step_recs<double>(c, c.recTame, &bp));
     else go(force_args<float, float, float>(c, fp), step_recs<float>(c, c.recTame, &bp));
